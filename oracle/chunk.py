@@ -1,0 +1,360 @@
+"""The reference chunk estimator, restated on the CPU (numpy + LAPACK).
+
+Follows core/chunk_estimate.cpp step by step:
+
+* construction: QSOFile subset copy with masked-boundary cut and median dv
+  (io/qso_file.cpp:61-89,130-157,300-329), short-chunk test, table lookup
+  (core/sq_table.cpp:388-397), KBIN_UPP (core/chunk_estimate.cpp:32-39,216),
+  redshift bins and the triangular extension (:218-257), stored Q list
+  (:150-157);
+* ``one_iteration``: v/z matrices (:384-398), Q_k from the 1D table
+  (:429-480) copied into the other z bins (:871-875), S_fid from the 2D table
+  (:400-426), redshift weights (:483-536), C = S + sum theta_k Q_k, the LU
+  inverse of N^-1 C + I and the column scaling (:668-719), SVD + sequential
+  Sherman-Morrison continuum marginalisation (:576-665), y = C^-1 delta by
+  dsymv on the upper triangle, d_k, W_k = C^-1 Q_k by dgemm, b_k, t_k and the
+  pairwise Fisher ddot loop (:738-835), accumulation (:893-903).
+
+LAPACK/BLAS: dgetrf+dgetri (LAPACKE_InvertMatrixLU,
+mathtools/matrix_helper.cpp:184-204) and dgesvd with jobu='O'
+(mathtools/matrix_helper.cpp:437-447) come from scipy's LAPACK; dgemm from
+numpy.  Row-major CBLAS "Upper" calls are expressed as products with the
+symmetric matrix rebuilt from the upper triangle.
+
+Pins (tests/test_oracle_chunk.py): the golden 488x488 S_fid and Q_0 matrices
+(all entries), the golden 10x10 Fisher matrix, the golden d/b/t sums and
+theta/sigma of tests/truth/test_it1_*; LU inverse against
+truth_inverse_matrix.txt; the marginalised inverse against the block-Woodbury
+closed form and C_marg^-1 v = 0; the special case S=0, theta=0, no
+marginalisation where F = Tr(N^-1 Q N^-1 Q'), b = Tr(N^-1 Q), t = 0; and
+brute force in extended precision for N <= 16.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+from scipy.linalg import lapack
+
+from .bins import (Bins, LYA_REST, MY_PI, SPEED_OF_LIGHT, DOUBLE_EPSILON,
+                   TRIANGULAR, redshift_weight)
+from .interp import evaluate_1d, evaluate_2d
+
+
+class ChunkError(RuntimeError):
+    """Construction failure; the caller skips the chunk
+    (core/one_qso_estimate.cpp:115-125)."""
+
+
+class LapackError(RuntimeError):
+    """LU failure inside an iteration; accumulation is skipped but the chunk
+    is kept (core/chunk_estimate.cpp:905-913)."""
+
+
+@dataclass
+class TableSet:
+    """SQLookupTable's interpolation grids for one (R, dv) pair
+    (core/sq_table.cpp:148-175,325-368)."""
+    R: int
+    dv: float
+    nv: int
+    v_length: float
+    nz: int
+    deriv: np.ndarray   # (nk, nv)
+    signal: np.ndarray  # (nv, nz), flat index iz + nz*iv
+
+    def grids(self, bins: Bins):
+        v1 = 0.0
+        dv = 0.0 if self.nv == 1 else self.v_length / (self.nv - 1.0)
+        z1 = bins.zbin_centers[0] - bins.z_bin_width
+        zlen = bins.z_bin_width * (bins.nz + 1)
+        dz = 0.0 if self.nz == 1 else zlen / (self.nz - 1.0)
+        return v1, dv, z1, dz
+
+
+def median_sorted(a: np.ndarray) -> float:
+    # stats::medianOfSortedArray, mathtools/stats.cpp:8-16
+    n = a.size
+    if n % 2 == 0:
+        return (a[n // 2 - 1] + a[n // 2]) / 2
+    return a[n // 2]
+
+
+def find_spec_res_index(tables: List[TableSet], R: int, dv: float) -> int:
+    # core/sq_table.cpp:388-397
+    # C round(): halves away from zero
+    rounded = np.floor(dv * 10 + 0.5) / 10 if dv >= 0 else -np.floor(-dv * 10 + 0.5) / 10
+    for r, t in enumerate(tables):
+        if t.R == R and abs(rounded - t.dv) < 0.01:
+            return r
+    return -1
+
+
+class Chunk:
+    """Mirror of class Chunk (core/chunk_estimate.hpp:46-119)."""
+
+    def __init__(self, bins: Bins, tables: List[TableSet], wave, delta, ivar,
+                 R_fwhm: int, i1: int = 0, i2: Optional[int] = None):
+        wave = np.asarray(wave, dtype=np.float64)
+        i2 = wave.size if i2 is None else i2
+        w = wave[i1:i2].copy()
+        d = np.asarray(delta, dtype=np.float64)[i1:i2].copy()
+        iv = np.asarray(ivar, dtype=np.float64)[i1:i2].copy()
+        # QSOFile::_cutMaskedBoundary, io/qso_file.cpp:139-157
+        nz_idx = np.flatnonzero(iv != 0)
+        if nz_idx.size == 0:
+            raise ChunkError("Empty spectrum when masked pixels at boundary are removed!")
+        a, b = nz_idx[0], nz_idx[-1] + 1
+        w, d, iv = w[a:b], d[a:b], iv[a:b]
+        self.bins = bins
+        self.N = w.size
+        self.real_size = int(np.count_nonzero(iv))
+        # recalcDvDLam, io/qso_file.cpp:325-329 (median of the sorted array)
+        if self.N > 1:
+            self.dv_kms = median_sorted(np.sort(np.log(w[1:] / w[:-1]) * SPEED_OF_LIGHT))
+        else:
+            self.dv_kms = -1.0
+        self.R_fwhm = int(R_fwhm)
+        # _copyQSOFile short-chunk test, core/chunk_estimate.cpp:172-187
+        lenv = SPEED_OF_LIGHT * np.log(w[-1] / w[0]) if self.N > 0 else 0.0
+        minnopix = int(lenv / bins.max_pixel_length_v)
+        if self.real_size < minnopix or lenv < bins.min_forest_length_v:
+            raise ChunkError("Short chunk")
+        self.res_index = find_spec_res_index(tables, self.R_fwhm, self.dv_kms)
+        if self.res_index == -1:
+            raise ChunkError(f"SpecRes R: {self.R_fwhm}, dv: {self.dv_kms} not found in tables!")
+        self.table = tables[self.res_index]
+        # readMinMaxMedRedshift, io/qso_file.cpp:280-300
+        self.lower_z = w[0] / LYA_REST - 1
+        self.upper_z = w[-1] / LYA_REST - 1
+        self.median_z = median_sorted(w) / LYA_REST - 1
+        self.kbin_upp = bins.max_k_index(MY_PI / self.dv_kms)
+        # _findRedshiftBin, core/chunk_estimate.cpp:225-236
+        self.zbin = bins.find_redshift_bin(self.median_z)
+        self.zbin_low = bins.find_redshift_bin(self.lower_z)
+        self.zbin_upp = bins.find_redshift_bin(self.upper_z)
+        self.wave = w / LYA_REST
+        self.delta = d
+        self.ivar = iv
+        self._set_nq_and_fisher_index()
+        nk = bins.nk
+        self.stored = [i for i in range(self.n_q) if (i + self.fisher_index_start) % nk < self.kbin_upp]
+        self.dbt = np.zeros((3, self.n_q))
+        self.fisher = np.zeros((self.n_q, self.n_q))
+
+    def _set_nq_and_fisher_index(self):
+        # core/chunk_estimate.cpp:240-257
+        b = self.bins
+        nq = self.zbin_upp - self.zbin_low + 1
+        fis = b.nk * self.zbin_low
+        if b.method == TRIANGULAR:
+            if self.lower_z < b.zbin_centers[self.zbin_low] and self.zbin_low != 0:
+                nq += 1
+                fis -= b.nk
+            if b.zbin_centers[self.zbin_upp] < self.upper_z and self.zbin_upp < b.nz - 1:
+                nq += 1
+        self.n_q = nq * b.nk
+        self.fisher_index_start = fis
+
+    # -- matrices -----------------------------------------------------------
+    def vz_matrices(self):
+        """_setVZMatrices, core/chunk_estimate.cpp:384-398, mirrored."""
+        li = self.wave[:, None]
+        lj = self.wave[None, :]
+        iu = np.triu_indices(self.N)
+        v = np.zeros((self.N, self.N))
+        z = np.zeros((self.N, self.N))
+        v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
+        z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
+        return v, z
+
+    def _mirror(self, m):
+        iu = np.triu_indices(self.N, 1)
+        m.T[iu] = m[iu]
+        return m
+
+    def signal_matrix(self, v, z):
+        """_setFiducialSignalMatrix, core/chunk_estimate.cpp:400-426."""
+        v1, dv, z1, dz = self.table.grids(self.bins)
+        iu = np.triu_indices(self.N)
+        s = np.zeros((self.N, self.N))
+        s[iu] = evaluate_2d(z[iu], v[iu], z1, dz, v1, dv, self.table.signal.reshape(-1),
+                            self.table.nz, self.table.nv)
+        return self._mirror(s)
+
+    def q_base(self, v, kn):
+        """_setQiMatrix for a first-z-bin k band, core/chunk_estimate.cpp:429-480."""
+        v1, dv, _, _ = self.table.grids(self.bins)
+        iu = np.triu_indices(self.N)
+        q = np.zeros((self.N, self.N))
+        q[iu] = evaluate_1d(v[iu], v1, dv, self.table.deriv[kn])
+        return self._mirror(q)
+
+    def redshift_weights(self, z, zm):
+        """The weight matrix built row-wise in _applyRedshiftInterp
+        (core/chunk_estimate.cpp:483-536): for j >= i it is w_zm(z_ij)."""
+        iu = np.triu_indices(self.N)
+        t = np.zeros((self.N, self.N))
+        t[iu] = redshift_weight(self.bins, z[iu], zm)
+        return self._mirror(t)
+
+    def q_matrices(self, v, z):
+        """Stored Q_k after copies and z weights, in stored order."""
+        nk = self.bins.nk
+        base = {}
+        for kn in range(self.kbin_upp):
+            base[kn] = self.q_base(v, kn)
+        nzq = len(self.stored) // self.kbin_upp if self.kbin_upp else 0
+        out = []
+        for iz in range(nzq):
+            zm = self.fisher_index_start // nk + iz
+            w = self.redshift_weights(z, zm)
+            for s in self.stored[iz * self.kbin_upp:(iz + 1) * self.kbin_upp]:
+                out.append(base[s % nk] * w)
+        return out
+
+    # -- the iteration -------------------------------------------------------
+    def inverse_covariance(self, sfid, qs, theta, mode="lu"):
+        """setInvertCovarianceMatrix, core/chunk_estimate.cpp:668-719."""
+        N = self.N
+        c = np.zeros((N, N)) if self.bins.turn_off_sfid else sfid.copy()
+        for i_kz, q in zip(self.stored, qs):
+            c += theta[self.fisher_index_start + i_kz] * q
+        iv = self.ivar
+        if mode == "lu":
+            m = c * iv[:, None]
+            m[np.diag_indices(N)] += 1.0
+            lu, piv, info = lapack.dgetrf(m)
+            if info != 0:
+                raise LapackError("ERROR in LU decomposition.")
+            inv, info = lapack.dgetri(lu, piv)
+            if info != 0:
+                raise LapackError("ERROR in LU invert.")
+            cinv = inv * iv[None, :]
+        else:
+            # SPD form N^{-1/2} (I + N^{-1/2} S N^{-1/2})^{-1} N^{-1/2}: the
+            # same matrix, used only to measure the rounding floor
+            ds = np.sqrt(iv)
+            a = c * ds[:, None] * ds[None, :]
+            a[np.diag_indices(N)] += 1.0
+            l, info = lapack.dpotrf(a, lower=1)
+            if info != 0:
+                raise LapackError("ERROR in Cholesky decomposition.")
+            ainv, info = lapack.dpotri(l, lower=1)
+            ainv = np.tril(ainv) + np.tril(ainv, -1).T
+            cinv = ainv * ds[:, None] * ds[None, :]
+        if self.bins.cont_nvecs > 0:
+            cinv = self.add_marginalizations(cinv)
+        return cinv
+
+    def templates(self):
+        """_addMarginalizations template rows, core/chunk_estimate.cpp:576-594,
+        609-623 (wave in units of lambda_Lya, masked pixels included)."""
+        b, N, w = self.bins, self.N, self.wave
+        rows = [np.full(N, 1.0 / np.sqrt(N))]
+        for cmo in range(1, b.cont_logl_order + 1):
+            t = np.power(np.log(w), cmo)
+            rows.append(t / np.linalg.norm(t))
+        for cmo in range(1, b.cont_lam_order + 1):
+            t = np.power(w, cmo)
+            rows.append(t / np.linalg.norm(t))
+        return np.array(rows)
+
+    def add_marginalizations(self, cinv):
+        """SVD (jobu='O') + sequential Sherman-Morrison,
+        core/chunk_estimate.cpp:596-665."""
+        vt = self.templates()                      # (nvecs, N) == col-major N x nvecs
+        u, s, _vt, info = lapack.dgesvd(vt.T.copy(order="F"), compute_uv=1, full_matrices=0)
+        if info != 0:
+            raise LapackError("ERROR in SVD.")
+        nuse = len(s)
+        while nuse > 0:
+            if s[nuse - 1] / s[0] > DOUBLE_EPSILON:
+                break
+            nuse -= 1
+        cinv = cinv.copy()
+        for i in range(nuse):
+            v = u[:, i]
+            y = _symv_upper(cinv, v)
+            norm = -1.0 / np.dot(v, y)
+            cinv += norm * np.outer(y, y)
+        return cinv
+
+    def ps_before_fisher(self, cinv, sfid, qs):
+        """computePSbeforeFvector, core/chunk_estimate.cpp:738-835."""
+        N = self.N
+        dk, nk_, tk = self.dbt
+        y = _symv_upper(cinv, self.delta)
+        ws = []
+        for i_kz, q in zip(self.stored, qs):
+            dk[i_kz] = np.dot(y, _symv_upper(q, y))
+            ws.append(cinv @ q)
+        iv = self.ivar
+        with np.errstate(divide="ignore"):
+            inv_iv = np.where(iv != 0, 1.0 / np.where(iv != 0, iv, 1.0), 0.0)
+        ncinv = cinv * inv_iv[:, None]
+        for i_kz, wmat in zip(self.stored, ws):
+            nk_[i_kz] = np.dot(wmat.ravel(), ncinv.ravel())
+        if not self.bins.turn_off_sfid:
+            scinv = sfid @ cinv
+            for i_kz, wmat in zip(self.stored, ws):
+                tk[i_kz] = np.dot(wmat.ravel(), scinv.ravel())
+        if self.bins.use_precomputed_fisher:
+            return ws
+        for a, (i_kz, wa) in enumerate(zip(self.stored, ws)):
+            wt = wa.T.ravel()
+            for j_kz, wb in zip(self.stored[a:], ws[a:]):
+                self.fisher[i_kz, j_kz] = np.dot(wt, wb.ravel())
+        return ws
+
+    def one_iteration(self, theta, fisher_sum, dbt_sum, mode="lu"):
+        """Chunk::oneQSOiteration, core/chunk_estimate.cpp:837-914."""
+        v, z = self.vz_matrices()
+        qs = self.q_matrices(v, z)
+        sfid = None if self.bins.turn_off_sfid else self.signal_matrix(v, z)
+        try:
+            cinv = self.inverse_covariance(sfid, qs, theta, mode)
+            self.ps_before_fisher(cinv, sfid, qs)
+        except LapackError:
+            return False
+        accumulate(self, fisher_sum, dbt_sum)
+        return True
+
+
+def _symv_upper(a, x):
+    """cblas_dsymv(RowMajor, Upper): only a[i, j] with j >= i is read."""
+    au = np.triu(a)
+    return au @ x + np.triu(a, 1).T @ x
+
+
+def accumulate(ch: Chunk, fisher_sum: np.ndarray, dbt_sum: np.ndarray) -> None:
+    """core/chunk_estimate.cpp:893-903: upper triangle at offset
+    (T+1)*fisher_index_start, and d/b/t at fisher_index_start."""
+    fis, nq = ch.fisher_index_start, ch.n_q
+    iu = np.triu_indices(nq)
+    blk = fisher_sum[fis:fis + nq, fis:fis + nq]
+    blk[iu] += ch.fisher[iu]
+    dbt_sum[:, fis:fis + nq] += ch.dbt
+
+
+def decide_indices(wave: np.ndarray, bins: Bins) -> List[int]:
+    """OneQSOEstimate::decideIndices, core/one_qso_estimate.cpp:17-50."""
+    size = wave.size
+    vel = SPEED_OF_LIGHT * np.log(wave / wave[0])
+    vmax = vel[-1]
+    nchunks = 1
+    if bins.number_of_chunks > 1:
+        nchunks += int(vmax / bins.max_forest_length_per_chunk)
+        vmax = min(bins.max_forest_length_per_chunk, vmax / nchunks)
+    idx = [0]
+    lo = 0
+    vel = vel.copy()
+    for _ in range(nchunks):
+        jj = lo + int(np.searchsorted(vel[lo:], vmax, side="right"))
+        idx.append(jj)
+        if jj == size:
+            break
+        lo = jj
+        vel[lo:] -= vel[lo]
+    return idx

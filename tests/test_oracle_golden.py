@@ -1,0 +1,205 @@
+"""Pins the CPU oracle against the reference's own golden vectors and against
+closed forms the mathematics fixes."""
+import numpy as np
+import pytest
+
+import goldencase as gc
+from oracle.bins import Bins, redshift_weight
+from oracle.chunk import Chunk, TableSet, LapackError
+from oracle.driver import power_estimates
+from oracle import interp
+from oracle.tables import derivative_table_qawo, signal_table_fft
+
+
+def _golden_chunk(**over):
+    kw = dict(gc.BINS_KW)
+    kw.update(over)
+    b = Bins(**kw)
+    t = gc.table()
+    ts = [TableSet(70000, 20.0, t.nvpoints, t.v_length, t.nzpoints, t.deriv, t.signal)]
+    sp = gc.spectrum()
+    return b, Chunk(b, ts, sp.wave, sp.flux, sp.ivar, sp.R_fwhm)
+
+
+def test_chunk_metadata():
+    b, ch = _golden_chunk()
+    assert ch.N == 488 and ch.n_q == 10 and ch.fisher_index_start == 0
+    assert ch.kbin_upp == 5 and ch.stored == list(range(10))
+
+
+def test_signal_and_q0_matrices_all_entries():
+    # tests/testSQCMatrices.cpp:38-95 compares row 0 only; here all 488^2
+    b, ch = _golden_chunk()
+    v, z = ch.vz_matrices()
+    s_gold, q0_gold = gc.chunk_matrices()
+    assert gc.all_close(ch.signal_matrix(v, z), s_gold)
+    assert gc.all_close(ch.q_matrices(v, z)[0], q0_gold)
+
+
+def test_end_to_end_fisher_theta_sigma_dbt():
+    b, ch = _golden_chunk()
+    T = b.total_kz_bins
+    F, dbt = np.zeros((T, T)), np.zeros((3, T))
+    assert ch.one_iteration(np.zeros(T), F, dbt)
+    res = power_estimates(F, dbt)
+    rows = gc.detailed_rows()
+    assert gc.all_close(res["fisher"], gc.golden_fisher())
+    assert gc.all_close(res["inverse"], gc.golden_inverse_fisher())
+    assert gc.all_close(rows[:, 0], b.zbin_centers[np.arange(T) // b.nk])
+    assert gc.all_close(rows[:, 1], b.kband_edges[np.arange(T) % b.nk])
+    assert gc.all_close(rows[:, 3], b.kband_centers[np.arange(T) % b.nk])
+    assert gc.all_close(rows[:, 5], res["theta"])
+    assert gc.all_close(rows[:, 7], res["sigma"])
+    for i in range(3):
+        assert gc.all_close(rows[:, 8 + i], res["weighted"][i])
+        assert gc.all_close(rows[:, 11 + i], dbt[i])
+
+
+def test_tables_against_golden_fits():
+    b = Bins(**gc.BINS_KW)
+    t = gc.table()
+    s = signal_table_fft(b, 70000, 20.0, t.nvpoints, t.v_length, t.nzpoints, gc.PD13)
+    assert gc.all_close(s, t.signal)
+    q = derivative_table_qawo(b, 70000, 20.0, t.nvpoints, t.v_length)
+    assert gc.all_close(q, t.deriv)
+
+
+def test_lu_inverse_against_truth(golden_dir):
+    from scipy.linalg import lapack
+    d = np.load(f"{golden_dir}/lu_inverse.npz")
+    lu, piv, info = lapack.dgetrf(d["matrix"])
+    inv, info2 = lapack.dgetri(lu, piv)
+    assert info == 0 and info2 == 0
+    assert gc.all_close(inv, d["inverse"])
+
+
+def test_svd_jobu_o_against_truth(golden_dir):
+    # tests/cblas_tests.cpp:285-302: col-major 6x5, jobu='O'
+    from scipy.linalg import lapack
+    d = np.load(f"{golden_dir}/cblas_inline.npz")
+    a = d["matrix_for_SVD_A"].reshape(5, 6).T          # col-major m=6, n=5
+    u, s, vt, info = lapack.dgesvd(a.copy(order="F"), compute_uv=1, full_matrices=0)
+    assert info == 0
+    got = u.T.reshape(-1)                              # back to the col-major buffer
+    assert gc.all_close(got, d["truth_SVD_A"])
+
+
+def test_redshift_weights_partition_of_unity():
+    b = Bins(**dict(gc.BINS_KW, nz=5))
+    z = np.linspace(b.zbin_centers[0] - 0.05, b.zbin_centers[-1] + 0.05, 2001)
+    w = sum(redshift_weight(b, z, m) for m in range(b.nz))
+    assert np.allclose(w, 1.0, rtol=0, atol=1e-12)
+    zc, dz = b.zbin_centers[2], b.z_bin_width
+    wi = redshift_weight(b, np.array([zc - dz, zc - dz / 2, zc, zc + dz / 4, zc + 1.01 * dz]), 2)
+    assert np.allclose(wi, [0.0, 0.5, 1.0, 0.75, 0.0], atol=1e-12)
+    bt = Bins(**dict(gc.BINS_KW, nz=3, method=0))
+    wt = redshift_weight(bt, np.array([1.999, 2.099, 2.101]), 1)
+    assert np.array_equal(wt, [1.0, 1.0, 0.0])
+
+
+def test_interp_closed_forms():
+    y = np.array([1.0, 3.0, 2.0, 5.0])
+    assert np.allclose(interp.evaluate_1d([0, 1, 2, 3], 0.0, 1.0, y), y)
+    assert np.isclose(interp.evaluate_1d(0.25, 0.0, 1.0, y), 1.5)
+    assert np.isclose(interp.evaluate_1d(4.0, 0.0, 1.0, y), 8.0)     # extrapolates
+    assert np.isclose(interp.evaluate_1d(-1.0, 0.0, 1.0, y), -1.0)
+    zz = np.arange(12.0) ** 2                                          # nx=4, ny=3
+    f = interp.evaluate_2d(1.5, 0.5, 0.0, 1.0, 0.0, 1.0, zz, 4, 3)
+    g = 0.25 * (zz[1] + zz[2] + zz[5] + zz[6])
+    assert np.isclose(f, g)
+
+
+def test_special_case_noise_only_closed_form():
+    """S off, theta = 0, no marginalisation: C = N, so
+    F_kk' = Tr(N^-1 Q_k N^-1 Q_k'), b_k = sum_i ivar_i Q_k,ii, t_k = 0."""
+    b, ch = _golden_chunk(turn_off_sfid=True)
+    T = b.total_kz_bins
+    F, dbt = np.zeros((T, T)), np.zeros((3, T))
+    assert ch.one_iteration(np.zeros(T), F, dbt)
+    v, z = ch.vz_matrices()
+    qs = ch.q_matrices(v, z)
+    iv = ch.ivar
+    for a in range(len(qs)):
+        wa = iv[:, None] * qs[a]
+        assert np.isclose(dbt[1, a], np.sum(iv * np.diag(qs[a])), rtol=1e-12)
+        for c in range(a, len(qs)):
+            wc = iv[:, None] * qs[c]
+            assert np.isclose(F[a, c], np.sum(wa * wc.T), rtol=1e-11)
+    assert np.all(dbt[2] == 0)
+
+
+def test_marginalisation_invariants():
+    """C_marg^-1 V = 0 and equality with C^-1 - C^-1 V (V^T C^-1 V)^-1 V^T C^-1."""
+    b, ch = _golden_chunk(cont_logl_order=2, cont_lam_order=1)
+    v, z = ch.vz_matrices()
+    s = ch.signal_matrix(v, z)
+    qs = ch.q_matrices(v, z)
+    theta = np.zeros(b.total_kz_bins)
+    b0 = Bins(**dict(gc.BINS_KW))
+    ch.bins = b0
+    cinv = ch.inverse_covariance(s, qs, theta)
+    ch.bins = b
+    cm = ch.add_marginalizations(cinv)
+    V = ch.templates().T
+    assert np.max(np.abs(cm @ V)) < 1e-10 * np.max(np.abs(cinv @ V))
+    # the projector onto the templates' span, written with an orthonormal
+    # basis of that span (the 4 templates here are nearly degenerate, which
+    # makes V^T C^-1 V itself too ill-conditioned to solve with directly)
+    U, _ = np.linalg.qr(V)
+    x = cinv @ U
+    wood = cinv - x @ np.linalg.solve(U.T @ x, x.T)
+    assert np.max(np.abs(cm - wood)) < 1e-9 * np.max(np.abs(cinv))
+
+
+def test_lu_vs_cholesky_floor():
+    """The SPD (Cholesky) form of the same inverse agrees with the LU form to
+    the cond*eps floor; this is the floor the GPU FP64 path is held to."""
+    b, ch = _golden_chunk(cont_logl_order=1, cont_lam_order=-1)
+    T = b.total_kz_bins
+    F1, d1 = np.zeros((T, T)), np.zeros((3, T))
+    F2, d2 = np.zeros((T, T)), np.zeros((3, T))
+    ch.one_iteration(np.zeros(T), F1, d1, mode="lu")
+    ch.one_iteration(np.zeros(T), F2, d2, mode="cholesky")
+    assert np.max(np.abs(F1 - F2) / np.abs(F1).max()) < 1e-12
+    assert np.max(np.abs(d1 - d2) / np.abs(d1).max(axis=1, keepdims=True)) < 1e-11
+
+
+def _gauss_jordan_longdouble(a):
+    n = a.shape[0]
+    m = np.concatenate([a.astype(np.longdouble), np.eye(n, dtype=np.longdouble)], axis=1)
+    for c in range(n):
+        p = c + int(np.argmax(np.abs(m[c:, c])))
+        m[[c, p]] = m[[p, c]]
+        m[c] /= m[c, c]
+        for r in range(n):
+            if r != c:
+                m[r] -= m[r, c] * m[c]
+    return m[:, n:]
+
+
+def test_brute_force_tiny_chunk_long_double():
+    """N=12 pixels: the oracle's LU-based C^-1 and F against a long-double
+    Gauss-Jordan inverse and the trace definition."""
+    b = Bins(**dict(gc.BINS_KW, max_forest_length_v=400.0, max_pixel_length_v=1e9))
+    t = gc.table()
+    ts = [TableSet(70000, 20.0, t.nvpoints, t.v_length, t.nzpoints, t.deriv, t.signal)]
+    sp = gc.spectrum()
+    ch = Chunk(b, ts, sp.wave, sp.flux, sp.ivar, sp.R_fwhm, 100, 112)
+    assert ch.N == 12
+    v, z = ch.vz_matrices()
+    s = ch.signal_matrix(v, z)
+    qs = ch.q_matrices(v, z)
+    theta = np.zeros(b.total_kz_bins)
+    cinv = ch.inverse_covariance(s, qs, theta)
+    c = s.astype(np.longdouble) + np.diag(1.0 / ch.ivar.astype(np.longdouble))
+    exact = _gauss_jordan_longdouble(c)
+    assert np.max(np.abs(cinv - exact.astype(np.float64))) < 1e-12 * np.max(np.abs(cinv))
+    T = b.total_kz_bins
+    F, dbt = np.zeros((T, T)), np.zeros((3, T))
+    ch.one_iteration(theta, F, dbt)
+    for a in range(len(qs)):
+        for cc in range(a, len(qs)):
+            wa = exact @ qs[a].astype(np.longdouble)
+            wc = exact @ qs[cc].astype(np.longdouble)
+            ref = float(np.trace(wa @ wc))
+            assert abs(F[ch.stored[a], ch.stored[cc]] - ref) <= 1e-11 * abs(F).max()

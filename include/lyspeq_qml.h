@@ -1,0 +1,167 @@
+/*
+ * lyspeq_qml.h -- C ABI of the MI355X per-chunk quadratic maximum-likelihood
+ * (QML) engine.
+ *
+ * The engine replaces the body of lyspeq's Chunk / OneQSOEstimate
+ * (core/chunk_estimate.hpp:46-119, core/one_qso_estimate.hpp:16-61): per
+ * chunk it assembles the covariance from the S/Q lookup tables, factorises
+ * it, applies continuum marginalisation and computes d_k, b_k, t_k and the
+ * unhalved per-chunk Fisher matrix F_kk' = Tr(C^-1 Q_k C^-1 Q_k'), then adds
+ * them into the driver's global arrays exactly as Chunk::oneQSOiteration
+ * does (core/chunk_estimate.cpp:893-903).  Chunks are deferred: a driver
+ * registers all of them once, then calls qml_iterate() once per
+ * Newton-Raphson iteration where OneDQuadraticPowerEstimate::iterate runs
+ * its chunk loop (core/quadratic_estimate.cpp:437-447), i.e. before the
+ * 0.5 scaling at core/quadratic_estimate.cpp:456.
+ *
+ * Conventions: plain pointers and sizes, row-major doubles, no exceptions
+ * across the boundary.  Every call returns QML_OK (0) or a negative status;
+ * qml_last_error() returns the message of the calling thread's last failure.
+ * One context is driven by one host thread.
+ */
+#ifndef LYSPEQ_QML_H
+#define LYSPEQ_QML_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define QML_OK 0
+#define QML_ERR_ARG -1          /* invalid argument or call order */
+#define QML_ERR_HIP -2          /* HIP runtime failure or no device */
+#define QML_ERR_SHORT_CHUNK -3  /* Chunk ctor's short-chunk test failed
+                                   (core/chunk_estimate.cpp:172-187) */
+#define QML_ERR_NO_TABLE -4     /* (R, dv) not among the tables
+                                   (core/chunk_estimate.cpp:198-206) */
+#define QML_ERR_EMPTY -5        /* all pixels masked (io/qso_file.cpp:150-153) */
+#define QML_ERR_MEMORY -6       /* device memory budget too small for a chunk */
+#define QML_ERR_LIMIT -7        /* a size beyond this build's limits */
+
+/* Per-chunk status after an iteration (qml_get_chunk_result). */
+#define QML_CHUNK_OK 0
+#define QML_CHUNK_NOT_POSITIVE 1  /* factorisation failed: accumulation
+                                     skipped, chunk kept, as for a LAPACK
+                                     failure at core/chunk_estimate.cpp:905-913 */
+
+/* Arithmetic of the dense products (W_k, Fisher, bias contractions). The
+ * factorisation, the data-vector solve, d_k and all accumulation are FP64
+ * in every mode. */
+#define QML_PREC_FP64 0
+#define QML_PREC_FP32 1    /* FP32 MFMA, FP32 accumulation */
+#define QML_PREC_BF16X3 2  /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulation */
+
+typedef struct qml_ctx qml_ctx;
+
+typedef struct {
+    int device;            /* HIP device ordinal */
+    int precision;         /* QML_PREC_* */
+    double scratch_gb;     /* device scratch budget for one batch of chunks; 0 = default */
+    int max_batch_chunks;  /* cap on chunks per batch; 0 = default */
+} qml_opts;
+
+/* bins:: and the specifics:: keys the chunk path reads
+ * (core/global_numbers.hpp:17-22,55-213, core/global_numbers.cpp:78-168). */
+typedef struct {
+    int nk;                    /* NUMBER_OF_K_BANDS */
+    int nz;                    /* NUMBER_OF_Z_BINS */
+    const double *kband_edges; /* nk + 1 values (KBAND_EDGES) */
+    const double *zbin_centers;/* nz values (ZBIN_CENTERS) */
+    double z_bin_width;        /* Z_BIN_WIDTH */
+    int z_binning_method;      /* 0 top hat, 1 triangular (RedshiftBinningMethod) */
+    int cont_logl_order;       /* ContinuumLogLambdaMargOrder */
+    int cont_lam_order;        /* ContinuumLambdaMargOrder */
+    int turn_off_sfid;         /* TurnOffBaseline > 0 */
+    double max_forest_length_v;/* MaximumForestLength / DynamicChunkNumber */
+    double max_pixel_length_v; /* MaximumMeanPixelSize */
+} qml_bins;
+
+/* One chunk: the pixel subset [i1, i2) of a spectrum after the file-level
+ * preprocessing of OneQSOEstimate::_readQsoFile.  Pointers are read during
+ * the call only. */
+typedef struct {
+    const double *wave;   /* observed wavelength, Angstrom */
+    const double *delta;  /* flux fluctuations */
+    const double *ivar;   /* inverse variance, 0 = masked */
+    int size;             /* number of pixels */
+    int R_fwhm;           /* spectrograph resolution (integer R) */
+} qml_chunk_in;
+
+/* What the Chunk constructor derived (core/chunk_estimate.cpp:141-283). */
+typedef struct {
+    int size;               /* pixels after the masked-boundary cut */
+    int real_size;          /* unmasked pixels */
+    int res_index;          /* table index (RES_INDEX) */
+    int kbin_upp;           /* KBIN_UPP */
+    int zbin, zbin_low, zbin_upp;
+    int n_q_matrices;       /* N_Q_MATRICES */
+    int fisher_index_start; /* fisher_index_start */
+    int n_stored;           /* stored Q_k count (kn < KBIN_UPP) */
+    int n_templates;        /* continuum templates kept after the SVD cut */
+    double dv_kms;          /* median pixel width */
+    double lower_z, upper_z, median_z;
+} qml_chunk_info;
+
+/* Per-phase device time of the last qml_iterate, milliseconds, in the
+ * categories of mytime:: (core/global_numbers.cpp:484-502). */
+typedef struct {
+    double assemble_ms;   /* v/z, S_fid, Q_k interpolation, C assembly  (T_Sfid + T_Q) */
+    double invert_ms;     /* factorisation, inverse, solve, marginalisation (T_Cinv) */
+    double weight_ms;     /* d_k pass, projected Q_k generation, W products (T_Qmod) */
+    double fisher_ms;     /* Gram contraction and reductions (T_F) */
+    double accumulate_ms; /* cross-chunk accumulation and copy-out */
+    double total_ms;      /* host submit to host arrays updated */
+    long long n_batches;
+} qml_timings;
+
+/* Context lifetime.  qml_create selects the device and creates a stream. */
+int qml_create(qml_ctx **ctx, const qml_opts *opts);
+int qml_destroy(qml_ctx *ctx);
+const char *qml_last_error(void);
+
+/* Binning; must precede tables and chunks.  Replaces the bins::/specifics::
+ * globals read by Chunk. */
+int qml_set_bins(qml_ctx *ctx, const qml_bins *bins);
+
+/* One (R, dv) S/Q table, as SQLookupTable holds it after readTables()
+ * (core/sq_table.cpp:325-368): deriv is nk x nv with v contiguous
+ * (the DERIVATIVE HDU's byte order), signal is nv x nz with z contiguous
+ * (the SIGNAL HDU's).  The v grid is v_j = j * v_length/(nv-1); the z grid is
+ * ZBIN_CENTERS[0] - Z_BIN_WIDTH + j * Z_BIN_WIDTH*(NZBINS+1)/(nz-1).
+ * Returns the table index in *r_index (the order of calls, like the rows of
+ * the R-dv list read by ioh::readListRdv). */
+int qml_add_table(qml_ctx *ctx, int R_fwhm, double dv_kms, int nv, double v_length,
+                  int nz, const double *deriv, const double *signal, int *r_index);
+
+/* Registers a chunk (the work of Chunk::Chunk).  On QML_ERR_SHORT_CHUNK,
+ * QML_ERR_NO_TABLE or QML_ERR_EMPTY the chunk is not registered: the caller
+ * skips it, as OneQSOEstimate does (core/one_qso_estimate.cpp:115-125).
+ * *chunk_id receives the chunk's index. */
+int qml_add_chunk(qml_ctx *ctx, const qml_chunk_in *in, qml_chunk_info *info, int *chunk_id);
+
+/* Uploads all registered chunks, plans the size-bucketed batches and
+ * allocates device scratch.  Inputs stay resident in device memory. */
+int qml_commit(qml_ctx *ctx);
+
+/* One Newton-Raphson iteration's chunk loop.  theta is TOTAL_KZ_BINS values
+ * (the ps_estimate argument of Chunk::oneQSOiteration).  Adds every chunk's
+ * unhalved Fisher matrix into the upper triangle of fisher_sum (T x T,
+ * row-major) and d, b, t into dbt_sum (3 x T), exactly like
+ * core/chunk_estimate.cpp:893-903. */
+int qml_iterate(qml_ctx *ctx, const double *theta, double *fisher_sum, double *dbt_sum);
+
+/* The last iteration's result for one chunk: fisher is n_q x n_q (upper
+ * triangle filled, like Chunk::fisher_matrix), dbt is 3 x n_q (like
+ * Chunk::dbt_estimate_before_fisher_vector).  Either pointer may be NULL. */
+int qml_get_chunk_result(qml_ctx *ctx, int chunk_id, double *fisher, double *dbt, int *status);
+
+int qml_get_timings(qml_ctx *ctx, qml_timings *t);
+
+/* Number of chunks registered, and TOTAL_KZ_BINS. */
+int qml_num_chunks(qml_ctx *ctx);
+int qml_total_kz_bins(qml_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LYSPEQ_QML_H */

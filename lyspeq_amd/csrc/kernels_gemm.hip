@@ -1,0 +1,361 @@
+// Dense MFMA products of the QML engine (gfx950).
+//
+//   whiten_left :  Y_j = Li X_j              (Li lower triangular: K-range of
+//                                             tile row I is [0, 64 (I+1)))
+//   whiten_right:  G_j = Y_j Li^T            (upper tiles only, K-range
+//                                             [0, 64 (J+1)), tile-packed out)
+//   gram        :  E_s = sum_{k in G tile s} G_j[k] G_j'[k]   (split-K partials,
+//                                             x2 for off-diagonal G tiles)
+//
+// One 256-thread workgroup computes a 64x64 output tile; each of its four
+// waves owns a 32x32 quarter made of 2x2 16x16 MFMA blocks.  Precision:
+//   FP64   v_mfma_f64_16x16x4f64, FP64 operands and accumulation;
+//   FP32   v_mfma_f32_16x16x4f32, FP32 operands and accumulation;
+//   BF16x3 v_mfma_f32_16x16x32_bf16 three times per block with the FP32
+//          operands split into bf16 hi + lo at LDS staging:
+//          a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation.
+// The work list holds (chunk, member or split, I, J) per workgroup; work is
+// assigned so that consecutive items stay on one XCD (blockIdx % 8 selects
+// the XCD), keeping a chunk's operands in that XCD's L2.
+#include "qml_common.hpp"
+#include "qml_kernels.hpp"
+
+namespace qml {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+enum { MODE_LEFT = 0, MODE_RIGHT = 1, MODE_GRAM = 2 };
+
+__device__ __forceinline__ int upper_tile_index(int I, int J, int nt) {
+    return I * nt - I * (I - 1) / 2 + (J - I);
+}
+
+__device__ __forceinline__ bool packed_tile_is_diag(int s, int nt) {
+    int I = 0;
+    while (s >= nt - I) { s -= nt - I; ++I; }
+    return s == 0;
+}
+
+__device__ __forceinline__ int xcd_logical(int nwork) {
+    const int per = (nwork + 7) / 8;
+    return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+// Problem description of one work item.
+template <class T>
+struct Prob {
+    const T *A;  // row-major, rows of the output tile, K contiguous (lda)
+    const T *B;  // NN: row-major K x N (ldb);  NT: row-major N x K (ldb)
+    int lda, ldb;
+    int kbeg, kend;
+    int arows, brows;   // valid rows of A / B^T (members for the Gram); others read as 0
+    bool nt;
+};
+
+// ------------------------------------------------------ FP64 / FP32 path --
+template <class T> struct MF;
+template <> struct MF<double> {
+    using acc_t = d4;
+    static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MF<float> {
+    using acc_t = f4;
+    static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <class T>
+__device__ __forceinline__ void mainloop_k4(const Prob<T> &p, int I0, int J0,
+                                            typename MF<T>::acc_t (&acc)[2][2]) {
+    constexpr int KB = 16, PAD = 4;
+    __shared__ T As[KB][64 + PAD];
+    __shared__ T Bs[KB][64 + PAD];
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int wm = w / 2, wn = w % 2;
+    // A pattern: row tid/4, four consecutive k at (tid%4)*4
+    const int ar = tid / 4, ak = (tid % 4) * 4;
+    // NN pattern for B: k row tid/16, four consecutive n at (tid%16)*4
+    const int bk = tid / 16, bn = (tid % 16) * 4;
+    T ra[4], rb[4];
+    auto gload = [&](int k0) {
+        const bool aok = (I0 + ar) < p.arows;
+        const T *pa = p.A + (size_t)(I0 + ar) * p.lda + k0 + ak;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ra[u] = aok ? pa[u] : (T)0;
+        if (p.nt) {
+            const bool bok = (J0 + ar) < p.brows;
+            const T *pb = p.B + (size_t)(J0 + ar) * p.ldb + k0 + ak;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rb[u] = bok ? pb[u] : (T)0;
+        } else {
+            const T *pb = p.B + (size_t)(k0 + bk) * p.ldb + J0 + bn;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rb[u] = pb[u];
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) As[ak + u][ar] = ra[u];
+        if (p.nt) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Bs[ak + u][ar] = rb[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Bs[bk][bn + u] = rb[u];
+        }
+    };
+    if (p.kbeg >= p.kend) return;
+    gload(p.kbeg);
+    sstore();
+    __syncthreads();
+    for (int k0 = p.kbeg; k0 < p.kend; k0 += KB) {
+        const bool more = k0 + KB < p.kend;
+        if (more) gload(k0 + KB);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 4) {
+            T a[2], b[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                a[q] = As[kk + lane / 16][wm * 32 + q * 16 + lane % 16];
+                b[q] = Bs[kk + lane / 16][wn * 32 + q * 16 + lane % 16];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = MF<T>::mfma(a[x], b[y], acc[x][y]);
+        }
+        __syncthreads();
+        if (more) sstore();
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------- BF16x3 path --
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+__device__ __forceinline__ void mainloop_bf16x3(const Prob<float> &p, int I0, int J0, f4 (&acc)[2][2]) {
+    constexpr int KB = 32, LD = 40;   // row pitch 80 bytes keeps 16-byte alignment
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[64][LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[64][LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[64][LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[64][LD];
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int wm = w / 2, wn = w % 2;
+    const int ar = tid / 4, ak = (tid % 4) * 8;      // A / NT pattern: 8 consecutive k
+    const int bk = tid / 8, bn = (tid % 8) * 8;      // NN pattern: 8 consecutive n
+    float ra[8], rb[8];
+    auto gload = [&](int k0) {
+        const bool aok = (I0 + ar) < p.arows;
+        const float *pa = p.A + (size_t)(I0 + ar) * p.lda + k0 + ak;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ra[u] = aok ? pa[u] : 0.f;
+        if (p.nt) {
+            const bool bok = (J0 + ar) < p.brows;
+            const float *pb = p.B + (size_t)(J0 + ar) * p.ldb + k0 + ak;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rb[u] = bok ? pb[u] : 0.f;
+        } else {
+            const float *pb = p.B + (size_t)(k0 + bk) * p.ldb + J0 + bn;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rb[u] = pb[u];
+        }
+    };
+    auto sstore = [&]() {
+        bf8 h, l;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            __bf16 a, b;
+            split_bf16(ra[u], a, b);
+            h[u] = a;
+            l[u] = b;
+        }
+        *(bf8 *)&Ah[ar][ak] = h;
+        *(bf8 *)&Al[ar][ak] = l;
+        if (p.nt) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                __bf16 a, b;
+                split_bf16(rb[u], a, b);
+                h[u] = a;
+                l[u] = b;
+            }
+            *(bf8 *)&Bh[ar][ak] = h;
+            *(bf8 *)&Bl[ar][ak] = l;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                __bf16 a, b;
+                split_bf16(rb[u], a, b);
+                Bh[bn + u][bk] = a;
+                Bl[bn + u][bk] = b;
+            }
+        }
+    };
+    if (p.kbeg >= p.kend) return;
+    gload(p.kbeg);
+    sstore();
+    __syncthreads();
+    for (int k0 = p.kbeg; k0 < p.kend; k0 += KB) {
+        const bool more = k0 + KB < p.kend;
+        if (more) gload(k0 + KB);
+        bf8 ah[2], al[2], bh[2], bl[2];
+        const int kq = 8 * (lane / 16);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            ah[q] = *(const bf8 *)&Ah[wm * 32 + q * 16 + lane % 16][kq];
+            al[q] = *(const bf8 *)&Al[wm * 32 + q * 16 + lane % 16][kq];
+            bh[q] = *(const bf8 *)&Bh[wn * 32 + q * 16 + lane % 16][kq];
+            bl[q] = *(const bf8 *)&Bl[wn * 32 + q * 16 + lane % 16][kq];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+            }
+        __syncthreads();
+        if (more) sstore();
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ kernel --
+template <int PREC, int MODE>
+__global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *work, int nwork,
+                                              const void *Abase, const void *Bbase, void *Cbase) {
+    using T = typename std::conditional<PREC == 0, double, float>::type;
+    using acc_t = typename std::conditional<PREC == 0, d4, f4>::type;
+    const int li = xcd_logical(nwork);
+    if (li >= nwork) return;
+    const Work wk = work[li];
+    const ChunkDev c = cs[wk.c];
+    const int Np = c.Np;
+    const size_t msz = (size_t)Np * Np;
+    Prob<T> p;
+    int I0, J0;
+    if (MODE == MODE_LEFT) {
+        p.A = (const T *)Abase + c.lic_off;
+        p.B = (const T *)Bbase + c.stk_off + wk.j * msz;
+        p.lda = p.ldb = Np;
+        p.kbeg = 0;
+        p.kend = (wk.I + 1) * TILE;
+        p.arows = p.brows = Np;
+        p.nt = false;
+        I0 = wk.I * TILE;
+        J0 = wk.J * TILE;
+    } else if (MODE == MODE_RIGHT) {
+        p.A = (const T *)Abase + c.stk_off + wk.j * msz;
+        p.B = (const T *)Bbase + c.lic_off;
+        p.lda = p.ldb = Np;
+        p.kbeg = 0;
+        p.kend = (wk.J + 1) * TILE;
+        p.arows = p.brows = Np;
+        p.nt = true;
+        I0 = wk.I * TILE;
+        J0 = wk.J * TILE;
+    } else {
+        const int ktot = c.ntu * TILE * TILE;
+        p.A = (const T *)Abase + c.stk_off;
+        p.B = p.A;
+        p.lda = p.ldb = ktot;
+        p.kbeg = wk.j * TILE * TILE;
+        p.kend = p.kbeg + TILE * TILE;
+        p.arows = p.brows = c.m;
+        p.nt = true;
+        I0 = wk.I * TILE;
+        J0 = wk.J * TILE;
+    }
+    acc_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[x][y][u] = 0;
+    if constexpr (PREC == 2)
+        mainloop_bf16x3(p, I0, J0, acc);
+    else
+        mainloop_k4<T>(p, I0, J0, acc);
+
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int wm = w / 2, wn = w % 2;
+    if (MODE == MODE_LEFT) {
+        T *C = (T *)Cbase + c.stk_off + wk.j * msz;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = I0 + wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int q = J0 + wn * 32 + y * 16 + lane % 16;
+                    C[(size_t)r * Np + q] = (T)acc[x][y][u];
+                }
+    } else if (MODE == MODE_RIGHT) {
+        T *C = (T *)Cbase + c.stk_off + (size_t)wk.j * c.ntu * TILE * TILE
+             + (size_t)upper_tile_index(wk.I, wk.J, c.nt) * TILE * TILE;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int q = wn * 32 + y * 16 + lane % 16;
+                    C[r * TILE + q] = (T)acc[x][y][u];
+                }
+    } else {
+        const int ntile = c.mt * (c.mt + 1) / 2;
+        const double wgt = packed_tile_is_diag(wk.j, c.nt) ? 1.0 : 2.0;
+        double *C = (double *)Cbase + c.gram_off
+                  + ((size_t)wk.j * ntile + upper_tile_index(wk.I, wk.J, c.mt)) * TILE * TILE;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int q = wn * 32 + y * 16 + lane % 16;
+                    C[r * TILE + q] = wgt * (double)acc[x][y][u];
+                }
+    }
+}
+
+template <int MODE>
+static void launch_mode(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                        const void *A, const void *B, void *C) {
+    if (nwork <= 0) return;
+    const int grid = 8 * ((nwork + 7) / 8);
+    if (prec == 0)
+        k_gemm<0, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+    else if (prec == 1)
+        k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+    else
+        k_gemm<2, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+}
+
+void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                             const void *li, const void *X, void *Y) {
+    launch_mode<MODE_LEFT>(st, prec, chunks, work, nwork, li, X, Y);
+}
+void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                              const void *li, const void *Y, void *G) {
+    launch_mode<MODE_RIGHT>(st, prec, chunks, work, nwork, Y, li, G);
+}
+void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                      const void *G, double *gram) {
+    launch_mode<MODE_GRAM>(st, prec, chunks, work, nwork, G, G, gram);
+}
+
+}  // namespace qml

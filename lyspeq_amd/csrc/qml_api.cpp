@@ -1,0 +1,683 @@
+// C ABI of the QML engine: chunk registration (the Chunk constructor's
+// bookkeeping), batch planning, and the per-iteration launch sequence.
+#include "lyspeq_qml.h"
+#include "qml_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace qml;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail(QML_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+constexpr double MY_PI = 3.14159265358979323846;
+constexpr double DOUBLE_EPSILON = 1e-15;
+
+struct TableHost {
+    int R;
+    double dv;
+    int nv, nz;
+    double v_length;
+    double *d_q = nullptr, *d_s = nullptr;
+};
+
+struct ChunkHost {
+    qml_chunk_info info{};
+    std::vector<double> wave, delta, ivar, U;  // wave in units of lambda_Lya; U is nu x N
+    int N = 0, Np = 0, nu = 0;
+    int n = 0, m = 0, kbu = 0, nzq = 0, zm0 = 0, nq = 0, fis = 0;
+    long long in_off = 0;
+};
+
+struct Batch {
+    std::vector<int> ids;
+    ChunkDev *d_chunks = nullptr;
+    Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
+    int nwl = 0, nwr = 0, nwg = 0;
+    int np_max = 0;
+    hipEvent_t ev[6]{};
+};
+
+double median_sorted(const std::vector<double> &a) {
+    const size_t n = a.size();
+    if (n % 2 == 0) return (a[n / 2 - 1] + a[n / 2]) / 2;
+    return a[n / 2];
+}
+
+// One-sided Jacobi SVD of the N x nt column set V (column-major, columns
+// contiguous).  Returns singular values (descending) and overwrites V with
+// the matching left singular vectors.
+void jacobi_svd(std::vector<double> &V, int N, int nt, std::vector<double> &sv) {
+    auto col = [&](int j) { return V.data() + (size_t)j * N; };
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < nt; ++p)
+            for (int q = p + 1; q < nt; ++q) {
+                double a = 0, b = 0, g = 0;
+                const double *vp = col(p), *vq = col(q);
+                for (int i = 0; i < N; ++i) { a += vp[i] * vp[i]; b += vq[i] * vq[i]; g += vp[i] * vq[i]; }
+                if (std::fabs(g) <= 1e-17 * std::sqrt(a * b) || g == 0.0) continue;
+                rotated = true;
+                const double zeta = (b - a) / (2 * g);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+                const double cs = 1 / std::sqrt(1 + t * t), sn = cs * t;
+                double *wp = col(p), *wq = col(q);
+                for (int i = 0; i < N; ++i) {
+                    const double x = wp[i], y = wq[i];
+                    wp[i] = cs * x - sn * y;
+                    wq[i] = sn * x + cs * y;
+                }
+            }
+        if (!rotated) break;
+    }
+    std::vector<double> s(nt);
+    for (int j = 0; j < nt; ++j) {
+        double a = 0;
+        for (int i = 0; i < N; ++i) a += col(j)[i] * col(j)[i];
+        s[j] = std::sqrt(a);
+    }
+    std::vector<int> order(nt);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return s[x] > s[y]; });
+    std::vector<double> out((size_t)N * nt);
+    sv.assign(nt, 0.0);
+    for (int j = 0; j < nt; ++j) {
+        const int o = order[j];
+        sv[j] = s[o];
+        for (int i = 0; i < N; ++i) out[(size_t)j * N + i] = (s[o] > 0) ? col(o)[i] / s[o] : 0.0;
+    }
+    V.swap(out);
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct qml_ctx {
+    int device = 0, prec = 0;
+    double scratch_gb = 32.0;
+    int max_batch = 256;
+    hipStream_t st = nullptr;
+    bool bins_set = false, committed = false;
+    std::vector<double> kedges, kcenters, zcenters;
+    qml_bins bins{};
+    BinsDev binsdev{};
+    int T = 0;
+    std::vector<TableHost> tables;
+    TableDev *d_tables = nullptr;
+    std::vector<ChunkHost> chunks;
+
+    double *d_in = nullptr;
+    std::vector<ChunkOut> outs;
+    ChunkOut *d_outs = nullptr;
+    double *d_fc = nullptr, *d_dbt = nullptr;
+    int *d_status = nullptr;
+    double *d_mat = nullptr, *d_vec = nullptr, *d_pj = nullptr, *d_sj = nullptr, *d_gram = nullptr;
+    void *d_X = nullptr, *d_Y = nullptr, *d_lic = nullptr;
+    double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
+    std::vector<Batch> batches;
+    hipEvent_t ev_begin{}, ev_end{};
+    qml_timings tm{};
+    std::vector<double> hF, hdbt;
+};
+
+extern "C" {
+
+const char *qml_last_error(void) { return g_err.c_str(); }
+
+int qml_create(qml_ctx **out, const qml_opts *opts) {
+    if (!out) return fail(QML_ERR_ARG, "qml_create: null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(QML_ERR_HIP, "qml_create: no HIP device available");
+    auto *c = new qml_ctx();
+    if (opts) {
+        c->device = opts->device;
+        c->prec = opts->precision;
+        if (opts->scratch_gb > 0) c->scratch_gb = opts->scratch_gb;
+        if (opts->max_batch_chunks > 0) c->max_batch = opts->max_batch_chunks;
+    }
+    if (c->prec < 0 || c->prec > 2) {
+        delete c;
+        return fail(QML_ERR_ARG, "qml_create: unknown precision");
+    }
+    if (c->device < 0 || c->device >= ndev) {
+        delete c;
+        return fail(QML_ERR_ARG, "qml_create: device ordinal out of range");
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        delete c;
+        return fail(QML_ERR_HIP, "qml_create: stream/event creation failed");
+    }
+    *out = c;
+    return QML_OK;
+}
+
+int qml_destroy(qml_ctx *c) {
+    if (!c) return QML_OK;
+    hipSetDevice(c->device);
+    if (c->st) hipStreamSynchronize(c->st);
+    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_s); }
+    for (auto &b : c->batches) {
+        hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
+        for (auto &e : b.ev) if (e) hipEventDestroy(e);
+    }
+    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_mat, c->d_vec,
+                    c->d_pj, c->d_sj, c->d_gram, c->d_X, c->d_Y, c->d_lic, c->d_theta, c->d_F, c->d_dbtsum};
+    for (void *p : ptrs) if (p) hipFree(p);
+    if (c->ev_begin) hipEventDestroy(c->ev_begin);
+    if (c->ev_end) hipEventDestroy(c->ev_end);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return QML_OK;
+}
+
+int qml_set_bins(qml_ctx *c, const qml_bins *b) {
+    if (!c || !b) return fail(QML_ERR_ARG, "qml_set_bins: null argument");
+    if (c->committed || !c->chunks.empty() || !c->tables.empty())
+        return fail(QML_ERR_ARG, "qml_set_bins: bins must be set before tables and chunks");
+    if (b->nk <= 0 || b->nk > MAX_NK || b->nz <= 0 || b->nz > MAX_ZB || !b->kband_edges || !b->zbin_centers ||
+        !(b->z_bin_width > 0))
+        return fail(QML_ERR_ARG, "qml_set_bins: invalid bins");
+    c->bins = *b;
+    c->kedges.assign(b->kband_edges, b->kband_edges + b->nk + 1);
+    c->zcenters.assign(b->zbin_centers, b->zbin_centers + b->nz);
+    c->kcenters.resize(b->nk);
+    for (int k = 0; k < b->nk; ++k) c->kcenters[k] = (c->kedges[k] + c->kedges[k + 1]) / 2.;
+    c->bins.kband_edges = c->kedges.data();
+    c->bins.zbin_centers = c->zcenters.data();
+    std::memset(&c->binsdev, 0, sizeof(c->binsdev));
+    for (int z = 0; z < b->nz; ++z) c->binsdev.zc[z] = c->zcenters[z];
+    c->binsdev.dz = b->z_bin_width;
+    c->binsdev.nz = b->nz;
+    c->binsdev.nk = b->nk;
+    c->binsdev.method = b->z_binning_method;
+    c->binsdev.turn_off_sfid = b->turn_off_sfid;
+    c->T = b->nk * b->nz;
+    int nvec = 0;
+    if (b->cont_logl_order >= 0 || b->cont_lam_order >= 0) {
+        nvec = 1 + std::max(0, b->cont_logl_order) + std::max(0, b->cont_lam_order);
+    }
+    if (nvec > MAX_NU) return fail(QML_ERR_LIMIT, "qml_set_bins: more than 4 continuum templates");
+    c->bins_set = true;
+    return QML_OK;
+}
+
+int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length, int nz,
+                  const double *deriv, const double *signal, int *r_index) {
+    if (!c || !c->bins_set) return fail(QML_ERR_ARG, "qml_add_table: set bins first");
+    if (c->committed) return fail(QML_ERR_ARG, "qml_add_table: context already committed");
+    if (nv < 2 || !deriv || (!signal && !c->bins.turn_off_sfid) || (!c->bins.turn_off_sfid && nz < 2))
+        return fail(QML_ERR_ARG, "qml_add_table: invalid table");
+    hipSetDevice(c->device);
+    const int nk = c->bins.nk;
+    TableHost t;
+    t.R = R_fwhm;
+    t.dv = dv_kms;
+    t.nv = nv;
+    t.nz = std::max(nz, 2);
+    t.v_length = v_length;
+    std::vector<double> q((size_t)nv * nk);
+    for (int kn = 0; kn < nk; ++kn)
+        for (int j = 0; j < nv; ++j) q[(size_t)j * nk + kn] = deriv[(size_t)kn * nv + j];
+    std::vector<double> s((size_t)nv * t.nz, 0.0);
+    if (signal && !c->bins.turn_off_sfid) std::copy(signal, signal + (size_t)nv * nz, s.begin());
+    HIPCHK(hipMalloc(&t.d_q, q.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&t.d_s, s.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(t.d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.d_s, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->tables.push_back(t);
+    if (r_index) *r_index = (int)c->tables.size() - 1;
+    return QML_OK;
+}
+
+int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int *chunk_id) {
+    if (!c || !in || !c->bins_set) return fail(QML_ERR_ARG, "qml_add_chunk: set bins first");
+    if (c->committed) return fail(QML_ERR_ARG, "qml_add_chunk: context already committed");
+    if (in->size <= 0 || !in->wave || !in->delta || !in->ivar)
+        return fail(QML_ERR_ARG, "qml_add_chunk: empty chunk");
+    const qml_bins &b = c->bins;
+    ChunkHost ch;
+    // QSOFile(qmaster, i1, i2) + _cutMaskedBoundary (io/qso_file.cpp:61-89,139-157)
+    int i1 = 0, i2 = in->size;
+    while (i1 < in->size && in->ivar[i1] == 0) ++i1;
+    while (i2 > 0 && in->ivar[i2 - 1] == 0) --i2;
+    if (i2 - i1 <= 0) return fail(QML_ERR_EMPTY, "Empty spectrum when masked pixels at boundary are removed!");
+    const int N = i2 - i1;
+    ch.wave.assign(in->wave + i1, in->wave + i2);
+    ch.delta.assign(in->delta + i1, in->delta + i2);
+    ch.ivar.assign(in->ivar + i1, in->ivar + i2);
+    int real = 0;
+    for (double v : ch.ivar) real += (v != 0);
+    // recalcDvDLam (io/qso_file.cpp:325-329)
+    double dv = -1;
+    if (N > 1) {
+        std::vector<double> d(N - 1);
+        for (int i = 0; i < N - 1; ++i) d[i] = std::log(ch.wave[i + 1] / ch.wave[i]) * SPEED_OF_LIGHT;
+        std::sort(d.begin(), d.end());
+        dv = median_sorted(d);
+    }
+    // short chunk (core/chunk_estimate.cpp:172-187)
+    const double lenv = SPEED_OF_LIGHT * std::log(ch.wave[N - 1] / ch.wave[0]);
+    const int minnopix = (int)(lenv / b.max_pixel_length_v);
+    const double min_len = 0.90 / 2 * b.max_forest_length_v;
+    if (real < minnopix || lenv < min_len) return fail(QML_ERR_SHORT_CHUNK, "Short chunk");
+    // findSpecResIndex (core/sq_table.cpp:388-397)
+    const double rounded = std::round(dv * 10) / 10;
+    int rindex = -1;
+    for (size_t r = 0; r < c->tables.size(); ++r)
+        if (c->tables[r].R == in->R_fwhm && std::fabs(rounded - c->tables[r].dv) < 0.01) { rindex = (int)r; break; }
+    if (rindex < 0) {
+        char msg[128];
+        std::snprintf(msg, sizeof msg, "SpecRes R: %d, dv: %g not found in tables!", in->R_fwhm, dv);
+        return fail(QML_ERR_NO_TABLE, msg);
+    }
+    // readMinMaxMedRedshift (io/qso_file.cpp:280-300)
+    const double zlo = ch.wave[0] / LYA_REST - 1, zhi = ch.wave[N - 1] / LYA_REST - 1;
+    const double zmed = median_sorted(ch.wave) / LYA_REST - 1;
+    const int kbu = (int)(std::lower_bound(c->kcenters.begin(), c->kcenters.end(), 0.85 * (MY_PI / dv)) -
+                          c->kcenters.begin());
+    auto findz = [&](double z) {   // bins::findRedshiftBin
+        const double zle = c->zcenters[0] - b.z_bin_width / 2.;
+        if (z < zle) return -1;
+        int r = (int)((z - zle) / b.z_bin_width);
+        if (r >= b.nz) r = b.nz;
+        return r;
+    };
+    const int zb = findz(zmed), zl = findz(zlo), zu = findz(zhi);
+    // _setNQandFisherIndex (core/chunk_estimate.cpp:240-257)
+    int nq = zu - zl + 1, fis = b.nk * zl;
+    if (b.z_binning_method == 1) {
+        if (zl >= 0 && zlo < c->zcenters[zl] && zl != 0) { ++nq; fis -= b.nk; }
+        if (zu >= 0 && zu < b.nz && c->zcenters[zu] < zhi && zu < b.nz - 1) ++nq;
+    }
+    nq *= b.nk;
+    if (zl < 0 || zu >= b.nz || fis < 0 || fis + nq > c->T)
+        return fail(QML_ERR_LIMIT, "qml_add_chunk: chunk lies outside the redshift bins");
+    for (double &w : ch.wave) w /= LYA_REST;
+    ch.N = N;
+    ch.nq = nq;
+    ch.fis = fis;
+    ch.kbu = kbu;
+    ch.nzq = nq / b.nk;
+    ch.zm0 = fis / b.nk;
+    ch.n = ch.nzq * kbu;
+    if (ch.nzq > 8) return fail(QML_ERR_LIMIT, "qml_add_chunk: a chunk spans more than 8 redshift bins");
+    // continuum templates + SVD cut (core/chunk_estimate.cpp:576-665)
+    int nt = 0;
+    if (b.cont_logl_order >= 0 || b.cont_lam_order >= 0)
+        nt = 1 + std::max(0, b.cont_logl_order) + std::max(0, b.cont_lam_order);
+    ch.nu = 0;
+    if (nt > 0) {
+        std::vector<double> V((size_t)nt * N);
+        for (int i = 0; i < N; ++i) V[i] = 1. / std::sqrt((double)N);
+        int row = 1;
+        for (int cmo = 1; cmo <= b.cont_logl_order; ++cmo, ++row) {
+            double nrm = 0;
+            for (int i = 0; i < N; ++i) {
+                const double v = std::pow(std::log(ch.wave[i]), cmo);
+                V[(size_t)row * N + i] = v;
+                nrm += v * v;
+            }
+            const double sc = 1.0 / std::sqrt(nrm);
+            for (int i = 0; i < N; ++i) V[(size_t)row * N + i] *= sc;
+        }
+        for (int cmo = 1; cmo <= b.cont_lam_order; ++cmo, ++row) {
+            double nrm = 0;
+            for (int i = 0; i < N; ++i) {
+                const double v = std::pow(ch.wave[i], cmo);
+                V[(size_t)row * N + i] = v;
+                nrm += v * v;
+            }
+            const double sc = 1.0 / std::sqrt(nrm);
+            for (int i = 0; i < N; ++i) V[(size_t)row * N + i] *= sc;
+        }
+        std::vector<double> sv;
+        jacobi_svd(V, N, nt, sv);
+        int use = nt;
+        while (use > 0) {
+            if (sv[use - 1] / sv[0] > DOUBLE_EPSILON) break;
+            --use;
+        }
+        ch.nu = use;
+        ch.U.assign(V.begin(), V.begin() + (size_t)use * N);
+    }
+    qml_chunk_info &inf = ch.info;
+    inf.size = N;
+    inf.real_size = real;
+    inf.res_index = rindex;
+    inf.kbin_upp = kbu;
+    inf.zbin = zb;
+    inf.zbin_low = zl;
+    inf.zbin_upp = zu;
+    inf.n_q_matrices = nq;
+    inf.fisher_index_start = fis;
+    inf.n_stored = ch.n;
+    inf.n_templates = ch.nu;
+    inf.dv_kms = dv;
+    inf.lower_z = zlo;
+    inf.upper_z = zhi;
+    inf.median_z = zmed;
+    if (info) *info = inf;
+    c->chunks.push_back(std::move(ch));
+    if (chunk_id) *chunk_id = (int)c->chunks.size() - 1;
+    return QML_OK;
+}
+
+int qml_commit(qml_ctx *c) {
+    if (!c || !c->bins_set) return fail(QML_ERR_ARG, "qml_commit: set bins first");
+    if (c->committed) return fail(QML_ERR_ARG, "qml_commit: already committed");
+    HIPCHK(hipSetDevice(c->device));
+    const int nch = (int)c->chunks.size();
+    const size_t es = (c->prec == 0) ? 8 : 4;
+    // tables
+    std::vector<TableDev> td(c->tables.size());
+    for (size_t r = 0; r < c->tables.size(); ++r) {
+        const auto &t = c->tables[r];
+        td[r].q = t.d_q;
+        td[r].s = t.d_s;
+        td[r].nv = t.nv;
+        td[r].nz = t.nz;
+        td[r].v1 = 0.0;
+        td[r].dv = t.v_length / (t.nv - 1.0);
+        td[r].z1 = c->zcenters[0] - c->bins.z_bin_width;
+        td[r].dz = c->bins.z_bin_width * (c->bins.nz + 1) / (t.nz - 1.0);
+    }
+    if (!td.empty()) {
+        HIPCHK(hipMalloc(&c->d_tables, td.size() * sizeof(TableDev)));
+        HIPCHK(hipMemcpy(c->d_tables, td.data(), td.size() * sizeof(TableDev), hipMemcpyHostToDevice));
+    }
+    // inputs and persistent outputs
+    size_t in_total = 0, fc_total = 0, dbt_total = 0;
+    c->outs.resize(nch);
+    for (int i = 0; i < nch; ++i) {
+        auto &ch = c->chunks[i];
+        ch.Np = (int)align_up(ch.N, TILE);
+        ch.m = ch.n + 2;
+        ch.in_off = (long long)in_total;
+        in_total += (size_t)IN_SLOTS * ch.Np;
+        c->outs[i].fc_off = (long long)fc_total;
+        c->outs[i].dbt_off = (long long)dbt_total;
+        c->outs[i].nq = ch.nq;
+        c->outs[i].fis = ch.fis;
+        fc_total += (size_t)ch.nq * ch.nq;
+        dbt_total += (size_t)3 * ch.nq;
+    }
+    std::vector<double> hin(std::max<size_t>(in_total, 1), 0.0);
+    for (int i = 0; i < nch; ++i) {
+        const auto &ch = c->chunks[i];
+        double *base = hin.data() + ch.in_off;
+        const size_t Np = ch.Np;
+        for (int p = 0; p < ch.Np; ++p) {
+            const bool real = p < ch.N;
+            base[IN_WAVE * Np + p] = real ? ch.wave[p] : ch.wave[ch.N - 1];
+            base[IN_DELTA * Np + p] = real ? ch.delta[p] : 0.0;
+            const double iv = real ? ch.ivar[p] : 0.0;
+            base[IN_IVAR * Np + p] = iv;
+            base[IN_D * Np + p] = std::sqrt(iv);
+            base[IN_NPRIME * Np + p] = (iv != 0) ? 1.0 / iv : 0.0;
+            for (int t = 0; t < ch.nu; ++t) base[(IN_U0 + t) * Np + p] = real ? ch.U[(size_t)t * ch.N + p] : 0.0;
+        }
+    }
+    HIPCHK(hipMalloc(&c->d_in, hin.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_in, hin.data(), hin.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->d_outs, std::max(nch, 1) * sizeof(ChunkOut)));
+    if (nch) HIPCHK(hipMemcpy(c->d_outs, c->outs.data(), nch * sizeof(ChunkOut), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->d_fc, std::max<size_t>(fc_total, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_dbt, std::max<size_t>(dbt_total, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_status, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMemset(c->d_status, 0, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_theta, std::max(c->T, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_F, (size_t)c->T * c->T * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
+
+    // per-chunk scratch sizes (elements of each arena)
+    struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
+    auto need = [&](const ChunkHost &ch) {
+        Need n{};
+        const size_t Np = ch.Np, m = ch.m, nt = Np / TILE, ntu = nt * (nt + 1) / 2;
+        const size_t mt = (m + TILE - 1) / TILE;
+        n.mat = 4 * Np * Np;
+        n.vec = (size_t)VEC_SLOTS * Np;
+        n.pj = m * Np * (1 + MAX_NU);
+        n.sj = m * MAX_NU * MAX_NU + ch.n;
+        n.stk = m * Np * Np;
+        n.lic = (c->prec == 0) ? 0 : Np * Np;
+        n.gram = ntu * (mt * (mt + 1) / 2) * TILE * TILE;
+        return n;
+    };
+    auto bytes = [&](const Need &n) {
+        return 8 * (n.mat + n.vec + n.pj + n.sj + n.gram) + es * (2 * n.stk + n.lic);
+    };
+    std::vector<int> order(nch);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (c->chunks[a].Np != c->chunks[b].Np) return c->chunks[a].Np > c->chunks[b].Np;
+        return c->chunks[a].n > c->chunks[b].n;
+    });
+    const size_t budget = (size_t)(c->scratch_gb * 1073741824.0);
+    std::vector<std::vector<int>> groups;
+    {
+        std::vector<int> cur;
+        size_t used = 0;
+        for (int id : order) {
+            const size_t bt = bytes(need(c->chunks[id]));
+            if (bt > budget) {
+                char msg[160];
+                std::snprintf(msg, sizeof msg, "qml_commit: chunk %d needs %.2f GB, above the %.2f GB scratch budget",
+                              id, bt / 1073741824.0, c->scratch_gb);
+                return fail(QML_ERR_MEMORY, msg);
+            }
+            if (!cur.empty() && (used + bt > budget || (int)cur.size() >= c->max_batch)) {
+                groups.push_back(cur);
+                cur.clear();
+                used = 0;
+            }
+            cur.push_back(id);
+            used += bt;
+        }
+        if (!cur.empty()) groups.push_back(cur);
+    }
+    Need mx{};
+    std::vector<std::vector<ChunkDev>> devs;
+    for (auto &g : groups) {
+        Need acc{};
+        std::vector<ChunkDev> cd;
+        for (int id : g) {
+            const auto &ch = c->chunks[id];
+            const Need n = need(ch);
+            ChunkDev d{};
+            d.N = ch.N;
+            d.Np = ch.Np;
+            d.nt = ch.Np / TILE;
+            d.n = ch.n;
+            d.m = ch.m;
+            d.kbu = ch.kbu;
+            d.nzq = ch.nzq;
+            d.zm0 = ch.zm0;
+            d.table = ch.info.res_index;
+            d.nu = ch.nu;
+            d.gidx = id;
+            d.ntu = d.nt * (d.nt + 1) / 2;
+            d.mt = (ch.m + TILE - 1) / TILE;
+            d.in_off = ch.in_off;
+            d.mat_off = (long long)acc.mat;
+            d.vec_off = (long long)acc.vec;
+            d.pj_off = (long long)acc.pj;
+            d.sj_off = (long long)acc.sj;
+            d.stk_off = (long long)acc.stk;
+            d.gram_off = (long long)acc.gram;
+            d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic;
+            acc.mat += n.mat; acc.vec += n.vec; acc.pj += n.pj; acc.sj += n.sj;
+            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram;
+            cd.push_back(d);
+        }
+        mx.mat = std::max(mx.mat, acc.mat); mx.vec = std::max(mx.vec, acc.vec);
+        mx.pj = std::max(mx.pj, acc.pj); mx.sj = std::max(mx.sj, acc.sj);
+        mx.stk = std::max(mx.stk, acc.stk); mx.lic = std::max(mx.lic, acc.lic);
+        mx.gram = std::max(mx.gram, acc.gram);
+        devs.push_back(cd);
+    }
+    HIPCHK(hipMalloc(&c->d_mat, std::max<size_t>(mx.mat, 1) * 8));
+    HIPCHK(hipMalloc(&c->d_vec, std::max<size_t>(mx.vec, 1) * 8));
+    HIPCHK(hipMalloc(&c->d_pj, std::max<size_t>(mx.pj, 1) * 8));
+    HIPCHK(hipMalloc(&c->d_sj, std::max<size_t>(mx.sj, 1) * 8));
+    HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
+    HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
+    HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
+    if (c->prec != 0) HIPCHK(hipMalloc(&c->d_lic, std::max<size_t>(mx.lic, 1) * es));
+    HIPCHK(hipMemset(c->d_vec, 0, std::max<size_t>(mx.vec, 1) * 8));
+
+    for (size_t bi = 0; bi < groups.size(); ++bi) {
+        Batch B;
+        B.ids = groups[bi];
+        const auto &cd = devs[bi];
+        std::vector<Work> wl, wr, wg;
+        for (int li = 0; li < (int)cd.size(); ++li) {
+            const ChunkDev &d = cd[li];
+            B.np_max = std::max(B.np_max, d.Np);
+            for (int j = 0; j < d.m; ++j)
+                for (int I = 0; I < d.nt; ++I)
+                    for (int J = 0; J < d.nt; ++J) wl.push_back({li, j, I, J});
+            for (int j = 0; j < d.m; ++j)
+                for (int I = 0; I < d.nt; ++I)
+                    for (int J = I; J < d.nt; ++J) wr.push_back({li, j, I, J});
+            for (int s = 0; s < d.ntu; ++s)
+                for (int I = 0; I < d.mt; ++I)
+                    for (int J = I; J < d.mt; ++J) wg.push_back({li, s, I, J});
+        }
+        B.nwl = (int)wl.size();
+        B.nwr = (int)wr.size();
+        B.nwg = (int)wg.size();
+        HIPCHK(hipMalloc(&B.d_chunks, cd.size() * sizeof(ChunkDev)));
+        HIPCHK(hipMemcpy(B.d_chunks, cd.data(), cd.size() * sizeof(ChunkDev), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&B.d_wl, std::max<size_t>(wl.size(), 1) * sizeof(Work)));
+        HIPCHK(hipMalloc(&B.d_wr, std::max<size_t>(wr.size(), 1) * sizeof(Work)));
+        HIPCHK(hipMalloc(&B.d_wg, std::max<size_t>(wg.size(), 1) * sizeof(Work)));
+        if (!wl.empty()) HIPCHK(hipMemcpy(B.d_wl, wl.data(), wl.size() * sizeof(Work), hipMemcpyHostToDevice));
+        if (!wr.empty()) HIPCHK(hipMemcpy(B.d_wr, wr.data(), wr.size() * sizeof(Work), hipMemcpyHostToDevice));
+        if (!wg.empty()) HIPCHK(hipMemcpy(B.d_wg, wg.data(), wg.size() * sizeof(Work), hipMemcpyHostToDevice));
+        for (auto &e : B.ev) HIPCHK(hipEventCreate(&e));
+        c->batches.push_back(B);
+    }
+    c->hF.assign((size_t)c->T * c->T, 0.0);
+    c->hdbt.assign((size_t)3 * c->T, 0.0);
+    c->committed = true;
+    return QML_OK;
+}
+
+int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt_sum) {
+    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_iterate: commit first");
+    if (!theta || !fisher_sum || !dbt_sum) return fail(QML_ERR_ARG, "qml_iterate: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int T = c->T;
+    HIPCHK(hipEventRecord(c->ev_begin, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_theta, theta, T * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->d_F, 0, (size_t)T * T * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->d_dbtsum, 0, (size_t)3 * T * sizeof(double), c->st));
+    for (auto &B : c->batches) {
+        LaunchCtx L;
+        L.stream = c->st;
+        L.chunks = B.d_chunks;
+        L.nchunks = (int)B.ids.size();
+        L.np_max = B.np_max;
+        L.in = c->d_in;
+        L.tables = c->d_tables;
+        L.bins = c->binsdev;
+        L.mat = c->d_mat;
+        L.vec = c->d_vec;
+        L.pj = c->d_pj;
+        L.sj = c->d_sj;
+        const void *li = (c->prec == 0) ? (const void *)c->d_mat : c->d_lic;
+        HIPCHK(hipEventRecord(B.ev[0], c->st));
+        launch_assemble(L, c->d_theta);
+        HIPCHK(hipEventRecord(B.ev[1], c->st));
+        launch_cholesky(L, c->d_status);
+        launch_trinv(L);
+        launch_solve(L);
+        HIPCHK(hipEventRecord(B.ev[2], c->st));
+        launch_dpass(L);
+        launch_small(L);
+        launch_xgen(L, c->d_X, c->prec);
+        if (c->prec != 0) launch_convert_li(L, c->d_lic);
+        launch_gemm_whiten_left(c->st, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
+        launch_gemm_whiten_right(c->st, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
+        HIPCHK(hipEventRecord(B.ev[3], c->st));
+        launch_gemm_gram(c->st, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
+        launch_gram_finalize(L, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
+        HIPCHK(hipEventRecord(B.ev[4], c->st));
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(c->ev_end, c->st));
+    launch_accumulate(c->st, c->d_outs, c->d_status, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
+                      c->d_dbtsum);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->hdbt.data(), c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < T; ++i)
+        for (int j = i; j < T; ++j) fisher_sum[(size_t)i * T + j] += c->hF[(size_t)i * T + j];
+    for (int k = 0; k < 3 * T; ++k) dbt_sum[k] += c->hdbt[k];
+    qml_timings tm{};
+    for (auto &B : c->batches) {
+        float a = 0, b = 0, d = 0, e = 0;
+        hipEventElapsedTime(&a, B.ev[0], B.ev[1]);
+        hipEventElapsedTime(&b, B.ev[1], B.ev[2]);
+        hipEventElapsedTime(&d, B.ev[2], B.ev[3]);
+        hipEventElapsedTime(&e, B.ev[3], B.ev[4]);
+        tm.assemble_ms += a;
+        tm.invert_ms += b;
+        tm.weight_ms += d;
+        tm.fisher_ms += e;
+    }
+    float tot = 0;
+    hipEventElapsedTime(&tot, c->ev_begin, c->ev_end);
+    tm.total_ms = tot;
+    tm.n_batches = (long long)c->batches.size();
+    c->tm = tm;
+    return QML_OK;
+}
+
+int qml_get_chunk_result(qml_ctx *c, int id, double *fisher, double *dbt, int *status) {
+    if (!c || !c->committed || id < 0 || id >= (int)c->chunks.size())
+        return fail(QML_ERR_ARG, "qml_get_chunk_result: bad chunk id or not committed");
+    HIPCHK(hipSetDevice(c->device));
+    const ChunkOut &o = c->outs[id];
+    if (fisher)
+        HIPCHK(hipMemcpy(fisher, c->d_fc + o.fc_off, (size_t)o.nq * o.nq * sizeof(double), hipMemcpyDeviceToHost));
+    if (dbt) HIPCHK(hipMemcpy(dbt, c->d_dbt + o.dbt_off, (size_t)3 * o.nq * sizeof(double), hipMemcpyDeviceToHost));
+    if (status) HIPCHK(hipMemcpy(status, c->d_status + id, sizeof(int), hipMemcpyDeviceToHost));
+    return QML_OK;
+}
+
+int qml_get_timings(qml_ctx *c, qml_timings *t) {
+    if (!c || !t) return fail(QML_ERR_ARG, "qml_get_timings: null argument");
+    *t = c->tm;
+    return QML_OK;
+}
+
+int qml_num_chunks(qml_ctx *c) { return c ? (int)c->chunks.size() : 0; }
+int qml_total_kz_bins(qml_ctx *c) { return c ? c->T : 0; }
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Shared host/device definitions of the QML engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qml {
+
+constexpr int TILE = 64;          // GEMM tile and padding granularity
+constexpr int MAX_NU = 4;         // continuum templates handled on the device
+constexpr int MAX_ZB = 64;        // redshift bins
+constexpr int MAX_NK = 64;        // k bands
+constexpr int VEC_SLOTS = 8 + 2 * MAX_NU;  // FP64 vectors of length Np per chunk
+
+constexpr double SPEED_OF_LIGHT = 299792.458;
+constexpr double LYA_REST = 1215.67;
+
+// Per-chunk input vectors, each Np long, at ChunkDev::in_off.
+enum InSlot { IN_WAVE = 0, IN_DELTA = 1, IN_IVAR = 2, IN_D = 3, IN_NPRIME = 4, IN_U0 = 5,
+              IN_SLOTS = 5 + MAX_NU };
+// Per-chunk scratch vectors, each Np long, at ChunkDev::vec_off.
+enum VecSlot { V_X = 0, V_Y = 1, V_T0 = 2, V_T1 = 3, V_T2 = 4, V_W0 = 5 };
+
+struct BinsDev {
+    double zc[MAX_ZB];
+    double dz;
+    int nz, nk, method;
+    int turn_off_sfid;
+};
+
+struct TableDev {
+    const double *q;   // nv x nk, k contiguous (interleaved for row gathers)
+    const double *s;   // nv x nz, z contiguous
+    int nv, nz;
+    double v1, dv, z1, dz;
+};
+
+// Per chunk, per batch.  Offsets are in elements of the arena they index.
+struct ChunkDev {
+    int N, Np, nt;       // pixels, padded size, Np / TILE
+    int n, m;            // stored Q count, stack members (n + 2: H and T)
+    int kbu, nzq, zm0;   // KBIN_UPP, z bins of the stored Q, first z bin
+    int table, nu;       // table index, templates in use
+    int gidx;            // chunk id
+    int ntu;             // upper TILE x TILE tiles of Np x Np
+    int mt;              // member tiles, ceil(m / TILE)
+    long long in_off;    // into inputs (doubles)
+    long long mat_off;   // into the FP64 matrix arena: A, L, Li, S, each Np*Np
+    long long vec_off;   // into the FP64 vector arena
+    long long stk_off;   // into the stack arenas (compute-type elements), m*Np*Np each
+    long long pj_off;    // into the P arena (doubles): (n+1) x Np x (1+MAX_NU)
+    long long sj_off;    // into the s arena (doubles): m x MAX_NU^2, then d (n)
+    long long gram_off;  // into the Gram partials (doubles)
+    long long lic_off;   // into the compute-type copy of Li
+};
+
+struct ChunkOut {        // per chunk id, persistent
+    long long fc_off;    // n_q x n_q
+    long long dbt_off;   // 3 x n_q
+    int nq, fis;
+};
+
+// GEMM work item: (local chunk, member or split, tile row, tile col)
+struct Work {
+    int c, j, I, J;
+};
+
+// ---------------------------------------------------------------- device ---
+
+// DiscreteInterpolation1D/2D index rule (mathtools/discrete_interpolation.cpp
+// :61-72,221-242): truncation toward zero, clamped to [0, n-2].
+__device__ __forceinline__ int interp_index(double xx, int n) {
+    int i = (int)xx;
+    if (i < 0) i = 0;
+    else if (i >= n - 1) i = n - 2;
+    return i;
+}
+
+// Redshift bin weight (core/global_numbers.cpp:393-471), elementwise.
+__device__ __forceinline__ double zweight(const BinsDev &b, double z, int zm) {
+    const double zc = b.zc[zm], dz = b.dz;
+    if (b.method == 0 || b.nz == 1) {
+        return (z >= zc - dz / 2 && z <= zc + dz / 2) ? 1.0 : 0.0;
+    }
+    if (zm == 0) {
+        if (z < zc) return 1.0;
+        if (z <= zc + dz) return 1. - (z - zc) / dz;
+        return 0.0;
+    }
+    if (zm == b.nz - 1) {
+        if (z < zc - dz) return 0.0;
+        if (z <= zc) return 1. - (zc - z) / dz;
+        return 1.0;
+    }
+    if (z >= zc - dz && z <= zc + dz) return 1. - fabs(z - zc) / dz;
+    return 0.0;
+}
+
+// S_fid(z, v) by DiscreteInterpolation2D::evaluate with x = z (fast axis).
+__device__ __forceinline__ double signal_interp(const TableDev &t, double z, double v) {
+    const double xx = (z - t.z1) / t.dz, yy = (v - t.v1) / t.dv;
+    const int nx = interp_index(xx, t.nz), ny = interp_index(yy, t.nv);
+    const size_t ind = (size_t)nx + (size_t)t.nz * ny;
+    const double dnx = xx - nx, dny = yy - ny;
+    return t.s[ind] * (1 - dnx) * (1 - dny) + t.s[ind + 1] * (dnx) * (1 - dny)
+         + t.s[ind + t.nz] * (1 - dnx) * (dny) + t.s[ind + t.nz + 1] * (dnx) * (dny);
+}
+
+// v_ij and z_ij of Chunk::_setVZMatrices (core/chunk_estimate.cpp:384-398),
+// for any i, j (the reference fills j >= i and mirrors).
+__device__ __forceinline__ void vz_pair(double li, double lj, int i, int j, double &v, double &z) {
+    if (j >= i) {
+        v = SPEED_OF_LIGHT * log(lj / li);
+        z = sqrt(li * lj) - 1.;
+    } else {
+        v = SPEED_OF_LIGHT * log(li / lj);
+        z = sqrt(lj * li) - 1.;
+    }
+}
+
+}  // namespace qml

@@ -1,0 +1,47 @@
+// Launch interface between the host orchestration and the kernels.
+#pragma once
+
+#include "qml_common.hpp"
+
+namespace qml {
+
+struct LaunchCtx {
+    hipStream_t stream;
+    const ChunkDev *chunks;   // device, nchunks entries
+    int nchunks;
+    int np_max;               // largest Np in the batch
+    const double *in;         // per-chunk input vectors
+    const TableDev *tables;   // device table descriptors
+    BinsDev bins;
+    double *mat;              // FP64 matrix arena
+    double *vec;              // FP64 vector arena
+    double *pj;               // P arena
+    double *sj;               // s_j and d arena
+};
+
+// kernels_chunk.hip
+void launch_assemble(const LaunchCtx &L, const double *theta);
+void launch_cholesky(const LaunchCtx &L, int *status);
+void launch_trinv(const LaunchCtx &L);
+void launch_solve(const LaunchCtx &L);
+void launch_dpass(const LaunchCtx &L);
+void launch_small(const LaunchCtx &L);
+void launch_xgen(const LaunchCtx &L, void *X, int prec);
+void launch_convert_li(const LaunchCtx &L, void *lic);
+void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double *gram, const int *status,
+                          double *fc, double *dbt);
+void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, int nchunks, int T,
+                       const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
+
+// kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3.
+// Y_j = Li X_j for every member (lower-triangular Li).
+void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                             const void *li, const void *X, void *Y);
+// G_j = Y_j Li^T on upper tiles, written tile-packed.
+void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                              const void *li, const void *Y, void *G);
+// Gram partials E_s = sum over G tile s of G_j G_j'.
+void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+                      const void *G, double *gram);
+
+}  // namespace qml

@@ -1,0 +1,308 @@
+"""ctypes binding of ``libqmlhip.so`` (include/lyspeq_qml.h).
+
+``QMLEngine`` is the Python face of the C ABI; ``Chunk`` mirrors the
+reference's class of the same name (core/chunk_estimate.hpp:46-119): its
+constructor registers a chunk and raises the reference's errors, and
+``QMLEngine.iterate`` runs what every Chunk::oneQSOiteration of one
+Newton-Raphson iteration does, accumulating into the caller's fisher_sum and
+dbt_sum.  There is no CPU fallback: without the built library, or without a
+GPU, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqmlhip.so")
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+QML_OK = 0
+QML_ERR_ARG, QML_ERR_HIP, QML_ERR_SHORT_CHUNK, QML_ERR_NO_TABLE = -1, -2, -3, -4
+QML_ERR_EMPTY, QML_ERR_MEMORY, QML_ERR_LIMIT = -5, -6, -7
+
+
+class QMLError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ShortChunkError(QMLError):
+    """std::runtime_error("Short chunk ...") of core/chunk_estimate.cpp:172-187."""
+
+
+class TableNotFoundError(QMLError):
+    """std::out_of_range of core/chunk_estimate.cpp:198-206."""
+
+
+class EmptyChunkError(QMLError):
+    """io/qso_file.cpp:150-153."""
+
+
+_ERRORS = {QML_ERR_SHORT_CHUNK: ShortChunkError, QML_ERR_NO_TABLE: TableNotFoundError,
+           QML_ERR_EMPTY: EmptyChunkError}
+
+
+class _Opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("scratch_gb", C.c_double),
+                ("max_batch_chunks", C.c_int)]
+
+
+class _Bins(C.Structure):
+    _fields_ = [("nk", C.c_int), ("nz", C.c_int), ("kband_edges", C.POINTER(C.c_double)),
+                ("zbin_centers", C.POINTER(C.c_double)), ("z_bin_width", C.c_double),
+                ("z_binning_method", C.c_int), ("cont_logl_order", C.c_int),
+                ("cont_lam_order", C.c_int), ("turn_off_sfid", C.c_int),
+                ("max_forest_length_v", C.c_double), ("max_pixel_length_v", C.c_double)]
+
+
+class _ChunkIn(C.Structure):
+    _fields_ = [("wave", C.POINTER(C.c_double)), ("delta", C.POINTER(C.c_double)),
+                ("ivar", C.POINTER(C.c_double)), ("size", C.c_int), ("R_fwhm", C.c_int)]
+
+
+class ChunkInfo(C.Structure):
+    _fields_ = [("size", C.c_int), ("real_size", C.c_int), ("res_index", C.c_int),
+                ("kbin_upp", C.c_int), ("zbin", C.c_int), ("zbin_low", C.c_int),
+                ("zbin_upp", C.c_int), ("n_q_matrices", C.c_int),
+                ("fisher_index_start", C.c_int), ("n_stored", C.c_int),
+                ("n_templates", C.c_int), ("dv_kms", C.c_double), ("lower_z", C.c_double),
+                ("upper_z", C.c_double), ("median_z", C.c_double)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("assemble_ms", C.c_double), ("invert_ms", C.c_double),
+                ("weight_ms", C.c_double), ("fisher_ms", C.c_double),
+                ("accumulate_ms", C.c_double), ("total_ms", C.c_double),
+                ("n_batches", C.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
+           "qml_add_chunk", "qml_commit", "qml_iterate", "qml_get_chunk_result",
+           "qml_get_timings", "qml_num_chunks", "qml_total_kz_bins"]
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m lyspeq_amd.build`")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.qml_create.argtypes = [C.POINTER(vp), C.POINTER(_Opts)]
+    lib.qml_destroy.argtypes = [vp]
+    lib.qml_last_error.restype = C.c_char_p
+    lib.qml_set_bins.argtypes = [vp, C.POINTER(_Bins)]
+    lib.qml_add_table.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp, dp,
+                                  C.POINTER(C.c_int)]
+    lib.qml_add_chunk.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(ChunkInfo), C.POINTER(C.c_int)]
+    lib.qml_commit.argtypes = [vp]
+    lib.qml_iterate.argtypes = [vp, dp, dp, dp]
+    lib.qml_get_chunk_result.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_int)]
+    lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
+    lib.qml_num_chunks.argtypes = [vp]
+    lib.qml_total_kz_bins.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(code: int, lib):
+    if code != QML_OK:
+        msg = lib.qml_last_error().decode()
+        raise _ERRORS.get(code, QMLError)(code, msg)
+
+
+@dataclass
+class Binning:
+    """bins:: as set up by bins::setUpBins (core/global_numbers.cpp:209-251)
+    plus the specifics:: keys the chunk path reads."""
+    k0: float
+    nklin: int
+    dklin: float
+    nklog: int
+    dklog: float
+    z0: float
+    z_bin_width: float
+    nz: int
+    klast: float = -1.0
+    method: int = 1
+    cont_logl_order: int = 1
+    cont_lam_order: int = -1
+    turn_off_sfid: bool = False
+    max_forest_length_v: float = 3.5e4
+    number_of_chunks: int = 1
+    max_pixel_length_v: float = 150.0
+
+    def kband_edges(self) -> np.ndarray:
+        nk = self.nklin + self.nklog
+        ktemp = (self.k0 + self.dklin * self.nklin) * math.pow(10, self.nklog * self.dklog)
+        edges = [self.k0 + self.dklin * i for i in range(self.nklin + 1)]
+        for i in range(1, self.nklog + 1):
+            edges.append(edges[self.nklin] * math.pow(10.0, i * self.dklog))
+        if self.klast > ktemp:
+            edges.append(self.klast)
+        return np.array(edges)
+
+    def kband_centers(self) -> np.ndarray:
+        e = self.kband_edges()
+        return (e[:-1] + e[1:]) / 2.0
+
+    def zbin_centers(self) -> np.ndarray:
+        return np.array([self.z0 + self.z_bin_width * m for m in range(self.nz)])
+
+    @property
+    def nk(self) -> int:
+        return len(self.kband_edges()) - 1
+
+    @property
+    def total_kz_bins(self) -> int:
+        return self.nk * self.nz
+
+    @property
+    def z_lower_edge(self) -> float:
+        return self.zbin_centers()[0] - self.z_bin_width / 2.0
+
+    @property
+    def z_upper_edge(self) -> float:
+        return self.zbin_centers()[-1] + self.z_bin_width / 2.0
+
+
+class QMLEngine:
+    """One device context holding tables and chunks for repeated iterations."""
+
+    def __init__(self, binning: Binning, precision="fp64", device: int = 0,
+                 scratch_gb: float = 0.0, max_batch_chunks: int = 0):
+        self._lib = load_library()
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        opts = _Opts(device, prec, scratch_gb, max_batch_chunks)
+        self._ctx = C.c_void_p()
+        _check(self._lib.qml_create(C.byref(self._ctx), C.byref(opts)), self._lib)
+        self.binning = binning
+        self.precision = prec
+        self._edges = np.ascontiguousarray(binning.kband_edges(), dtype=np.float64)
+        self._zc = np.ascontiguousarray(binning.zbin_centers(), dtype=np.float64)
+        b = _Bins(len(self._edges) - 1, binning.nz, _dptr(self._edges), _dptr(self._zc),
+                  binning.z_bin_width, binning.method, binning.cont_logl_order,
+                  binning.cont_lam_order, int(bool(binning.turn_off_sfid)),
+                  binning.max_forest_length_v / max(1, binning.number_of_chunks),
+                  binning.max_pixel_length_v)
+        _check(self._lib.qml_set_bins(self._ctx, C.byref(b)), self._lib)
+        self.T = len(self._edges) - 1
+        self.T *= binning.nz
+        self.tables = []
+        self.chunk_infos: List[ChunkInfo] = []
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.qml_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_table(self, R_fwhm: int, dv_kms: float, deriv: np.ndarray, signal: Optional[np.ndarray],
+                  v_length: float) -> int:
+        deriv = np.ascontiguousarray(deriv, dtype=np.float64)
+        nk, nv = deriv.shape
+        if nk != self.binning.nk:
+            raise ValueError("derivative table has the wrong number of k bands")
+        if signal is not None:
+            signal = np.ascontiguousarray(signal, dtype=np.float64)
+            nz = signal.shape[1]
+            sp = _dptr(signal)
+        else:
+            nz, sp = 2, None
+        r = C.c_int(-1)
+        _check(self._lib.qml_add_table(self._ctx, int(R_fwhm), float(dv_kms), nv, float(v_length), nz,
+                                       _dptr(deriv), sp, C.byref(r)), self._lib)
+        self.tables.append((int(R_fwhm), float(dv_kms)))
+        return r.value
+
+    def add_chunk(self, wave, delta, ivar, R_fwhm: int):
+        w = np.ascontiguousarray(wave, dtype=np.float64)
+        d = np.ascontiguousarray(delta, dtype=np.float64)
+        iv = np.ascontiguousarray(ivar, dtype=np.float64)
+        cin = _ChunkIn(_dptr(w), _dptr(d), _dptr(iv), w.size, int(R_fwhm))
+        info = ChunkInfo()
+        cid = C.c_int(-1)
+        _check(self._lib.qml_add_chunk(self._ctx, C.byref(cin), C.byref(info), C.byref(cid)), self._lib)
+        self.chunk_infos.append(info)
+        return cid.value, info
+
+    def commit(self):
+        _check(self._lib.qml_commit(self._ctx), self._lib)
+
+    def iterate(self, theta: np.ndarray, fisher_sum: np.ndarray, dbt_sum: np.ndarray):
+        """Adds every chunk's F (upper) and d, b, t into the given arrays."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        assert th.size == self.T and fisher_sum.shape == (self.T, self.T) and dbt_sum.shape == (3, self.T)
+        assert fisher_sum.flags.c_contiguous and dbt_sum.flags.c_contiguous
+        _check(self._lib.qml_iterate(self._ctx, _dptr(th), _dptr(fisher_sum), _dptr(dbt_sum)), self._lib)
+
+    def chunk_result(self, cid: int):
+        nq = self.chunk_infos[cid].n_q_matrices
+        F = np.zeros((nq, nq))
+        dbt = np.zeros((3, nq))
+        st = C.c_int(0)
+        _check(self._lib.qml_get_chunk_result(self._ctx, cid, _dptr(F), _dptr(dbt), C.byref(st)), self._lib)
+        return F, dbt, st.value
+
+    def timings(self) -> dict:
+        t = Timings()
+        _check(self._lib.qml_get_timings(self._ctx, C.byref(t)), self._lib)
+        return t.as_dict()
+
+
+class Chunk:
+    """Chunk(qmaster, i1, i2) of core/chunk_estimate.cpp:141-167: registers
+    pixels [i1, i2) of a spectrum with the engine.  Raises ShortChunkError,
+    TableNotFoundError or EmptyChunkError like the reference constructor."""
+
+    def __init__(self, engine: QMLEngine, wave, delta, ivar, R_fwhm: int, i1: int = 0,
+                 i2: Optional[int] = None):
+        i2 = len(wave) if i2 is None else i2
+        self.engine = engine
+        self.id, info = engine.add_chunk(np.asarray(wave)[i1:i2], np.asarray(delta)[i1:i2],
+                                         np.asarray(ivar)[i1:i2], R_fwhm)
+        self.info = info
+        self.fisher_index_start = info.fisher_index_start
+        self.N_Q_MATRICES = info.n_q_matrices
+        self.KBIN_UPP = info.kbin_upp
+        self.ZBIN, self.ZBIN_LOW, self.ZBIN_UPP = info.zbin, info.zbin_low, info.zbin_upp
+
+    def size(self) -> int:
+        return self.info.size
+
+    def realSize(self) -> int:
+        return self.info.real_size
+
+    @property
+    def fisher_matrix(self) -> np.ndarray:
+        return self.engine.chunk_result(self.id)[0]
+
+    @property
+    def dbt_estimate_before_fisher_vector(self) -> np.ndarray:
+        return self.engine.chunk_result(self.id)[1]

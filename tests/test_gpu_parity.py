@@ -1,0 +1,126 @@
+"""HIP engine vs the CPU oracle on the same chunks (needs an MI355X)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import goldencase as gc
+import parity as P
+from lyspeq_amd import synth
+from lyspeq_amd.qml import Binning, QMLEngine
+
+pytestmark = pytest.mark.gpu
+
+# FP64 path: the Cholesky/whitened form vs the oracle's LU form; the bar is
+# relative 1e-10 against the largest entry of each quantity.
+FP64_TOL = 1e-10
+
+
+def _golden_workload(**over):
+    kw = dict(gc.BINS_KW)
+    kw.update(over)
+    b = Binning(**kw)
+    t = gc.table()
+    sp = gc.spectrum()
+    w = synth.Workload("golden", b, nv=t.nvpoints, v_length=t.v_length, nz_table=t.nzpoints,
+                       pd13=gc.PD13)
+    w.pairs = [(70000, 20.0)]
+    w.deriv = t.deriv[None]
+    w.signal = t.signal[None]
+    w.chunks.append(dict(wave=sp.wave, delta=sp.flux, ivar=sp.ivar, R=sp.R_fwhm, dv_table=20.0))
+    return w
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def test_golden_chunk_fp64_matches_reference_goldens():
+    _print_case("golden chunk fp64")
+    w = _golden_workload()
+    T = w.binning.total_kz_bins
+    eng, out, F, dbt = P.run_engine(w, np.zeros(T), [0])
+    from oracle.driver import power_estimates
+    res = power_estimates(F, dbt)
+    assert gc.all_close(res["fisher"], gc.golden_fisher())
+    rows = gc.detailed_rows()
+    assert gc.all_close(rows[:, 5], res["theta"])
+    assert gc.all_close(rows[:, 7], res["sigma"])
+    for i in range(3):
+        assert gc.all_close(rows[:, 11 + i], dbt[i])
+    oout, oF, od = P.run_oracle(w, np.zeros(T), [0])
+    assert P.rel_err(F, oF) < FP64_TOL
+    for i in range(3):
+        assert P.rel_err(dbt[i], od[i]) < FP64_TOL
+
+
+@pytest.mark.parametrize("orders", [(1, -1), (2, 1)])
+def test_golden_chunk_marginalised_masked_theta_fp64(orders):
+    _print_case(f"golden chunk marginalised {orders}")
+    w = _golden_workload(cont_logl_order=orders[0], cont_lam_order=orders[1])
+    rng = np.random.default_rng(5)
+    c = w.chunks[0]
+    c["ivar"] = c["ivar"].copy()
+    c["ivar"][1 + rng.choice(c["ivar"].size - 2, 30, replace=False)] = 0.0
+    th = P.theta_perturbed(w.binning, gc.PD13, 0.1, 3)
+    eng, out, F, dbt = P.run_engine(w, th, [0])
+    oout, oF, od = P.run_oracle(w, th, [0])
+    assert P.rel_err(F, oF) < FP64_TOL
+    for i in range(3):
+        assert P.rel_err(dbt[i], od[i]) < FP64_TOL
+
+
+def test_cpu_config_64_chunks_fp64():
+    _print_case("cpu-64x256 fp64")
+    w = synth.cpu_reference_config(nchunks=64)
+    T = w.binning.total_kz_bins
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, np.zeros(T), idx)
+    oout, oF, od = P.run_oracle(w, np.zeros(T), idx)
+    assert set(out) == set(oout)
+    worst = 0.0
+    for i in idx:
+        Fg, dg, okg = out[i]
+        Fo, do, oko = oout[i][:3]
+        assert okg and oko
+        worst = max(worst, P.rel_err(Fg, Fo), *(P.rel_err(dg[k], do[k]) for k in range(3)))
+    print("cpu-64x256 worst per-chunk relative error", worst)
+    assert worst < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+
+
+def test_desi_sample_fp64_multiple_tables_and_zbins():
+    _print_case("desi sample fp64")
+    w = synth.desi_config(nchunks=24)
+    T = w.binning.total_kz_bins
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 11)
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, th, idx, max_batch_chunks=7)
+    oout, oF, od = P.run_oracle(w, th, idx)
+    assert set(out) == set(oout) and len(out) >= 20
+    worst = 0.0
+    for i in out:
+        worst = max(worst, P.rel_err(out[i][0], oout[i][0]),
+                    *(P.rel_err(out[i][1][k], oout[i][1][k]) for k in range(3)))
+    print("desi sample worst per-chunk relative error", worst)
+    assert worst < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+    for k in range(3):
+        assert P.rel_err(dbt[k], od[k]) < FP64_TOL
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("bf16x3", 2e-3)])
+def test_mixed_precision_against_fp64(precision, tol):
+    _print_case(f"desi sample {precision}")
+    w = synth.desi_config(nchunks=24)
+    T = w.binning.total_kz_bins
+    idx = list(range(len(w.chunks)))
+    _, _, F64, d64 = P.run_engine(w, np.zeros(T), idx, precision="fp64")
+    _, _, Fm, dm = P.run_engine(w, np.zeros(T), idx, precision=precision)
+    ef = P.fisher_norm_err(Fm, F64)
+    eb = P.rel_err(dm[1], d64[1])
+    et = P.rel_err(dm[2], d64[2])
+    print(f"{precision}: F {ef:.2e} b {eb:.2e} t {et:.2e} d {P.rel_err(dm[0], d64[0]):.2e}")
+    assert P.rel_err(dm[0], d64[0]) < 1e-12        # d_k stays FP64
+    assert ef < tol and eb < tol and et < tol

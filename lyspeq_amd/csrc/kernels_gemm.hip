@@ -289,6 +289,9 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
 
     const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     const int wm = w / 2, wn = w % 2;
+    // row of accumulator element u inside a 16x16 block: the FP64 MFMA
+    // holds rows u*4 + lane/16, the FP32-output MFMAs rows 4*(lane/16) + u
+    auto brow = [&](int u) { return (PREC == 0) ? (4 * u + lane / 16) : (4 * (lane / 16) + u); };
     if (MODE == MODE_LEFT) {
         T *C = (T *)Cbase + c.stk_off + wk.j * msz;
 #pragma unroll
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int r = I0 + wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int r = I0 + wm * 32 + x * 16 + brow(u);
                     const int q = J0 + wn * 32 + y * 16 + lane % 16;
                     C[(size_t)r * Np + q] = (T)acc[x][y][u];
                 }
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int r = wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int r = wm * 32 + x * 16 + brow(u);
                     const int q = wn * 32 + y * 16 + lane % 16;
                     C[r * TILE + q] = (T)acc[x][y][u];
                 }
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int r = wm * 32 + x * 16 + 4 * (lane / 16) + u;
+                    const int r = wm * 32 + x * 16 + brow(u);
                     const int q = wn * 32 + y * 16 + lane % 16;
                     C[r * TILE + q] = wgt * (double)acc[x][y][u];
                 }

@@ -1,0 +1,222 @@
+"""Benchmark of the per-chunk QML step on one MI355X.
+
+One step = one Newton-Raphson iteration's chunk loop over the whole batch
+(every chunk's covariance, factorisation, marginalisation, d/b/t and Fisher,
+accumulated into the global F and d/b/t on the host), with all inputs and
+tables already resident in device memory.  The headline value is the
+mixed-precision (split-BF16x3 MFMA) throughput on the DESI-like batch; the
+FP64 path runs on the same batch for the speed-up and the error figures.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from lyspeq_amd import synth                                   # noqa: E402
+from lyspeq_amd.driver import power_estimates                  # noqa: E402
+from lyspeq_amd.qml import QMLEngine, KERNEL_NAMES             # noqa: E402
+
+METRIC = "chunks/s (F,d,b) mixed-prec vs FP64 on 1 MI355X; max rel err of F and P1D"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def chunk_shapes(eng):
+    out = []
+    for info in eng.chunk_infos:
+        out.append((info.size, info.n_stored, info.n_templates))
+    return out
+
+
+def algorithmic_flops(shapes):
+    """FLOP per chunk of the reference algorithm:
+    2nN^3 (W_k) + 2N^3 (S_fid C^-1) + N^3 (inverse) + n(n+1)N^2 (Fisher)
+    + 8nN^2 (assembly, d, b, t) + 4 n_t N^2 (marginalisation)."""
+    tot = 0.0
+    for N, n, nt in shapes:
+        N = float(N)
+        tot += 2 * n * N ** 3 + 2 * N ** 3 + N ** 3 + n * (n + 1) * N ** 2 + 8 * n * N ** 2 + 4 * nt * N ** 2
+    return tot
+
+
+def kernel_work(shapes, prec):
+    """Algorithmic work of each dense kernel at the real chunk sizes
+    (m = n + 2 stack members: the stored Q_k, the noise and the signal)."""
+    es = 8 if prec == "fp64" else 4
+    w = {"whiten_left": 0.0, "whiten_right": 0.0, "gram": 0.0, "xgen_bytes": 0.0}
+    for N, n, nt in shapes:
+        m = n + 2
+        N = float(N)
+        w["whiten_left"] += m * N * N * (N + 1)               # lower-triangular x dense
+        w["whiten_right"] += m * N * (N + 1) * (N + 2) / 3.0  # upper half, triangular operand
+        w["gram"] += m * (m + 1) / 2.0 * N * (N + 1)          # symmetric pairs over packed halves
+        w["xgen_bytes"] += m * N * N * es                      # projected Q stack written once
+    return w
+
+
+def run_engine(w, precision, steps, warmup, scratch_gb):
+    eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
+    t0 = time.perf_counter()
+    accepted = synth.load_engine(eng, w)
+    eng.commit()
+    log(f"[{precision}] {len(accepted)} chunks registered and committed in {time.perf_counter() - t0:.1f} s")
+    T = eng.T
+    theta = np.zeros(T)
+    for i in range(warmup):
+        F, d = np.zeros((T, T)), np.zeros((3, T))
+        eng.iterate(theta, F, d)
+        log(f"[{precision}] warmup {i + 1}/{warmup} done")
+    times = []
+    for i in range(steps):
+        F, d = np.zeros((T, T)), np.zeros((3, T))
+        t0 = time.perf_counter()
+        eng.iterate(theta, F, d)
+        times.append(time.perf_counter() - t0)
+        log(f"[{precision}] step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms")
+    tm = eng.timings()
+    shapes = chunk_shapes(eng)
+    eng.close()
+    return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes)
+
+
+def cpu_baseline(w, budget_s=20.0, max_chunks=48):
+    """The CPU oracle (numpy + threaded LAPACK/BLAS) on the first chunks of
+    the same batch, until budget_s of CPU work is spent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import parity as P
+    from oracle.chunk import Chunk as OChunk, ChunkError
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    b = P.oracle_bins(w.binning)
+    tabs = P.oracle_tables(w)
+    T = b.total_kz_bins
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    th = np.zeros(T)
+    done = 0
+    t0 = time.perf_counter()
+    for c in w.chunks[:max_chunks]:
+        try:
+            ch = OChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+        except ChunkError:
+            continue
+        ch.one_iteration(th, F, d)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=done / el, unit="chunks/s", cores=int(threads), kind="port",
+                sample=f"first {done} chunks of the same batch, FP64 LU oracle, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--config", default="desi", choices=list(synth.CONFIGS))
+    ap.add_argument("--nchunks", type=int, default=0)
+    ap.add_argument("--scratch-gb", type=float, default=96.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU sharding of chunks is described in "
+              "DESIGN.md but not built, so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    kw = {"nchunks": args.nchunks} if args.nchunks else {}
+    w = synth.CONFIGS[args.config](**kw)
+    log(f"workload {w.name}: {len(w.chunks)} chunks, {len(w.pairs)} (R,dv) tables, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+
+    res64 = None
+    if not args.no_fp64:
+        res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb)
+    resm = run_engine(w, args.precision, args.steps, args.warmup, args.scratch_gb)
+
+    nch = resm["n"]
+    ms = 1e3 * float(np.mean(resm["times"]))
+    value = nch / (ms / 1e3)
+    cfg = {"workload": f"{w.name}: {w.description}", "chunks": nch, "tables": len(w.pairs),
+           "precision": args.precision, "theta": "0 (first iteration)"}
+    if res64:
+        ms64 = 1e3 * float(np.mean(res64["times"]))
+        cfg["fp64_chunks_per_s"] = nch / (ms64 / 1e3)
+        cfg["fp64_ms_per_step"] = ms64
+        cfg["speedup_vs_fp64"] = ms64 / ms
+        F64, d64 = res64["F"], res64["dbt"]
+        Fm, dm = resm["F"], resm["dbt"]
+        dF = np.sqrt(np.abs(np.diag(F64)))
+        den = np.outer(dF, dF)
+        iu = np.triu_indices(F64.shape[0])
+        msk = den[iu] > 0
+        errs = {"F_normalized": float(np.max(np.abs(Fm - F64)[iu][msk] / den[iu][msk]))}
+        big = np.abs(F64[iu]) > 1e-6 * np.abs(F64).max()
+        errs["F_elementwise"] = float(np.max(np.abs(Fm - F64)[iu][big] / np.abs(F64[iu][big])))
+        for k, name in enumerate("dbt"):
+            errs[name] = float(np.max(np.abs(dm[k] - d64[k])) / max(np.abs(d64[k]).max(), 1e-300))
+        p64 = power_estimates(F64, d64)
+        pm = power_estimates(Fm, dm)
+        ok = np.isfinite(p64["sigma"]) & (p64["sigma"] > 0)
+        dev = np.abs(pm["theta"] - p64["theta"])[ok] / p64["sigma"][ok]
+        errs["P1D_max_dev_over_sigma"] = float(dev.max())
+        errs["P1D_rel"] = float(np.max(np.abs(pm["theta"] - p64["theta"])[ok]) /
+                                np.max(np.abs(p64["theta"][ok])))
+        cfg["max_rel_err_vs_fp64"] = errs
+        log("errors vs fp64:", json.dumps(errs))
+        log("fp64 kernel ms:", json.dumps({k: round(v, 2) for k, v in res64["timings"]["kernel_ms"].items()}))
+    log(f"{args.precision} kernel ms:", json.dumps({k: round(v, 2) for k, v in resm["timings"]["kernel_ms"].items()}))
+
+    # roofline of the dominant kernel of the headline (mixed) run
+    km = resm["timings"]["kernel_ms"]
+    nb = max(1, resm["timings"]["n_batches"])
+    dom = max(km, key=lambda k: km[k])
+    work = kernel_work(resm["shapes"], args.precision)
+    if dom in ("whiten_left", "whiten_right", "gram"):
+        ach = work[dom] / (km[dom] / 1e3)
+        roof = {"kernel": dom, "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
+                "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": None,
+                "launches": nb, "avg_launch_ms": km[dom] / nb}
+    else:
+        byt = work["xgen_bytes"] if dom == "xgen" else None
+        ach = (byt / (km[dom] / 1e3)) if byt else None
+        roof = {"kernel": dom, "bound": "hbm", "achieved": (ach / 1e9) if ach else None, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": (ach / HBM_PEAK) if ach else None, "traffic": None,
+                "launches": nb, "avg_launch_ms": km[dom] / nb}
+    step_flops = algorithmic_flops(resm["shapes"])
+    cfg["step_algorithmic_tflop"] = step_flops / 1e12
+    cfg["step_tflops_achieved"] = step_flops / (ms / 1e3) / 1e12
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w)
+        log("cpu baseline:", json.dumps(cpu))
+
+    out = {"metric": METRIC, "value": value, "unit": "chunks/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": args.precision, "data": "synthetic", "config": cfg,
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

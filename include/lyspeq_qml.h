@@ -110,9 +110,25 @@ typedef struct {
     double weight_ms;     /* d_k pass, projected Q_k generation, W products (T_Qmod) */
     double fisher_ms;     /* Gram contraction and reductions (T_F) */
     double accumulate_ms; /* cross-chunk accumulation and copy-out */
-    double total_ms;      /* host submit to host arrays updated */
+    double total_ms;      /* first to last device event of the iteration */
     long long n_batches;
+    /* device time per kernel, summed over batches, indexed by QML_K_* */
+    double kernel_ms[12];
 } qml_timings;
+
+/* Indices into qml_timings.kernel_ms. */
+#define QML_K_ASSEMBLE 0
+#define QML_K_CHOLESKY 1
+#define QML_K_TRINV 2
+#define QML_K_SOLVE 3
+#define QML_K_DPASS 4
+#define QML_K_SMALL 5
+#define QML_K_XGEN 6
+#define QML_K_WHITEN_LEFT 7
+#define QML_K_WHITEN_RIGHT 8
+#define QML_K_GRAM 9
+#define QML_K_FINALIZE 10
+#define QML_K_ACCUMULATE 11
 
 /* Context lifetime.  qml_create selects the device and creates a stream. */
 int qml_create(qml_ctx **ctx, const qml_opts *opts);

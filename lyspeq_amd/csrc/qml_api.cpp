@@ -54,7 +54,7 @@ struct Batch {
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
     int np_max = 0;
-    hipEvent_t ev[6]{};
+    hipEvent_t ev[13]{};
 };
 
 double median_sorted(const std::vector<double> &a) {
@@ -135,7 +135,7 @@ struct qml_ctx {
     void *d_X = nullptr, *d_Y = nullptr, *d_lic = nullptr;
     double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
     std::vector<Batch> batches;
-    hipEvent_t ev_begin{}, ev_end{};
+    hipEvent_t ev_begin{}, ev_end{}, ev_acc{};
     qml_timings tm{};
     std::vector<double> hF, hdbt;
 };
@@ -166,7 +166,8 @@ int qml_create(qml_ctx **out, const qml_opts *opts) {
         return fail(QML_ERR_ARG, "qml_create: device ordinal out of range");
     }
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
-        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+        hipEventCreate(&c->ev_acc) != hipSuccess) {
         delete c;
         return fail(QML_ERR_HIP, "qml_create: stream/event creation failed");
     }
@@ -188,6 +189,7 @@ int qml_destroy(qml_ctx *c) {
     for (void *p : ptrs) if (p) hipFree(p);
     if (c->ev_begin) hipEventDestroy(c->ev_begin);
     if (c->ev_end) hipEventDestroy(c->ev_end);
+    if (c->ev_acc) hipEventDestroy(c->ev_acc);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
     return QML_OK;
@@ -614,25 +616,33 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         launch_assemble(L, c->d_theta);
         HIPCHK(hipEventRecord(B.ev[1], c->st));
         launch_cholesky(L, c->d_status);
-        launch_trinv(L);
-        launch_solve(L);
         HIPCHK(hipEventRecord(B.ev[2], c->st));
+        launch_trinv(L);
+        HIPCHK(hipEventRecord(B.ev[3], c->st));
+        launch_solve(L);
+        HIPCHK(hipEventRecord(B.ev[4], c->st));
         launch_dpass(L);
+        HIPCHK(hipEventRecord(B.ev[5], c->st));
         launch_small(L);
+        HIPCHK(hipEventRecord(B.ev[6], c->st));
         launch_xgen(L, c->d_X, c->prec);
         if (c->prec != 0) launch_convert_li(L, c->d_lic);
+        HIPCHK(hipEventRecord(B.ev[7], c->st));
         launch_gemm_whiten_left(c->st, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
+        HIPCHK(hipEventRecord(B.ev[8], c->st));
         launch_gemm_whiten_right(c->st, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
-        HIPCHK(hipEventRecord(B.ev[3], c->st));
+        HIPCHK(hipEventRecord(B.ev[9], c->st));
         launch_gemm_gram(c->st, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
+        HIPCHK(hipEventRecord(B.ev[10], c->st));
         launch_gram_finalize(L, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
-        HIPCHK(hipEventRecord(B.ev[4], c->st));
+        HIPCHK(hipEventRecord(B.ev[11], c->st));
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipEventRecord(c->ev_end, c->st));
+    HIPCHK(hipEventRecord(c->ev_acc, c->st));
     launch_accumulate(c->st, c->d_outs, c->d_status, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
                       c->d_dbtsum);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev_end, c->st));
     HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(c->hdbt.data(), c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -641,16 +651,23 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
     for (int k = 0; k < 3 * T; ++k) dbt_sum[k] += c->hdbt[k];
     qml_timings tm{};
     for (auto &B : c->batches) {
-        float a = 0, b = 0, d = 0, e = 0;
-        hipEventElapsedTime(&a, B.ev[0], B.ev[1]);
-        hipEventElapsedTime(&b, B.ev[1], B.ev[2]);
-        hipEventElapsedTime(&d, B.ev[2], B.ev[3]);
-        hipEventElapsedTime(&e, B.ev[3], B.ev[4]);
-        tm.assemble_ms += a;
-        tm.invert_ms += b;
-        tm.weight_ms += d;
-        tm.fisher_ms += e;
+        for (int k = 0; k < 11; ++k) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, B.ev[k], B.ev[k + 1]);
+            tm.kernel_ms[k] += ms;
+        }
     }
+    {
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev_acc, c->ev_end);
+        tm.kernel_ms[QML_K_ACCUMULATE] = ms;
+        tm.accumulate_ms = ms;
+    }
+    tm.assemble_ms = tm.kernel_ms[QML_K_ASSEMBLE];
+    tm.invert_ms = tm.kernel_ms[QML_K_CHOLESKY] + tm.kernel_ms[QML_K_TRINV] + tm.kernel_ms[QML_K_SOLVE];
+    tm.weight_ms = tm.kernel_ms[QML_K_DPASS] + tm.kernel_ms[QML_K_SMALL] + tm.kernel_ms[QML_K_XGEN] +
+                   tm.kernel_ms[QML_K_WHITEN_LEFT] + tm.kernel_ms[QML_K_WHITEN_RIGHT];
+    tm.fisher_ms = tm.kernel_ms[QML_K_GRAM] + tm.kernel_ms[QML_K_FINALIZE];
     float tot = 0;
     hipEventElapsedTime(&tot, c->ev_begin, c->ev_end);
     tm.total_ms = tot;

@@ -82,10 +82,16 @@ class Timings(C.Structure):
     _fields_ = [("assemble_ms", C.c_double), ("invert_ms", C.c_double),
                 ("weight_ms", C.c_double), ("fisher_ms", C.c_double),
                 ("accumulate_ms", C.c_double), ("total_ms", C.c_double),
-                ("n_batches", C.c_longlong)]
+                ("n_batches", C.c_longlong), ("kernel_ms", C.c_double * 12)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "kernel_ms"}
+        d["kernel_ms"] = {name: self.kernel_ms[i] for i, name in enumerate(KERNEL_NAMES)}
+        return d
+
+
+KERNEL_NAMES = ["assemble", "cholesky", "trinv", "solve", "dpass", "small", "xgen",
+                "whiten_left", "whiten_right", "gram", "finalize", "accumulate"]
 
 
 _lib = None

@@ -17,11 +17,29 @@
 namespace qml {
 
 // ---------------------------------------------------------------- assembly --
+// Theta-contracted derivative tables, per chunk and z bin of its stored Q:
+// Th_m[iv] = sum_kn theta[(zm0+m) nk + kn] T[iv][kn], so that
+// sum_k theta_k Q_k(i,j) = sum_m w_m(z_ij) ((1-f) Th_m[iv] + f Th_m[iv+1]).
+__global__ __launch_bounds__(256) void k_theta_tables(const ChunkDev *cs, const TableDev *tabs, BinsDev bins,
+                                                      const double *theta, double *aux) {
+    const ChunkDev c = cs[blockIdx.y];
+    const int iv = blockIdx.x * 256 + threadIdx.x;
+    if (iv >= c.nvu) return;
+    const TableDev t = tabs[c.table];
+    const double *row = t.q + (size_t)iv * bins.nk;
+    for (int m = 0; m < c.nzq; ++m) {
+        const double *th = theta + (size_t)(c.zm0 + m) * bins.nk;
+        double s = 0.0;
+        for (int kn = 0; kn < c.kbu; ++kn) s += th[kn] * row[kn];
+        aux[c.th_off + (size_t)m * c.nvu + iv] = s;
+    }
+}
+
 // A = I + D C D and S, C = S_fid + sum_k theta_k Q_k, on upper 32x32 tiles,
 // mirrored (core/chunk_estimate.cpp:384-536,668-698).
 __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const double *in,
                                                   const TableDev *tabs, BinsDev bins,
-                                                  const double *theta, double *mat) {
+                                                  const double *aux, double *mat) {
     const ChunkDev c = cs[blockIdx.y];
     const int n32 = c.Np / 32;
     const int I = blockIdx.x / n32, J = blockIdx.x % n32;
@@ -29,9 +47,9 @@ __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const doub
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)c.Np;
     const double *D = in + c.in_off + IN_D * (size_t)c.Np;
+    const double *th = aux + c.th_off;
     double *A = mat + c.mat_off;
     double *S = A + 3 * (size_t)c.Np * c.Np;
-    const int nk = bins.nk;
     for (int e = threadIdx.x; e < 1024; e += 256) {
         const int i = I * 32 + e / 32, j = J * 32 + e % 32;
         if (j < i) continue;
@@ -46,17 +64,12 @@ __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const doub
             const double xx = (v - t.v1) / t.dv;
             const int iv = interp_index(xx, t.nv);
             const double dn = xx - iv;
-            const double *r0 = t.q + (size_t)iv * nk, *r1 = r0 + nk;
             double cij = s;
             for (int iz = 0; iz < c.nzq; ++iz) {
-                const int zm = c.zm0 + iz;
-                const double w = zweight(bins, z, zm);
+                const double w = zweight(bins, z, c.zm0 + iz);
                 if (w == 0.0) continue;
-                const double *th = theta + (size_t)zm * nk;
-                for (int kn = 0; kn < c.kbu; ++kn) {
-                    const double q = r0[kn] * (1 - dn) + r1[kn] * dn;
-                    cij += th[kn] * (q * w);
-                }
+                const double *tm = th + (size_t)iz * c.nvu;
+                cij += w * (tm[iv] * (1 - dn) + tm[iv + 1] * dn);
             }
             a = D[i] * cij * D[j] + ((i == j) ? 1.0 : 0.0);
         }
@@ -386,104 +399,117 @@ __global__ __launch_bounds__(256) void k_solve(const ChunkDev *cs, const double 
 
 // ------------------------------------------------------------------ d pass --
 // P_j[i][0..nu] = sum_b Q_j[i][b] (y_b, W_b) for the stored Q_j and for S,
-// Q_j generated in FP64 from the tables.  Rows in blocks of 16, columns in
-// tiles of 8 staged through LDS.
-constexpr int DP_ROWS = 16, DP_COLS = 8, DP_MEMB = 4;
+// with Q_j generated in FP64 from the tables.  A workgroup owns 4 rows; in
+// each wave (one row) lane kn accumulates band kn for every z bin of the
+// chunk, so table reads are contiguous across lanes and every LDS read is a
+// broadcast.  Lane 63 accumulates the S member.
+constexpr int DP_R = 4, DP_B = 64, DP_ZB = 8;
 __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double *in,
                                                const TableDev *tabs, BinsDev bins,
-                                               const double *vec, double *P) {
+                                               const double *mat, const double *vec, double *P) {
     const ChunkDev c = cs[blockIdx.y];
-    const int i0 = blockIdx.x * DP_ROWS;
+    const int i0 = blockIdx.x * DP_R;
     if (i0 >= c.N) return;
     const int Np = c.Np, tid = threadIdx.x, nk = bins.nk, nu = c.nu;
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)Np;
+    const double *S = mat + c.mat_off + 3 * (size_t)Np * Np;
     const double *V = vec + c.vec_off;
-    __shared__ double sq[DP_ROWS * DP_COLS][MAX_NK];
-    __shared__ double sw[8][DP_ROWS][DP_COLS];
-    __shared__ double sS[DP_ROWS][DP_COLS];
-    __shared__ double sR[DP_COLS][1 + MAX_NU];
-    __shared__ int sv[DP_ROWS][DP_COLS];
-    __shared__ double sf[DP_ROWS][DP_COLS];
-    const int r = tid % DP_ROWS, g = tid / DP_ROWS;   // 16 groups
-    const int nmem = c.n + 1;                          // stored Q and S
-    const int npass = (nmem + 16 * DP_MEMB - 1) / (16 * DP_MEMB);
-    const int ncol = 1 + nu;
-    for (int pass = 0; pass < npass; ++pass) {
-        double acc[DP_MEMB][1 + MAX_NU];
-        for (int u = 0; u < DP_MEMB; ++u)
-            for (int q = 0; q <= MAX_NU; ++q) acc[u][q] = 0.0;
-        for (int b0 = 0; b0 < c.N; b0 += DP_COLS) {
-            __syncthreads();
-            if (tid < DP_ROWS * DP_COLS) {
-                const int rr = tid / DP_COLS, bb = tid % DP_COLS;
-                const int i = i0 + rr, b = b0 + bb;
-                int iv = 0;
-                double fr = 0.0, s = 0.0;
-                const bool ok = (i < c.N && b < c.N);
-                double z = 0.0;
-                if (ok) {
-                    double v;
-                    vz_pair(wave[i], wave[b], i, b, v, z);
-                    const double xx = (v - t.v1) / t.dv;
-                    iv = interp_index(xx, t.nv);
-                    fr = xx - iv;
-                    s = bins.turn_off_sfid ? 0.0 : signal_interp(t, z, v);
-                }
-                sv[rr][bb] = iv;
-                sf[rr][bb] = fr;
-                sS[rr][bb] = s;
-                for (int iz = 0; iz < c.nzq; ++iz) sw[iz][rr][bb] = ok ? zweight(bins, z, c.zm0 + iz) : 0.0;
-            }
-            if (tid < DP_COLS * ncol) {
-                const int bb = tid / ncol, q = tid % ncol;
-                const int b = b0 + bb;
-                double val = 0.0;
-                if (b < c.N) val = (q == 0) ? V[V_Y * (size_t)Np + b] : V[(V_W0 + q - 1) * (size_t)Np + b];
-                sR[bb][q] = val;
-            }
-            __syncthreads();
-            {
-                const int e = tid / 2, h = tid % 2;
-                const int rr = e / DP_COLS, bb = e % DP_COLS;
-                const int iv = sv[rr][bb];
-                const double fr = sf[rr][bb];
-                const double *r0 = t.q + (size_t)iv * nk, *r1 = r0 + nk;
-                for (int kn = h; kn < c.kbu; kn += 2) sq[e][kn] = r0[kn] * (1 - fr) + r1[kn] * fr;
-            }
-            __syncthreads();
-            for (int bb = 0; bb < DP_COLS; ++bb) {
-                double R[1 + MAX_NU];
-                for (int q = 0; q < ncol; ++q) R[q] = sR[bb][q];
+    __shared__ int siv[DP_R][DP_B];
+    __shared__ double sfr[DP_R][DP_B];
+    __shared__ double sw[DP_ZB][DP_R][DP_B];
+    __shared__ double sS[DP_R][DP_B];
+    __shared__ double sR[DP_B][1 + MAX_NU];
+    const int r = tid / 64, kn = tid % 64, i = i0 + r;
+    const bool isS = (kn == 63);
+    const bool active = (i < c.N) && (kn < c.kbu || isS);
+    double acc[DP_ZB][1 + MAX_NU];
 #pragma unroll
-                for (int u = 0; u < DP_MEMB; ++u) {
-                    const int j = g + 16 * (pass * DP_MEMB + u);
-                    if (j >= nmem) continue;
-                    double qv;
-                    if (j < c.n) {
-                        const int iz = j / c.kbu, kn = j % c.kbu;
-                        qv = sq[r * DP_COLS + bb][kn] * sw[iz][r][bb];
-                    } else {
-                        qv = sS[r][bb];
+    for (int m = 0; m < DP_ZB; ++m)
+#pragma unroll
+        for (int q = 0; q <= MAX_NU; ++q) acc[m][q] = 0.0;
+    for (int b0 = 0; b0 < c.N; b0 += DP_B) {
+        __syncthreads();
+        {
+            const int rr = tid / DP_B, bb = tid % DP_B;
+            const int ii = i0 + rr, b = b0 + bb;
+            const bool ok = (ii < c.N && b < c.N);
+            int iv = 0;
+            double fr = 0.0, z = 0.0, sv = 0.0;
+            if (ok) {
+                double v;
+                vz_pair(wave[ii], wave[b], ii, b, v, z);
+                const double xx = (v - t.v1) / t.dv;
+                iv = interp_index(xx, t.nv);
+                fr = xx - iv;
+                sv = S[(size_t)ii * Np + b];
+            }
+            siv[rr][bb] = iv;
+            sfr[rr][bb] = fr;
+            sS[rr][bb] = sv;
+#pragma unroll
+            for (int m = 0; m < DP_ZB; ++m)
+                sw[m][rr][bb] = (ok && m < c.nzq) ? zweight(bins, z, c.zm0 + m) : 0.0;
+        }
+        for (int e = tid; e < DP_B * (1 + MAX_NU); e += 256) {
+            const int bb = e / (1 + MAX_NU), q = e % (1 + MAX_NU);
+            const int b = b0 + bb;
+            double val = 0.0;
+            if (b < c.N && q <= nu) val = (q == 0) ? V[V_Y * (size_t)Np + b] : V[(V_W0 + q - 1) * (size_t)Np + b];
+            sR[bb][q] = val;
+        }
+        __syncthreads();
+        if (active) {
+            const int nb = min(DP_B, c.N - b0);
+            if (!isS) {
+#pragma unroll 4
+                for (int bb = 0; bb < nb; ++bb) {
+                    const int iv = siv[r][bb];
+                    const double fr = sfr[r][bb];
+                    const double q = t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
+                    double R[1 + MAX_NU];
+#pragma unroll
+                    for (int x = 0; x <= MAX_NU; ++x) R[x] = sR[bb][x];
+#pragma unroll
+                    for (int m = 0; m < DP_ZB; ++m) {
+                        if (m < c.nzq) {
+                            const double tq = q * sw[m][r][bb];
+#pragma unroll
+                            for (int x = 0; x <= MAX_NU; ++x) acc[m][x] += tq * R[x];
+                        }
                     }
-                    for (int q = 0; q < ncol; ++q) acc[u][q] += qv * R[q];
+                }
+            } else {
+                for (int bb = 0; bb < nb; ++bb) {
+                    const double tq = sS[r][bb];
+#pragma unroll
+                    for (int x = 0; x <= MAX_NU; ++x) acc[0][x] += tq * sR[bb][x];
                 }
             }
         }
-        const int i = i0 + r;
-        if (i < c.N) {
-            for (int u = 0; u < DP_MEMB; ++u) {
-                int j = g + 16 * (pass * DP_MEMB + u);
-                if (j >= nmem) continue;
-                const int slot = (j < c.n) ? j : c.n + 1;      // S is member T
-                double *dst = P + c.pj_off + ((size_t)slot * Np + i) * (1 + MAX_NU);
-                for (int q = 0; q < ncol; ++q) dst[q] = acc[u][q];
+    }
+    if (active) {
+        if (!isS) {
+#pragma unroll
+            for (int m = 0; m < DP_ZB; ++m) {
+                if (m < c.nzq) {
+                    const int j = m * c.kbu + kn;
+                    double *dst = P + c.pj_off + ((size_t)j * Np + i) * (1 + MAX_NU);
+#pragma unroll
+                    for (int x = 0; x <= MAX_NU; ++x) dst[x] = acc[m][x];
+                }
             }
+        } else {
+            double *dst = P + c.pj_off + ((size_t)(c.n + 1) * Np + i) * (1 + MAX_NU);
+#pragma unroll
+            for (int x = 0; x <= MAX_NU; ++x) dst[x] = acc[0][x];
         }
     }
 }
 
-// d_j = y^T Q_j y, a_H = N' W, s_j = W^T a_j for all members.
+// d_j = y^T Q_j y; a_H = N' W; s_j = W^T a_j; and the projected template
+// weights At_j = a_j - 1/2 U s_j (m x MAX_NU x Np, pixel contiguous), with
+// which D (K Q_j K^T) D = D (Q_j - U At_j^T - At_j U^T) D.
 __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double *in,
                                                const double *vec, double *P, double *sj) {
     const ChunkDev c = cs[blockIdx.x];
@@ -492,21 +518,13 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
     const double *y = V + V_Y * (size_t)Np;
     const double *Npr = in + c.in_off + IN_NPRIME * (size_t)Np;
     __shared__ double red[4];
+    __shared__ double ssh[MAX_NU][MAX_NU];
     double *s = sj + c.sj_off;
     double *d = s + (size_t)c.m * MAX_NU * MAX_NU;
-    // a_H
     for (int i = tid; i < Np; i += 256) {
         double *dst = P + c.pj_off + ((size_t)c.n * Np + i) * (1 + MAX_NU);
         dst[0] = 0.0;
         for (int q = 0; q < nu; ++q) dst[1 + q] = (i < c.N) ? Npr[i] * V[(V_W0 + q) * (size_t)Np + i] : 0.0;
-    }
-    // rows beyond N of the other members are never written by k_dpass
-    for (int j = 0; j < c.m; ++j) {
-        if (j == c.n) continue;
-        for (int i = c.N + tid; i < Np; i += 256) {
-            double *dst = P + c.pj_off + ((size_t)j * Np + i) * (1 + MAX_NU);
-            for (int q = 0; q <= MAX_NU; ++q) dst[q] = 0.0;
-        }
     }
     __syncthreads();
     for (int j = 0; j < c.m; ++j) {
@@ -523,77 +541,146 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
                 double p = 0.0;
                 for (int i = tid; i < c.N; i += 256) p += Wt[i] * Pj[(size_t)i * (1 + MAX_NU) + 1 + q];
                 p = block_sum(p, red);
-                if (tid == 0) s[(size_t)j * MAX_NU * MAX_NU + t * MAX_NU + q] = p;
+                if (tid == 0) {
+                    s[(size_t)j * MAX_NU * MAX_NU + t * MAX_NU + q] = p;
+                    ssh[t][q] = p;
+                }
             }
         }
+        __syncthreads();
+        double *At = P + c.at_off + (size_t)j * MAX_NU * Np;
+        for (int i = tid; i < Np; i += 256) {
+            double u[MAX_NU];
+            for (int t = 0; t < nu; ++t) u[t] = in[c.in_off + (IN_U0 + t) * (size_t)Np + i];
+            for (int t = 0; t < nu; ++t) {
+                double su = 0.0;
+                for (int q = 0; q < nu; ++q) su += ssh[t][q] * u[q];
+                At[(size_t)t * Np + i] = (i < c.N) ? Pj[(size_t)i * (1 + MAX_NU) + 1 + t] - 0.5 * su : 0.0;
+            }
+        }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------- stack generation --
-// X_j = D (K M_j K^T) D for every member, stored in the compute type,
-// 32x32 tiles over the full matrix.
+// X_j = D (Q_j - U At_j^T - At_j U^T) D for every member, in the compute
+// type.  A workgroup covers 16 rows x 64 columns; each thread owns one
+// column b and four rows, so stores are coalesced along b, At_j[b] is read
+// once per member per thread and At_j[i] is a wave-wide broadcast.  For each
+// k band the slice of the table row that the tile's velocity separations
+// reach is staged in LDS (double-buffered), so the interpolation gathers
+// are LDS reads; a slice longer than XG_SLICE falls back to global reads.
+constexpr int XG_SLICE = 2048;
 template <class OutT>
 __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *in,
                                               const TableDev *tabs, BinsDev bins,
-                                              const double *P, const double *sj, OutT *X) {
-    const ChunkDev c = cs[blockIdx.y];
-    const int n32 = c.Np / 32;
-    if (blockIdx.x >= n32 * n32) return;
-    const int I = blockIdx.x / n32, J = blockIdx.x % n32;
-    const int Np = c.Np, nk = bins.nk, nu = c.nu;
+                                              const double *mat, const double *aux, OutT *X) {
+    const ChunkDev c = cs[blockIdx.z];
+    const int i0 = blockIdx.y * 16, b = blockIdx.x * 64 + threadIdx.x % 64;
+    if (i0 >= c.Np || blockIdx.x * 64 >= c.Np) return;
+    const int Np = c.Np, nu = c.nu, tid = threadIdx.x;
+    const int rb = tid / 64;
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)Np;
     const double *D = in + c.in_off + IN_D * (size_t)Np;
     const double *Npr = in + c.in_off + IN_NPRIME * (size_t)Np;
-    const double *s = sj + c.sj_off;
+    const double *S = mat + c.mat_off + 3 * (size_t)Np * Np;
     OutT *Xc = X + c.stk_off;
     const size_t msz = (size_t)Np * Np;
-    for (int e = threadIdx.x; e < 1024; e += 256) {
-        const int i = I * 32 + e / 32, b = J * 32 + e % 32;
-        const size_t off = (size_t)i * Np + b;
-        if (i >= c.N || b >= c.N) {
-            for (int j = 0; j < c.m; ++j) Xc[j * msz + off] = (OutT)0;
-            continue;
-        }
-        double v, z;
-        vz_pair(wave[i], wave[b], i, b, v, z);
-        const double sc = D[i] * D[b];
+    __shared__ double slice[2][XG_SLICE];
+    __shared__ int red_lo[4], red_hi[4];
+    int iv[4], ii[4];
+    bool ok[4];
+    double fr[4], sc[4], zz[4], Ui[4][MAX_NU], Ub[MAX_NU];
+#pragma unroll
+    for (int q = 0; q < MAX_NU; ++q) Ub[q] = (q < nu && b < c.N) ? in[c.in_off + (IN_U0 + q) * (size_t)Np + b] : 0.0;
+    int lo = 1 << 30, hi = -1;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = i0 + rb + 4 * e;
+        ii[e] = i;
+        ok[e] = (i < c.N && b < c.N);
+        double v = 0.0, z = 0.0;
+        if (ok[e]) vz_pair(wave[i], wave[b], i, b, v, z);
         const double xx = (v - t.v1) / t.dv;
-        const int iv = interp_index(xx, t.nv);
-        const double dn = xx - iv;
-        const double *r0 = t.q + (size_t)iv * nk, *r1 = r0 + nk;
-        double Ui[MAX_NU], Ub[MAX_NU];
-        for (int q = 0; q < nu; ++q) {
-            Ui[q] = in[c.in_off + (IN_U0 + q) * (size_t)Np + i];
-            Ub[q] = in[c.in_off + (IN_U0 + q) * (size_t)Np + b];
+        iv[e] = ok[e] ? interp_index(xx, t.nv) : 0;
+        fr[e] = ok[e] ? xx - iv[e] : 0.0;
+        sc[e] = ok[e] ? D[i] * D[b] : 0.0;
+        zz[e] = z;
+        if (ok[e]) { lo = min(lo, iv[e]); hi = max(hi, iv[e]); }
+#pragma unroll
+        for (int q = 0; q < MAX_NU; ++q) Ui[e][q] = (q < nu && i < c.N) ? in[c.in_off + (IN_U0 + q) * (size_t)Np + i] : 0.0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if (tid % 64 == 0) { red_lo[rb] = lo; red_hi[rb] = hi; }
+    __syncthreads();
+    lo = min(min(red_lo[0], red_lo[1]), min(red_lo[2], red_lo[3]));
+    hi = max(max(red_hi[0], red_hi[1]), max(red_hi[2], red_hi[3]));
+    const int len = (hi >= lo) ? hi - lo + 2 : 0;
+    const bool staged = len <= XG_SLICE;
+    if (hi < lo) lo = 0;
+
+    auto emit = [&](int j, const double (&base)[4]) {
+        const double *At = aux + c.at_off + (size_t)j * MAX_NU * Np;
+        double ab[MAX_NU];
+#pragma unroll
+        for (int q = 0; q < MAX_NU; ++q) ab[q] = (q < nu) ? At[(size_t)q * Np + b] : 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double corr = 0.0;
+#pragma unroll
+            for (int q = 0; q < MAX_NU; ++q)
+                if (q < nu) corr += Ui[e][q] * ab[q] + At[(size_t)q * Np + ii[e]] * Ub[q];
+            if (ii[e] < Np && b < Np)
+                Xc[j * msz + (size_t)ii[e] * Np + b] = ok[e] ? (OutT)(sc[e] * (base[e] - corr)) : (OutT)0;
         }
-        double w[8];
-        for (int iz = 0; iz < c.nzq; ++iz) w[iz] = zweight(bins, z, c.zm0 + iz);
-        auto corr = [&](int j) {
-            if (nu == 0) return 0.0;
-            const double *ai = P + c.pj_off + ((size_t)j * Np + i) * (1 + MAX_NU) + 1;
-            const double *ab = P + c.pj_off + ((size_t)j * Np + b) * (1 + MAX_NU) + 1;
-            const double *sm = s + (size_t)j * MAX_NU * MAX_NU;
-            double r = 0.0;
-            for (int q = 0; q < nu; ++q) {
-                r -= Ui[q] * ab[q] + ai[q] * Ub[q];
-                double u = 0.0;
-                for (int p = 0; p < nu; ++p) u += sm[q * MAX_NU + p] * Ub[p];
-                r += Ui[q] * u;
+    };
+    if (staged) {
+        for (int x = tid; x < len; x += 256) slice[0][x] = t.qt[lo + x];
+    }
+    for (int kn = 0; kn < c.kbu; ++kn) {
+        const int cur = kn & 1;
+        __syncthreads();
+        if (staged && kn + 1 < c.kbu) {
+            const double *row = t.qt + (size_t)(kn + 1) * t.nv + lo;
+            for (int x = tid; x < len; x += 256) slice[cur ^ 1][x] = row[x];
+        }
+        double q[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double y0, y1;
+            if (staged) {
+                const int o = ok[e] ? iv[e] - lo : 0;   // in [0, len - 2] for valid elements
+                y0 = slice[cur][o];
+                y1 = slice[cur][o + 1];
+            } else {
+                const double *row = t.qt + (size_t)kn * t.nv;
+                y0 = row[iv[e]];
+                y1 = row[iv[e] + 1];
             }
-            return r;
-        };
-        for (int kn = 0; kn < c.kbu; ++kn) {
-            const double q = r0[kn] * (1 - dn) + r1[kn] * dn;
-            for (int iz = 0; iz < c.nzq; ++iz) {
-                const int j = iz * c.kbu + kn;
-                Xc[j * msz + off] = (OutT)(sc * (q * w[iz] + corr(j)));
+            q[e] = ok[e] ? y0 * (1 - fr[e]) + y1 * fr[e] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < DP_ZB; ++m) {
+            if (m < c.nzq) {
+                double base[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) base[e] = ok[e] ? q[e] * zweight(bins, zz[e], c.zm0 + m) : 0.0;
+                emit(m * c.kbu + kn, base);
             }
         }
-        const double hval = (i == b) ? Npr[i] : 0.0;
-        Xc[c.n * msz + off] = (OutT)(sc * (hval + corr(c.n)));
-        const double sval = bins.turn_off_sfid ? 0.0 : signal_interp(t, z, v);
-        Xc[(c.n + 1) * msz + off] = (OutT)(sc * (sval + corr(c.n + 1)));
+    }
+    {
+        double base[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) base[e] = (ii[e] == b && ok[e]) ? Npr[b] : 0.0;
+        emit(c.n, base);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) base[e] = ok[e] ? S[(size_t)ii[e] * Np + b] : 0.0;
+        emit(c.n + 1, base);
     }
 }
 
@@ -679,8 +766,10 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
 
 // ------------------------------------------------------------------ launch --
 void launch_assemble(const LaunchCtx &L, const double *theta) {
+    dim3 gt((L.nvu_max + 255) / 256, L.nchunks);
+    k_theta_tables<<<gt, 256, 0, L.stream>>>(L.chunks, L.tables, L.bins, theta, L.pj);
     dim3 grid((L.np_max / 32) * (L.np_max / 32), L.nchunks);
-    k_assemble<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, theta, L.mat);
+    k_assemble<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.pj, L.mat);
 }
 void launch_cholesky(const LaunchCtx &L, int *status) {
     k_cholesky<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat, status);
@@ -688,16 +777,16 @@ void launch_cholesky(const LaunchCtx &L, int *status) {
 void launch_trinv(const LaunchCtx &L) { k_trinv<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat); }
 void launch_solve(const LaunchCtx &L) { k_solve<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.mat, L.vec); }
 void launch_dpass(const LaunchCtx &L) {
-    dim3 grid((L.np_max + DP_ROWS - 1) / DP_ROWS, L.nchunks);
-    k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.vec, L.pj);
+    dim3 grid((L.np_max + DP_R - 1) / DP_R, L.nchunks);
+    k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
 }
 void launch_small(const LaunchCtx &L) { k_small<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj); }
 void launch_xgen(const LaunchCtx &L, void *X, int prec) {
-    dim3 grid((L.np_max / 32) * (L.np_max / 32), L.nchunks);
+    dim3 grid(L.np_max / 64, L.np_max / 16, L.nchunks);
     if (prec == 0)
-        k_xgen<double><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.pj, L.sj, (double *)X);
+        k_xgen<double><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, (double *)X);
     else
-        k_xgen<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.pj, L.sj, (float *)X);
+        k_xgen<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, (float *)X);
 }
 void launch_convert_li(const LaunchCtx &L, void *lic) {
     dim3 grid(64, L.nchunks);

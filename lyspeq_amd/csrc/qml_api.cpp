@@ -37,7 +37,7 @@ struct TableHost {
     double dv;
     int nv, nz;
     double v_length;
-    double *d_q = nullptr, *d_s = nullptr;
+    double *d_q = nullptr, *d_qt = nullptr, *d_s = nullptr;
 };
 
 struct ChunkHost {
@@ -45,6 +45,7 @@ struct ChunkHost {
     std::vector<double> wave, delta, ivar, U;  // wave in units of lambda_Lya; U is nu x N
     int N = 0, Np = 0, nu = 0;
     int n = 0, m = 0, kbu = 0, nzq = 0, zm0 = 0, nq = 0, fis = 0;
+    int nvu = 0;
     long long in_off = 0;
 };
 
@@ -53,7 +54,7 @@ struct Batch {
     ChunkDev *d_chunks = nullptr;
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
-    int np_max = 0;
+    int np_max = 0, nvu_max = 0;
     hipEvent_t ev[13]{};
 };
 
@@ -179,7 +180,7 @@ int qml_destroy(qml_ctx *c) {
     if (!c) return QML_OK;
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
-    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_s); }
+    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_s); }
     for (auto &b : c->batches) {
         hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
         for (auto &e : b.ev) if (e) hipEventDestroy(e);
@@ -246,6 +247,8 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
     std::vector<double> s((size_t)nv * t.nz, 0.0);
     if (signal && !c->bins.turn_off_sfid) std::copy(signal, signal + (size_t)nv * nz, s.begin());
     HIPCHK(hipMalloc(&t.d_q, q.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&t.d_qt, q.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(t.d_qt, deriv, q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&t.d_s, s.size() * sizeof(double)));
     HIPCHK(hipMemcpy(t.d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(t.d_s, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -317,6 +320,11 @@ int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int 
     nq *= b.nk;
     if (zl < 0 || zu >= b.nz || fis < 0 || fis + nq > c->T)
         return fail(QML_ERR_LIMIT, "qml_add_chunk: chunk lies outside the redshift bins");
+    {
+        const TableHost &tb = c->tables[rindex];
+        const double tdv = tb.v_length / (tb.nv - 1.0);
+        ch.nvu = (int)std::min<double>(tb.nv, std::floor(lenv / tdv) + 4.0);
+    }
     for (double &w : ch.wave) w /= LYA_REST;
     ch.N = N;
     ch.nq = nq;
@@ -398,6 +406,7 @@ int qml_commit(qml_ctx *c) {
     for (size_t r = 0; r < c->tables.size(); ++r) {
         const auto &t = c->tables[r];
         td[r].q = t.d_q;
+        td[r].qt = t.d_qt;
         td[r].s = t.d_s;
         td[r].nv = t.nv;
         td[r].nz = t.nz;
@@ -462,7 +471,7 @@ int qml_commit(qml_ctx *c) {
         const size_t mt = (m + TILE - 1) / TILE;
         n.mat = 4 * Np * Np;
         n.vec = (size_t)VEC_SLOTS * Np;
-        n.pj = m * Np * (1 + MAX_NU);
+        n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
         n.sj = m * MAX_NU * MAX_NU + ch.n;
         n.stk = m * Np * Np;
         n.lic = (c->prec == 0) ? 0 : Np * Np;
@@ -527,6 +536,9 @@ int qml_commit(qml_ctx *c) {
             d.mat_off = (long long)acc.mat;
             d.vec_off = (long long)acc.vec;
             d.pj_off = (long long)acc.pj;
+            d.at_off = (long long)(acc.pj + (size_t)ch.m * ch.Np * (1 + MAX_NU));
+            d.th_off = d.at_off + (long long)ch.m * MAX_NU * ch.Np;
+            d.nvu = ch.nvu;
             d.sj_off = (long long)acc.sj;
             d.stk_off = (long long)acc.stk;
             d.gram_off = (long long)acc.gram;
@@ -559,6 +571,7 @@ int qml_commit(qml_ctx *c) {
         for (int li = 0; li < (int)cd.size(); ++li) {
             const ChunkDev &d = cd[li];
             B.np_max = std::max(B.np_max, d.Np);
+            B.nvu_max = std::max(B.nvu_max, d.nvu);
             for (int j = 0; j < d.m; ++j)
                 for (int I = 0; I < d.nt; ++I)
                     for (int J = 0; J < d.nt; ++J) wl.push_back({li, j, I, J});
@@ -604,6 +617,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.chunks = B.d_chunks;
         L.nchunks = (int)B.ids.size();
         L.np_max = B.np_max;
+        L.nvu_max = B.nvu_max;
         L.in = c->d_in;
         L.tables = c->d_tables;
         L.bins = c->binsdev;

@@ -30,6 +30,7 @@ struct BinsDev {
 
 struct TableDev {
     const double *q;   // nv x nk, k contiguous (interleaved for row gathers)
+    const double *qt;  // nk x nv, v contiguous (for per-band slices)
     const double *s;   // nv x nz, z contiguous
     int nv, nz;
     double v1, dv, z1, dz;
@@ -48,7 +49,10 @@ struct ChunkDev {
     long long mat_off;   // into the FP64 matrix arena: A, L, Li, S, each Np*Np
     long long vec_off;   // into the FP64 vector arena
     long long stk_off;   // into the stack arenas (compute-type elements), m*Np*Np each
-    long long pj_off;    // into the P arena (doubles): (n+1) x Np x (1+MAX_NU)
+    long long pj_off;    // into the aux arena (doubles): P_j, m x Np x (1+MAX_NU)
+    long long at_off;    // into the aux arena: projected template weights, m x MAX_NU x Np
+    long long th_off;    // into the aux arena: theta-contracted Q tables, nzq x nvu
+    int nvu;             // table rows reachable by this chunk's velocity separations
     long long sj_off;    // into the s arena (doubles): m x MAX_NU^2, then d (n)
     long long gram_off;  // into the Gram partials (doubles)
     long long lic_off;   // into the compute-type copy of Li

@@ -10,12 +10,13 @@ struct LaunchCtx {
     const ChunkDev *chunks;   // device, nchunks entries
     int nchunks;
     int np_max;               // largest Np in the batch
+    int nvu_max;              // largest theta-table length in the batch
     const double *in;         // per-chunk input vectors
     const TableDev *tables;   // device table descriptors
     BinsDev bins;
     double *mat;              // FP64 matrix arena
     double *vec;              // FP64 vector arena
-    double *pj;               // P arena
+    double *pj;               // aux arena: P_j, At_j, theta tables
     double *sj;               // s_j and d arena
 };
 

@@ -399,112 +399,132 @@ __global__ __launch_bounds__(256) void k_solve(const ChunkDev *cs, const double 
 
 // ------------------------------------------------------------------ d pass --
 // P_j[i][0..nu] = sum_b Q_j[i][b] (y_b, W_b) for the stored Q_j and for S,
-// with Q_j generated in FP64 from the tables.  A workgroup owns 4 rows; in
-// each wave (one row) lane kn accumulates band kn for every z bin of the
-// chunk, so table reads are contiguous across lanes and every LDS read is a
-// broadcast.  Lane 63 accumulates the S member.
-constexpr int DP_R = 4, DP_B = 64, DP_ZB = 8;
+// with Q_j generated in FP64 from the tables.  For one row i this is the
+// product P_i = A_i^T B_i over the pixel index b, with A_i[b][kn] =
+// q_kn(v_ib) (plus a row for S_ib) and B_i[b][(m,x)] = w_m(z_ib) R[b][x],
+// R = (y, W): one wave per row runs it on v_mfma_f64_16x16x4f64 with the
+// operands interpolated in registers.  Pixel coordinates of 64 columns at a
+// time are staged in LDS per wave.
+constexpr int DP_ZB = 8, DQ_RB = 4, DQ_CB = 3;
+typedef double dq4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double *in,
                                                const TableDev *tabs, BinsDev bins,
                                                const double *mat, const double *vec, double *P) {
     const ChunkDev c = cs[blockIdx.y];
-    const int i0 = blockIdx.x * DP_R;
-    if (i0 >= c.N) return;
-    const int Np = c.Np, tid = threadIdx.x, nk = bins.nk, nu = c.nu;
+    if (blockIdx.x * 4 >= c.N) return;
+    const int Np = c.Np, nk = bins.nk, nu = c.nu, tid = threadIdx.x;
+    const int wv = tid / 64, l = tid % 64;
+    const int i = blockIdx.x * 4 + wv;
+    const bool rowok = i < c.N;
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)Np;
     const double *S = mat + c.mat_off + 3 * (size_t)Np * Np;
     const double *V = vec + c.vec_off;
-    __shared__ int siv[DP_R][DP_B];
-    __shared__ double sfr[DP_R][DP_B];
-    __shared__ double sw[DP_ZB][DP_R][DP_B];
-    __shared__ double sS[DP_R][DP_B];
-    __shared__ double sR[DP_B][1 + MAX_NU];
-    const int r = tid / 64, kn = tid % 64, i = i0 + r;
-    const bool isS = (kn == 63);
-    const bool active = (i < c.N) && (kn < c.kbu || isS);
-    double acc[DP_ZB][1 + MAX_NU];
+    __shared__ int siv[4][64];
+    __shared__ double sfr[4][64], sS[4][64];
+    __shared__ double sw[4][DP_ZB][64];
+    __shared__ double sR[64][1 + MAX_NU];
+    const int nx = 1 + nu;
+    const int nrow = c.kbu + 1, RB = (nrow + 15) / 16;
+    const int ncq = c.nzq * nx, ncol = ncq + nx, CB = (ncol + 15) / 16;
+    // this lane's B columns: (z bin m, component x); m = -1 for the S columns
+    int bm[DQ_CB], bx[DQ_CB];
 #pragma unroll
-    for (int m = 0; m < DP_ZB; ++m)
+    for (int cb = 0; cb < DQ_CB; ++cb) {
+        const int col = cb * 16 + l % 16;
+        if (col < ncq) { bm[cb] = col / nx; bx[cb] = col % nx; }
+        else if (col < ncol) { bm[cb] = -1; bx[cb] = col - ncq; }
+        else { bm[cb] = -2; bx[cb] = 0; }
+    }
+    dq4 acc[DQ_RB][DQ_CB];
 #pragma unroll
-        for (int q = 0; q <= MAX_NU; ++q) acc[m][q] = 0.0;
-    for (int b0 = 0; b0 < c.N; b0 += DP_B) {
+    for (int rb = 0; rb < DQ_RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < DQ_CB; ++cb) acc[rb][cb] = dq4{0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < c.N; b0 += 64) {
         __syncthreads();
         {
-            const int rr = tid / DP_B, bb = tid % DP_B;
-            const int ii = i0 + rr, b = b0 + bb;
-            const bool ok = (ii < c.N && b < c.N);
+            const int b = b0 + l;
+            const bool ok = rowok && b < c.N;
             int iv = 0;
             double fr = 0.0, z = 0.0, sv = 0.0;
             if (ok) {
                 double v;
-                vz_pair(wave[ii], wave[b], ii, b, v, z);
+                vz_pair(wave[i], wave[b], i, b, v, z);
                 const double xx = (v - t.v1) / t.dv;
                 iv = interp_index(xx, t.nv);
                 fr = xx - iv;
-                sv = S[(size_t)ii * Np + b];
+                sv = S[(size_t)i * Np + b];
             }
-            siv[rr][bb] = iv;
-            sfr[rr][bb] = fr;
-            sS[rr][bb] = sv;
+            siv[wv][l] = iv;
+            sfr[wv][l] = fr;
+            sS[wv][l] = sv;
 #pragma unroll
-            for (int m = 0; m < DP_ZB; ++m)
-                sw[m][rr][bb] = (ok && m < c.nzq) ? zweight(bins, z, c.zm0 + m) : 0.0;
+            for (int m = 0; m < DP_ZB; ++m) sw[wv][m][l] = (ok && m < c.nzq) ? zweight(bins, z, c.zm0 + m) : 0.0;
         }
-        for (int e = tid; e < DP_B * (1 + MAX_NU); e += 256) {
-            const int bb = e / (1 + MAX_NU), q = e % (1 + MAX_NU);
+        for (int e = tid; e < 64 * (1 + MAX_NU); e += 256) {
+            const int bb = e / (1 + MAX_NU), x = e % (1 + MAX_NU);
             const int b = b0 + bb;
             double val = 0.0;
-            if (b < c.N && q <= nu) val = (q == 0) ? V[V_Y * (size_t)Np + b] : V[(V_W0 + q - 1) * (size_t)Np + b];
-            sR[bb][q] = val;
+            if (b < c.N && x < nx) val = (x == 0) ? V[V_Y * (size_t)Np + b] : V[(V_W0 + x - 1) * (size_t)Np + b];
+            sR[bb][x] = val;
         }
         __syncthreads();
-        if (active) {
-            const int nb = min(DP_B, c.N - b0);
-            if (!isS) {
-#pragma unroll 4
-                for (int bb = 0; bb < nb; ++bb) {
-                    const int iv = siv[r][bb];
-                    const double fr = sfr[r][bb];
-                    const double q = t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
-                    double R[1 + MAX_NU];
+        const int nb = min(64, c.N - b0);
+        for (int kk = 0; kk < nb; kk += 4) {
+            const int bl = kk + l / 16;            // this lane's K slot
+            const bool bok = rowok && bl < nb;
+            const int iv = siv[wv][bl];
+            const double fr = sfr[wv][bl];
+            double a[DQ_RB], bv[DQ_CB];
 #pragma unroll
-                    for (int x = 0; x <= MAX_NU; ++x) R[x] = sR[bb][x];
-#pragma unroll
-                    for (int m = 0; m < DP_ZB; ++m) {
-                        if (m < c.nzq) {
-                            const double tq = q * sw[m][r][bb];
-#pragma unroll
-                            for (int x = 0; x <= MAX_NU; ++x) acc[m][x] += tq * R[x];
-                        }
-                    }
+            for (int rb = 0; rb < DQ_RB; ++rb) {
+                const int kn = rb * 16 + l % 16;
+                double val = 0.0;
+                if (rb < RB && bok) {
+                    if (kn < c.kbu)
+                        val = t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
+                    else if (kn == c.kbu)
+                        val = sS[wv][bl];
                 }
-            } else {
-                for (int bb = 0; bb < nb; ++bb) {
-                    const double tq = sS[r][bb];
+                a[rb] = val;
+            }
 #pragma unroll
-                    for (int x = 0; x <= MAX_NU; ++x) acc[0][x] += tq * sR[bb][x];
+            for (int cb = 0; cb < DQ_CB; ++cb) {
+                double val = 0.0;
+                if (cb < CB && bok && bm[cb] >= -1) {
+                    const double r = sR[bl][bx[cb]];
+                    val = (bm[cb] >= 0) ? sw[wv][bm[cb]][bl] * r : r;
+                }
+                bv[cb] = val;
+            }
+#pragma unroll
+            for (int rb = 0; rb < DQ_RB; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < DQ_CB; ++cb)
+                    if (rb < RB && cb < CB)
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bv[cb], acc[rb][cb], 0, 0, 0);
+        }
+    }
+    if (!rowok) return;
+    // FP64 MFMA layout: element u of a lane holds row 4u + lane/16, column lane%16
+#pragma unroll
+    for (int rb = 0; rb < DQ_RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < DQ_CB; ++cb) {
+            if (rb >= RB || cb >= CB || bm[cb] < -1) continue;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kn = rb * 16 + 4 * u + l / 16;
+                const double val = acc[rb][cb][u];
+                if (bm[cb] >= 0 && kn < c.kbu) {
+                    const int j = bm[cb] * c.kbu + kn;
+                    P[c.pj_off + ((size_t)j * Np + i) * (1 + MAX_NU) + bx[cb]] = val;
+                } else if (bm[cb] == -1 && kn == c.kbu) {
+                    P[c.pj_off + ((size_t)(c.n + 1) * Np + i) * (1 + MAX_NU) + bx[cb]] = val;
                 }
             }
         }
-    }
-    if (active) {
-        if (!isS) {
-#pragma unroll
-            for (int m = 0; m < DP_ZB; ++m) {
-                if (m < c.nzq) {
-                    const int j = m * c.kbu + kn;
-                    double *dst = P + c.pj_off + ((size_t)j * Np + i) * (1 + MAX_NU);
-#pragma unroll
-                    for (int x = 0; x <= MAX_NU; ++x) dst[x] = acc[m][x];
-                }
-            }
-        } else {
-            double *dst = P + c.pj_off + ((size_t)(c.n + 1) * Np + i) * (1 + MAX_NU);
-#pragma unroll
-            for (int x = 0; x <= MAX_NU; ++x) dst[x] = acc[0][x];
-        }
-    }
 }
 
 // d_j = y^T Q_j y; a_H = N' W; s_j = W^T a_j; and the projected template
@@ -563,23 +583,47 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
 }
 
 // ------------------------------------------------------- stack generation --
+// Stores of one stack element in the compute type.  __bf16 means the split
+// form: hi = bf16(v) in plane 2j and lo = bf16(v - hi) in plane 2j+1.
+template <class OutT> struct XStore;
+template <> struct XStore<double> {
+    static __device__ __forceinline__ void put(double *X, size_t msz, int j, size_t off, double v) {
+        X[j * msz + off] = v;
+    }
+};
+template <> struct XStore<float> {
+    static __device__ __forceinline__ void put(float *X, size_t msz, int j, size_t off, float v) {
+        X[j * msz + off] = v;
+    }
+};
+template <> struct XStore<__bf16> {
+    static __device__ __forceinline__ void put(__bf16 *X, size_t msz, int j, size_t off, float v) {
+        const __bf16 h = (__bf16)v;
+        const __bf16 l = (__bf16)(v - (float)h);
+        X[2 * j * msz + off] = h;
+        X[(2 * j + 1) * msz + off] = l;
+    }
+};
+
 // X_j = D (Q_j - U At_j^T - At_j U^T) D for every member, in the compute
-// type.  A workgroup covers 16 rows x 64 columns; each thread owns one
-// column b and four rows, so stores are coalesced along b, At_j[b] is read
-// once per member per thread and At_j[i] is a wave-wide broadcast.  For each
-// k band the slice of the table row that the tile's velocity separations
-// reach is staged in LDS (double-buffered), so the interpolation gathers
-// are LDS reads; a slice longer than XG_SLICE falls back to global reads.
+// type, with arithmetic in CT (FP64 for the FP64 path, FP32 otherwise).
+// A workgroup covers (4 EPT) rows x 64 columns; each thread owns one column
+// b and EPT rows, so stores are coalesced along b.  Per element the D
+// scaling is folded into the z-bin weights (sw) and template coefficients
+// (a, bq), so one member costs 1 + 2 nu multiply-adds.  For each k band the
+// table slice reached by the tile and the bands' At rows/columns are staged
+// in LDS (double-buffered); a slice longer than XG_SLICE is read from global.
 constexpr int XG_SLICE = 2048;
-template <class OutT>
+template <class OutT, class CT, int EPT>
 __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *in,
                                               const TableDev *tabs, BinsDev bins,
                                               const double *mat, const double *aux, OutT *X) {
+    constexpr int TR = 4 * EPT;   // rows per tile
     const ChunkDev c = cs[blockIdx.z];
-    const int i0 = blockIdx.y * 16, b = blockIdx.x * 64 + threadIdx.x % 64;
-    if (i0 >= c.Np || blockIdx.x * 64 >= c.Np) return;
-    const int Np = c.Np, nu = c.nu, tid = threadIdx.x;
-    const int rb = tid / 64;
+    const int i0 = blockIdx.y * TR, b0 = blockIdx.x * 64;
+    if (i0 >= c.Np || b0 >= c.Np) return;
+    const int Np = c.Np, nu = c.nu, tid = threadIdx.x, nzq = c.nzq;
+    const int rb = tid / 64, b = b0 + tid % 64;
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)Np;
     const double *D = in + c.in_off + IN_D * (size_t)Np;
@@ -587,29 +631,36 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     const double *S = mat + c.mat_off + 3 * (size_t)Np * Np;
     OutT *Xc = X + c.stk_off;
     const size_t msz = (size_t)Np * Np;
-    __shared__ double slice[2][XG_SLICE];
+    __shared__ CT slice[2][XG_SLICE];
+    __shared__ CT atc[2][DP_ZB][MAX_NU][64];     // At of the band's members at the tile's columns
+    __shared__ CT atr[2][DP_ZB][MAX_NU][TR];     // ... and at the tile's rows
     __shared__ int red_lo[4], red_hi[4];
-    int iv[4], ii[4];
-    bool ok[4];
-    double fr[4], sc[4], zz[4], Ui[4][MAX_NU], Ub[MAX_NU];
-#pragma unroll
-    for (int q = 0; q < MAX_NU; ++q) Ub[q] = (q < nu && b < c.N) ? in[c.in_off + (IN_U0 + q) * (size_t)Np + b] : 0.0;
+    int iv[EPT], ii[EPT];
+    bool ok[EPT];
+    CT fr[EPT], sw[EPT][DP_ZB], ca[EPT][MAX_NU], cb[EPT][MAX_NU];
     int lo = 1 << 30, hi = -1;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < EPT; ++e) {
         const int i = i0 + rb + 4 * e;
         ii[e] = i;
         ok[e] = (i < c.N && b < c.N);
-        double v = 0.0, z = 0.0;
-        if (ok[e]) vz_pair(wave[i], wave[b], i, b, v, z);
+        double v = 0.0, z = 0.0, sc = 0.0;
+        if (ok[e]) {
+            vz_pair(wave[i], wave[b], i, b, v, z);
+            sc = D[i] * D[b];
+        }
         const double xx = (v - t.v1) / t.dv;
         iv[e] = ok[e] ? interp_index(xx, t.nv) : 0;
-        fr[e] = ok[e] ? xx - iv[e] : 0.0;
-        sc[e] = ok[e] ? D[i] * D[b] : 0.0;
-        zz[e] = z;
+        fr[e] = ok[e] ? (CT)(xx - iv[e]) : (CT)0;
         if (ok[e]) { lo = min(lo, iv[e]); hi = max(hi, iv[e]); }
 #pragma unroll
-        for (int q = 0; q < MAX_NU; ++q) Ui[e][q] = (q < nu && i < c.N) ? in[c.in_off + (IN_U0 + q) * (size_t)Np + i] : 0.0;
+        for (int m = 0; m < DP_ZB; ++m) sw[e][m] = (ok[e] && m < nzq) ? (CT)(sc * zweight(bins, z, c.zm0 + m)) : (CT)0;
+#pragma unroll
+        for (int q = 0; q < MAX_NU; ++q) {
+            const bool on = ok[e] && q < nu;
+            ca[e][q] = on ? (CT)(sc * in[c.in_off + (IN_U0 + q) * (size_t)Np + i]) : (CT)0;
+            cb[e][q] = on ? (CT)(sc * in[c.in_off + (IN_U0 + q) * (size_t)Np + b]) : (CT)0;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         lo = min(lo, __shfl_xor(lo, o, 64));
@@ -623,75 +674,97 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     const bool staged = len <= XG_SLICE;
     if (hi < lo) lo = 0;
 
-    auto emit = [&](int j, const double (&base)[4]) {
-        const double *At = aux + c.at_off + (size_t)j * MAX_NU * Np;
-        double ab[MAX_NU];
-#pragma unroll
-        for (int q = 0; q < MAX_NU; ++q) ab[q] = (q < nu) ? At[(size_t)q * Np + b] : 0.0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            double corr = 0.0;
-#pragma unroll
-            for (int q = 0; q < MAX_NU; ++q)
-                if (q < nu) corr += Ui[e][q] * ab[q] + At[(size_t)q * Np + ii[e]] * Ub[q];
-            if (ii[e] < Np && b < Np)
-                Xc[j * msz + (size_t)ii[e] * Np + b] = ok[e] ? (OutT)(sc[e] * (base[e] - corr)) : (OutT)0;
+    // stage band kn into buffer bf: table slice and the At of members (m, kn)
+    auto stage = [&](int kn, int bf) {
+        if (staged) {
+            const double *row = t.qt + (size_t)kn * t.nv + lo;
+            for (int x = tid; x < len; x += 256) slice[bf][x] = (CT)row[x];
+        }
+        for (int x = tid; x < nzq * nu * (64 + TR); x += 256) {
+            const int m = x / (nu * (64 + TR)), rem = x % (nu * (64 + TR));
+            const int q = rem / (64 + TR), p = rem % (64 + TR);
+            const double *At = aux + c.at_off + ((size_t)(m * c.kbu + kn) * MAX_NU + q) * Np;
+            if (p < 64) atc[bf][m][q][p] = (CT)At[b0 + p];
+            else atr[bf][m][q][p - 64] = (i0 + p - 64 < Np) ? (CT)At[i0 + p - 64] : (CT)0;
         }
     };
-    if (staged) {
-        for (int x = tid; x < len; x += 256) slice[0][x] = t.qt[lo + x];
-    }
+    if (c.kbu > 0) stage(0, 0);
     for (int kn = 0; kn < c.kbu; ++kn) {
         const int cur = kn & 1;
         __syncthreads();
-        if (staged && kn + 1 < c.kbu) {
-            const double *row = t.qt + (size_t)(kn + 1) * t.nv + lo;
-            for (int x = tid; x < len; x += 256) slice[cur ^ 1][x] = row[x];
-        }
-        double q[4];
+        if (kn + 1 < c.kbu) stage(kn + 1, cur ^ 1);
+        CT q[EPT];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            double y0, y1;
+        for (int e = 0; e < EPT; ++e) {
+            CT y0, y1;
             if (staged) {
                 const int o = ok[e] ? iv[e] - lo : 0;   // in [0, len - 2] for valid elements
                 y0 = slice[cur][o];
                 y1 = slice[cur][o + 1];
             } else {
                 const double *row = t.qt + (size_t)kn * t.nv;
-                y0 = row[iv[e]];
-                y1 = row[iv[e] + 1];
+                y0 = (CT)row[iv[e]];
+                y1 = (CT)row[iv[e] + 1];
             }
-            q[e] = ok[e] ? y0 * (1 - fr[e]) + y1 * fr[e] : 0.0;
+            q[e] = y0 * ((CT)1 - fr[e]) + y1 * fr[e];
         }
 #pragma unroll
         for (int m = 0; m < DP_ZB; ++m) {
-            if (m < c.nzq) {
-                double base[4];
+            if (m >= nzq) break;
+            CT ac[MAX_NU];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) base[e] = ok[e] ? q[e] * zweight(bins, zz[e], c.zm0 + m) : 0.0;
-                emit(m * c.kbu + kn, base);
+            for (int x = 0; x < MAX_NU; ++x) ac[x] = (x < nu) ? atc[cur][m][x][tid % 64] : (CT)0;
+            const int j = m * c.kbu + kn;
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                CT val = q[e] * sw[e][m];
+#pragma unroll
+                for (int x = 0; x < MAX_NU; ++x)
+                    if (x < nu) val -= ca[e][x] * ac[x] + atr[cur][m][x][rb + 4 * e] * cb[e][x];
+                if (ii[e] < Np) XStore<OutT>::put(Xc, msz, j, (size_t)ii[e] * Np + b, ok[e] ? val : (CT)0);
             }
         }
     }
-    {
-        double base[4];
+    // the noise (H) and signal (T) members
 #pragma unroll
-        for (int e = 0; e < 4; ++e) base[e] = (ii[e] == b && ok[e]) ? Npr[b] : 0.0;
-        emit(c.n, base);
+    for (int h = 0; h < 2; ++h) {
+        const int j = c.n + h;
+        const double *At = aux + c.at_off + (size_t)j * MAX_NU * Np;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) base[e] = ok[e] ? S[(size_t)ii[e] * Np + b] : 0.0;
-        emit(c.n + 1, base);
+        for (int e = 0; e < EPT; ++e) {
+            if (ii[e] >= Np) continue;
+            double val = 0.0;
+            if (ok[e]) {
+                const double sc = D[ii[e]] * D[b];
+                const double base = (h == 0) ? ((ii[e] == b) ? Npr[b] : 0.0) : S[(size_t)ii[e] * Np + b];
+                double corr = 0.0;
+                for (int x = 0; x < nu; ++x) {
+                    corr += in[c.in_off + (IN_U0 + x) * (size_t)Np + ii[e]] * At[(size_t)x * Np + b]
+                          + At[(size_t)x * Np + ii[e]] * in[c.in_off + (IN_U0 + x) * (size_t)Np + b];
+                }
+                val = sc * (base - corr);
+            }
+            XStore<OutT>::put(Xc, msz, j, (size_t)ii[e] * Np + b, (CT)val);
+        }
     }
 }
 
-// Li in the compute type.
+// Li in the compute type (two bf16 planes for the split form).
 template <class OutT>
 __global__ void k_convert_li(const ChunkDev *cs, const double *mat, OutT *lic) {
     const ChunkDev c = cs[blockIdx.y];
     const size_t n = (size_t)c.Np * c.Np;
     const double *Li = mat + c.mat_off + 2 * n;
-    for (size_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
-        lic[c.lic_off + e] = (OutT)Li[e];
+    for (size_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        if constexpr (sizeof(OutT) == 2) {
+            const double v = Li[e];
+            const __bf16 h = (__bf16)(float)v;
+            lic[c.lic_off + e] = h;
+            lic[c.lic_off + n + e] = (__bf16)(float)(v - (double)h);
+        } else {
+            lic[c.lic_off + e] = (OutT)Li[e];
+        }
+    }
 }
 
 // ------------------------------------------------------------ Gram reduce --
@@ -777,20 +850,31 @@ void launch_cholesky(const LaunchCtx &L, int *status) {
 void launch_trinv(const LaunchCtx &L) { k_trinv<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat); }
 void launch_solve(const LaunchCtx &L) { k_solve<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.mat, L.vec); }
 void launch_dpass(const LaunchCtx &L) {
-    dim3 grid((L.np_max + DP_R - 1) / DP_R, L.nchunks);
+    dim3 grid((L.np_max + 3) / 4, L.nchunks);
     k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
 }
 void launch_small(const LaunchCtx &L) { k_small<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj); }
 void launch_xgen(const LaunchCtx &L, void *X, int prec) {
-    dim3 grid(L.np_max / 64, L.np_max / 16, L.nchunks);
-    if (prec == 0)
-        k_xgen<double><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, (double *)X);
-    else
-        k_xgen<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, (float *)X);
+    if (prec == 0) {
+        dim3 grid(L.np_max / 64, L.np_max / 8, L.nchunks);
+        k_xgen<double, double, 2><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
+                                                             (double *)X);
+    } else {
+        dim3 grid(L.np_max / 64, L.np_max / 16, L.nchunks);
+        if (prec == 1)
+            k_xgen<float, float, 4><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
+                                                               (float *)X);
+        else
+            k_xgen<__bf16, float, 4><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
+                                                                (__bf16 *)X);
+    }
 }
-void launch_convert_li(const LaunchCtx &L, void *lic) {
+void launch_convert_li(const LaunchCtx &L, void *lic, int prec) {
     dim3 grid(64, L.nchunks);
-    k_convert_li<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.mat, (float *)lic);
+    if (prec == 2)
+        k_convert_li<__bf16><<<grid, 256, 0, L.stream>>>(L.chunks, L.mat, (__bf16 *)lic);
+    else
+        k_convert_li<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.mat, (float *)lic);
 }
 void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double *gram, const int *status,
                           double *fc, double *dbt) {

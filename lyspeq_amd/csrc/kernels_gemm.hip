@@ -1,7 +1,8 @@
 // Dense MFMA products of the QML engine (gfx950).
 //
 //   whiten_left :  Y_j = Li X_j              (Li lower triangular: K-range of
-//                                             tile row I is [0, 64 (I+1)))
+//                                             tile row I is [0, 64 (I+1)); X_j is
+//                                             symmetric and read row-wise as B^T)
 //   whiten_right:  G_j = Y_j Li^T            (upper tiles only, K-range
 //                                             [0, 64 (J+1)), tile-packed out)
 //   gram        :  E_s = sum_{k in G tile s} G_j[k] G_j'[k]   (split-K partials,
@@ -245,12 +246,13 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     int I0, J0;
     if (MODE == MODE_LEFT) {
         p.A = (const T *)Abase + c.lic_off;
+        // X_j is symmetric, so its rows serve as the K-contiguous rows of B^T
         p.B = (const T *)Bbase + c.stk_off + wk.j * msz;
         p.lda = p.ldb = Np;
         p.kbeg = 0;
         p.kend = (wk.I + 1) * TILE;
         p.arows = p.brows = Np;
-        p.nt = false;
+        p.nt = (PREC != 0);
         I0 = wk.I * TILE;
         J0 = wk.J * TILE;
     } else if (MODE == MODE_RIGHT) {
@@ -335,17 +337,22 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     }
 }
 
+void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
+                const void *A, const void *B, void *C);
+
 template <int MODE>
 static void launch_mode(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                         const void *A, const void *B, void *C) {
     if (nwork <= 0) return;
+    if (prec == 2) {
+        launch_bf3(st, MODE, chunks, work, nwork, A, B, C);
+        return;
+    }
     const int grid = 8 * ((nwork + 7) / 8);
     if (prec == 0)
         k_gemm<0, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
-    else if (prec == 1)
-        k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
     else
-        k_gemm<2, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+        k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
 }
 
 void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
@@ -359,6 +366,189 @@ void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, 
 void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram) {
     launch_mode<MODE_GRAM>(st, prec, chunks, work, nwork, G, G, gram);
+}
+
+}  // namespace qml
+
+namespace qml {
+
+// ------------------------------------------- split-BF16x3, 128x128 tiles --
+// Operands are stored pre-split: element (r, k) of an operand is hi[r*ld + k]
+// + hi[lo_off + r*ld + k] (two bf16 planes), so the main loop moves bytes
+// and issues MFMAs only.  Every product is A B^T with both operands K-major
+// (X_j is symmetric; Li^T and the Gram are NT by construction).  A
+// workgroup computes a 128x128 tile; each wave a 64x64 quarter of 4x4
+// 16x16 blocks, three v_mfma_f32_16x16x32_bf16 per block and K step
+// (lo*hi, hi*lo, hi*hi).  Waves whose quarter lies outside the matrix, or
+// strictly below the diagonal for the upper-only products, skip their MFMAs.
+constexpr int BT = 128, BK = 32, BLD = 40;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork,
+                                                  const __bf16 *Abase, const __bf16 *Bbase, void *Cbase) {
+    const int li = xcd_logical(nwork);
+    if (li >= nwork) return;
+    const Work wk = work[li];
+    const ChunkDev c = cs[wk.c];
+    const int Np = c.Np;
+    const long long msz = (long long)Np * Np;
+    const int I0 = wk.I * BT, J0 = wk.J * BT;
+    const __bf16 *Ahi, *Bhi;
+    long long lda, ldb, alo, blo;
+    int arows, brows, kbeg = 0, kend, Mv, Nv;
+    if (MODE == MODE_LEFT) {
+        Ahi = Abase + c.lic_off;
+        Bhi = Bbase + c.stk_off + 2 * wk.j * msz;
+        lda = ldb = Np;
+        alo = blo = msz;
+        arows = brows = Np;
+        kend = min(Np, I0 + BT);
+        Mv = Nv = Np;
+    } else if (MODE == MODE_RIGHT) {
+        Ahi = Abase + c.stk_off + 2 * wk.j * msz;
+        Bhi = Bbase + c.lic_off;
+        lda = ldb = Np;
+        alo = blo = msz;
+        arows = brows = Np;
+        kend = min(Np, J0 + BT);
+        Mv = Nv = Np;
+    } else {
+        const long long ktot = (long long)c.ntu * TILE * TILE;
+        Ahi = Bhi = Abase + c.stk_off;
+        lda = ldb = 2 * ktot;
+        alo = blo = ktot;
+        arows = brows = c.m;
+        kbeg = wk.j * TILE * TILE;
+        kend = kbeg + TILE * TILE;
+        Mv = Nv = c.m;
+    }
+    __shared__ __attribute__((aligned(16))) __bf16 sA[2][2][BT][BLD];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[2][2][BT][BLD];
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int wm = w / 2, wn = w % 2;
+    const int r0 = I0 + 64 * wm, c0 = J0 + 64 * wn;
+    bool active = (r0 < Mv) && (c0 < Nv);
+    if (MODE != MODE_LEFT) active = active && (c0 >= r0);
+    f4 acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+    // global -> register staging: thread t owns row t/2, k-half (t%2)*16
+    const int lr = tid / 2, lk = (tid % 2) * 16;
+    const bool aok = (I0 + lr) < arows, bok = (J0 + lr) < brows;
+    const __bf16 *pa = Ahi + (long long)(aok ? I0 + lr : 0) * lda + lk;
+    const __bf16 *pb = Bhi + (long long)(bok ? J0 + lr : 0) * ldb + lk;
+    const uint4 z4 = {0u, 0u, 0u, 0u};
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define QML_GLOAD(k0)                                                  \
+    ra0 = aok ? *(const uint4 *)(pa + (k0)) : z4;                      \
+    ra1 = aok ? *(const uint4 *)(pa + (k0) + 8) : z4;                  \
+    ra2 = aok ? *(const uint4 *)(pa + alo + (k0)) : z4;                \
+    ra3 = aok ? *(const uint4 *)(pa + alo + (k0) + 8) : z4;            \
+    rb0 = bok ? *(const uint4 *)(pb + (k0)) : z4;                      \
+    rb1 = bok ? *(const uint4 *)(pb + (k0) + 8) : z4;                  \
+    rb2 = bok ? *(const uint4 *)(pb + blo + (k0)) : z4;                \
+    rb3 = bok ? *(const uint4 *)(pb + blo + (k0) + 8) : z4;
+#define QML_SSTORE(st)                                                 \
+    *(uint4 *)&sA[st][0][lr][lk] = ra0;                                \
+    *(uint4 *)&sA[st][0][lr][lk + 8] = ra1;                            \
+    *(uint4 *)&sA[st][1][lr][lk] = ra2;                                \
+    *(uint4 *)&sA[st][1][lr][lk + 8] = ra3;                            \
+    *(uint4 *)&sB[st][0][lr][lk] = rb0;                                \
+    *(uint4 *)&sB[st][0][lr][lk + 8] = rb1;                            \
+    *(uint4 *)&sB[st][1][lr][lk] = rb2;                                \
+    *(uint4 *)&sB[st][1][lr][lk + 8] = rb3;
+    if (kbeg < kend) {
+        QML_GLOAD(kbeg)
+        QML_SSTORE(0)
+        __syncthreads();
+        int st = 0;
+        const int kq = 8 * (lane / 16), rr = lane % 16;
+        for (int k0 = kbeg; k0 < kend; k0 += BK, st ^= 1) {
+            const bool more = k0 + BK < kend;
+            if (more) {
+                QML_GLOAD(k0 + BK)
+            }
+            if (active) {
+                bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    ah[x] = *(const bf8 *)&sA[st][0][64 * wm + 16 * x + rr][kq];
+                    al[x] = *(const bf8 *)&sA[st][1][64 * wm + 16 * x + rr][kq];
+                    bh[x] = *(const bf8 *)&sB[st][0][64 * wn + 16 * x + rr][kq];
+                    bl[x] = *(const bf8 *)&sB[st][1][64 * wn + 16 * x + rr][kq];
+                }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) {
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                    }
+            }
+            if (more) {
+                QML_SSTORE(st ^ 1)
+            }
+            __syncthreads();
+        }
+    }
+#undef QML_GLOAD
+#undef QML_SSTORE
+    if (!active) return;
+    if (MODE == MODE_GRAM) {
+        const int ntile = c.mt * (c.mt + 1) / 2;
+        const double wgt = packed_tile_is_diag(wk.j, c.nt) ? 1.0 : 2.0;
+        double *C = (double *)Cbase + c.gram_off
+                  + ((size_t)wk.j * ntile + upper_tile_index(r0 / TILE, c0 / TILE, c.mt)) * TILE * TILE;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    C[(16 * x + 4 * (lane / 16) + u) * TILE + 16 * y + lane % 16] = wgt * (double)acc[x][y][u];
+        return;
+    }
+    __bf16 *C = (__bf16 *)Cbase;
+    long long base, ldc, lo;
+    if (MODE == MODE_LEFT) {
+        base = c.stk_off + 2 * wk.j * msz + (long long)r0 * Np + c0;
+        ldc = Np;
+        lo = msz;
+    } else {
+        const long long ktot = (long long)c.ntu * TILE * TILE;
+        base = c.stk_off + 2 * wk.j * ktot + (long long)upper_tile_index(r0 / TILE, c0 / TILE, c.nt) * TILE * TILE;
+        ldc = TILE;
+        lo = ktot;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float v = acc[x][y][u];
+                const __bf16 h = (__bf16)v;
+                const __bf16 l = (__bf16)(v - (float)h);
+                const long long o = base + (long long)(16 * x + 4 * (lane / 16) + u) * ldc + 16 * y + lane % 16;
+                C[o] = h;
+                C[o + lo] = l;
+            }
+}
+
+void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
+                const void *A, const void *B, void *C) {
+    if (nwork <= 0) return;
+    const int grid = 8 * ((nwork + 7) / 8);
+    const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
+    if (mode == MODE_LEFT)
+        k_gemm_bf3<MODE_LEFT><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+    else if (mode == MODE_RIGHT)
+        k_gemm_bf3<MODE_RIGHT><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+    else
+        k_gemm_bf3<MODE_GRAM><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
 }
 
 }  // namespace qml

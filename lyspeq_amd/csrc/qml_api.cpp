@@ -200,7 +200,7 @@ int qml_set_bins(qml_ctx *c, const qml_bins *b) {
     if (!c || !b) return fail(QML_ERR_ARG, "qml_set_bins: null argument");
     if (c->committed || !c->chunks.empty() || !c->tables.empty())
         return fail(QML_ERR_ARG, "qml_set_bins: bins must be set before tables and chunks");
-    if (b->nk <= 0 || b->nk > MAX_NK || b->nz <= 0 || b->nz > MAX_ZB || !b->kband_edges || !b->zbin_centers ||
+    if (b->nk <= 0 || b->nk > MAX_NK - 1 || b->nz <= 0 || b->nz > MAX_ZB || !b->kband_edges || !b->zbin_centers ||
         !(b->z_bin_width > 0))
         return fail(QML_ERR_ARG, "qml_set_bins: invalid bins");
     c->bins = *b;
@@ -540,9 +540,11 @@ int qml_commit(qml_ctx *c) {
             d.th_off = d.at_off + (long long)ch.m * MAX_NU * ch.Np;
             d.nvu = ch.nvu;
             d.sj_off = (long long)acc.sj;
-            d.stk_off = (long long)acc.stk;
+            // the split form stores two bf16 planes per element: offsets in bf16 units
+            const long long su = (c->prec == 2) ? 2 : 1;
+            d.stk_off = (long long)acc.stk * su;
             d.gram_off = (long long)acc.gram;
-            d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic;
+            d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic * su;
             acc.mat += n.mat; acc.vec += n.vec; acc.pj += n.pj; acc.sj += n.sj;
             acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram;
             cd.push_back(d);
@@ -572,15 +574,18 @@ int qml_commit(qml_ctx *c) {
             const ChunkDev &d = cd[li];
             B.np_max = std::max(B.np_max, d.Np);
             B.nvu_max = std::max(B.nvu_max, d.nvu);
+            // tiles of 64 (FP64, FP32) or 128 (split BF16) output rows/columns
+            const int g = (c->prec == 2) ? 2 : 1;
+            const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
             for (int j = 0; j < d.m; ++j)
-                for (int I = 0; I < d.nt; ++I)
-                    for (int J = 0; J < d.nt; ++J) wl.push_back({li, j, I, J});
+                for (int I = 0; I < nt; ++I)
+                    for (int J = 0; J < nt; ++J) wl.push_back({li, j, I, J});
             for (int j = 0; j < d.m; ++j)
-                for (int I = 0; I < d.nt; ++I)
-                    for (int J = I; J < d.nt; ++J) wr.push_back({li, j, I, J});
+                for (int I = 0; I < nt; ++I)
+                    for (int J = I; J < nt; ++J) wr.push_back({li, j, I, J});
             for (int s = 0; s < d.ntu; ++s)
-                for (int I = 0; I < d.mt; ++I)
-                    for (int J = I; J < d.mt; ++J) wg.push_back({li, s, I, J});
+                for (int I = 0; I < mtg; ++I)
+                    for (int J = I; J < mtg; ++J) wg.push_back({li, s, I, J});
         }
         B.nwl = (int)wl.size();
         B.nwr = (int)wr.size();
@@ -640,7 +645,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         launch_small(L);
         HIPCHK(hipEventRecord(B.ev[6], c->st));
         launch_xgen(L, c->d_X, c->prec);
-        if (c->prec != 0) launch_convert_li(L, c->d_lic);
+        if (c->prec != 0) launch_convert_li(L, c->d_lic, c->prec);
         HIPCHK(hipEventRecord(B.ev[7], c->st));
         launch_gemm_whiten_left(c->st, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
         HIPCHK(hipEventRecord(B.ev[8], c->st));

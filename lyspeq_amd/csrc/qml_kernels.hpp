@@ -28,7 +28,7 @@ void launch_solve(const LaunchCtx &L);
 void launch_dpass(const LaunchCtx &L);
 void launch_small(const LaunchCtx &L);
 void launch_xgen(const LaunchCtx &L, void *X, int prec);
-void launch_convert_li(const LaunchCtx &L, void *lic);
+void launch_convert_li(const LaunchCtx &L, void *lic, int prec);
 void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double *gram, const int *status,
                           double *fc, double *dbt);
 void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, int nchunks, int T,

@@ -87,10 +87,21 @@ def run_engine(w, precision, steps, warmup, scratch_gb):
         eng.iterate(theta, F, d)
         times.append(time.perf_counter() - t0)
         log(f"[{precision}] step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms")
+    # one more (untimed) iteration with the two stages serialised on one
+    # stream: its per-kernel event durations are those of each kernel alone
+    eng.set_pipeline(False)
+    F2, d2 = np.zeros((T, T)), np.zeros((3, T))
+    t0 = time.perf_counter()
+    eng.iterate(theta, F2, d2)
+    serial_ms = (time.perf_counter() - t0) * 1e3
+    eng.set_pipeline(True)
+    log(f"[{precision}] serialised iteration for kernel timings: {serial_ms:.1f} ms")
+    if not (np.array_equal(F2, F) and np.array_equal(d2, d)):
+        raise RuntimeError("serialised and overlapped iterations differ")
     tm = eng.timings()
     shapes = chunk_shapes(eng)
     eng.close()
-    return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes)
+    return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms)
 
 
 def cpu_baseline(w, budget_s=20.0, max_chunks=48):

@@ -173,6 +173,14 @@ int qml_get_chunk_result(qml_ctx *ctx, int chunk_id, double *fisher, double *dbt
 
 int qml_get_timings(qml_ctx *ctx, qml_timings *t);
 
+/* Stage overlap of qml_iterate (default on): the factorisation and solves of
+ * batch b+1 run on one HIP stream while the dense products of batch b run on
+ * a second.  Off serialises both stages on one stream, so that the per-kernel
+ * durations in qml_timings are those of each kernel alone.  Results are
+ * identical either way.  No counterpart in the reference (its chunk loop is
+ * serial per MPI rank). */
+int qml_set_pipeline(qml_ctx *ctx, int on);
+
 /* Number of chunks registered, and TOTAL_KZ_BINS. */
 int qml_num_chunks(qml_ctx *ctx);
 int qml_total_kz_bins(qml_ctx *ctx);

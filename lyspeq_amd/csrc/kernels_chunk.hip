@@ -583,168 +583,247 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
 }
 
 // ------------------------------------------------------- stack generation --
-// Stores of one stack element in the compute type.  __bf16 means the split
-// form: hi = bf16(v) in plane 2j and lo = bf16(v - hi) in plane 2j+1.
-template <class OutT> struct XStore;
-template <> struct XStore<double> {
-    static __device__ __forceinline__ void put(double *X, size_t msz, int j, size_t off, double v) {
-        X[j * msz + off] = v;
+// Stores of four consecutive elements of one row in the compute type.
+// __bf16 means the split form: hi = bf16(v) in plane 2j and lo = bf16(v - hi)
+// in plane 2j+1.
+template <class OutT> struct XStore4;
+template <> struct XStore4<double> {
+    static __device__ __forceinline__ void put(double *X, size_t msz, int j, size_t off, const double *v) {
+        double *p = X + j * msz + off;
+        *(double2 *)p = double2{v[0], v[1]};
+        *(double2 *)(p + 2) = double2{v[2], v[3]};
     }
 };
-template <> struct XStore<float> {
-    static __device__ __forceinline__ void put(float *X, size_t msz, int j, size_t off, float v) {
-        X[j * msz + off] = v;
+template <> struct XStore4<float> {
+    static __device__ __forceinline__ void put(float *X, size_t msz, int j, size_t off, const float *v) {
+        *(float4 *)(X + j * msz + off) = float4{v[0], v[1], v[2], v[3]};
     }
 };
-template <> struct XStore<__bf16> {
-    static __device__ __forceinline__ void put(__bf16 *X, size_t msz, int j, size_t off, float v) {
-        const __bf16 h = (__bf16)v;
-        const __bf16 l = (__bf16)(v - (float)h);
-        X[2 * j * msz + off] = h;
-        X[(2 * j + 1) * msz + off] = l;
+template <> struct XStore4<__bf16> {
+    static __device__ __forceinline__ void put(__bf16 *X, size_t msz, int j, size_t off, const float *v) {
+        typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+        bf4 h, l;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            h[u] = (__bf16)v[u];
+            l[u] = (__bf16)(v[u] - (float)h[u]);
+        }
+        *(bf4 *)(X + 2 * j * msz + off) = h;
+        *(bf4 *)(X + (2 * j + 1) * msz + off) = l;
     }
 };
 
 // X_j = D (Q_j - U At_j^T - At_j U^T) D for every member, in the compute
 // type, with arithmetic in CT (FP64 for the FP64 path, FP32 otherwise).
-// A workgroup covers (4 EPT) rows x 64 columns; each thread owns one column
-// b and EPT rows, so stores are coalesced along b.  Per element the D
-// scaling is folded into the z-bin weights (sw) and template coefficients
-// (a, bq), so one member costs 1 + 2 nu multiply-adds.  For each k band the
-// table slice reached by the tile and the bands' At rows/columns are staged
-// in LDS (double-buffered); a slice longer than XG_SLICE is read from global.
-constexpr int XG_SLICE = 2048;
-template <class OutT, class CT, int EPT>
+//
+// A workgroup covers 16 EPT rows x 64 columns (EPT = 2 in FP32, 1 in
+// FP64); thread t owns the four consecutive columns 4(t%16).. and rows
+// t/16 + 16e, so each row's four values leave in one packed store.  NUM
+// bounds the chunks' template count at compile time.  Per element the pair geometry
+// (table index and fraction, D_i D_b, the at most two consecutive z bins of
+// the chunk with nonzero weight) is computed once; one member then costs a
+// table interpolation (shared by the band's z bins) and 2 nu multiply-adds.
+// For each k band the table slice reached by the tile and the band's At
+// rows/columns are staged in LDS, double-buffered: the next band's values
+// are loaded into registers before the current band is computed and written
+// to LDS after it, so their latency overlaps the work (values beyond the
+// prefetch registers are copied synchronously).  A slice longer than the LDS
+// buffer is read from global memory instead.
+template <class OutT, class CT, int NUM>
 __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *in,
                                               const TableDev *tabs, BinsDev bins,
                                               const double *mat, const double *aux, OutT *X) {
-    constexpr int TR = 4 * EPT;   // rows per tile
+    constexpr int EPT = (sizeof(CT) == 4) ? 2 : 1;               // row passes of 16
+    constexpr int TR = 16 * EPT, TC = 64;
+    constexpr int SL = (sizeof(CT) == 4) ? 2048 : 1024;        // slice buffer
+    constexpr int NSL = 4, NAT = 2;   // prefetch registers: slice values, At values
     const ChunkDev c = cs[blockIdx.z];
-    const int i0 = blockIdx.y * TR, b0 = blockIdx.x * 64;
+    const int i0 = blockIdx.y * TR, b0 = blockIdx.x * TC;
     if (i0 >= c.Np || b0 >= c.Np) return;
-    const int Np = c.Np, nu = c.nu, tid = threadIdx.x, nzq = c.nzq;
-    const int rb = tid / 64, b = b0 + tid % 64;
+    const int Np = c.Np, N = c.N, nu = c.nu, tid = threadIdx.x, nzq = c.nzq, kbu = c.kbu;
+    const int cg = tid % 16, rl = tid / 16;
+    const int bb = b0 + 4 * cg;            // first of the thread's columns
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)Np;
     const double *D = in + c.in_off + IN_D * (size_t)Np;
     const double *Npr = in + c.in_off + IN_NPRIME * (size_t)Np;
+    const double *U = in + c.in_off + IN_U0 * (size_t)Np;
     const double *S = mat + c.mat_off + 3 * (size_t)Np * Np;
     OutT *Xc = X + c.stk_off;
     const size_t msz = (size_t)Np * Np;
-    __shared__ CT slice[2][XG_SLICE];
-    __shared__ CT atc[2][DP_ZB][MAX_NU][64];     // At of the band's members at the tile's columns
-    __shared__ CT atr[2][DP_ZB][MAX_NU][TR];     // ... and at the tile's rows
+    __shared__ CT slice[2][SL];
+    __shared__ CT atv[2][DP_ZB * MAX_NU * (TC + TR)];   // [m][x][64 columns | 32 rows]
     __shared__ int red_lo[4], red_hi[4];
-    int iv[EPT], ii[EPT];
-    bool ok[EPT];
-    CT fr[EPT], sw[EPT][DP_ZB], ca[EPT][MAX_NU], cb[EPT][MAX_NU];
+
+    // pair geometry
+    int iv[EPT][4], mlo[EPT][4];
+    CT fr[EPT][4], sc[EPT][4], wlo[EPT][4], whi[EPT][4];
+    CT ur[EPT][NUM > 0 ? NUM : 1], ucl[4][NUM > 0 ? NUM : 1];
     int lo = 1 << 30, hi = -1;
 #pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int x = 0; x < NUM; ++x) ucl[u][x] = (x < nu && bb + u < N) ? (CT)U[x * (size_t)Np + bb + u] : (CT)0;
+#pragma unroll
     for (int e = 0; e < EPT; ++e) {
-        const int i = i0 + rb + 4 * e;
-        ii[e] = i;
-        ok[e] = (i < c.N && b < c.N);
-        double v = 0.0, z = 0.0, sc = 0.0;
-        if (ok[e]) {
-            vz_pair(wave[i], wave[b], i, b, v, z);
-            sc = D[i] * D[b];
-        }
-        const double xx = (v - t.v1) / t.dv;
-        iv[e] = ok[e] ? interp_index(xx, t.nv) : 0;
-        fr[e] = ok[e] ? (CT)(xx - iv[e]) : (CT)0;
-        if (ok[e]) { lo = min(lo, iv[e]); hi = max(hi, iv[e]); }
+        const int i = i0 + rl + 16 * e;
 #pragma unroll
-        for (int m = 0; m < DP_ZB; ++m) sw[e][m] = (ok[e] && m < nzq) ? (CT)(sc * zweight(bins, z, c.zm0 + m)) : (CT)0;
+        for (int x = 0; x < NUM; ++x) ur[e][x] = (x < nu && i < N) ? (CT)U[x * (size_t)Np + i] : (CT)0;
 #pragma unroll
-        for (int q = 0; q < MAX_NU; ++q) {
-            const bool on = ok[e] && q < nu;
-            ca[e][q] = on ? (CT)(sc * in[c.in_off + (IN_U0 + q) * (size_t)Np + i]) : (CT)0;
-            cb[e][q] = on ? (CT)(sc * in[c.in_off + (IN_U0 + q) * (size_t)Np + b]) : (CT)0;
+        for (int u = 0; u < 4; ++u) {
+            const int b = bb + u;
+            const bool ok = (i < N && b < N);
+            double v = 0.0, z = 0.0;
+            if (ok) vz_pair(wave[i], wave[b], i, b, v, z);
+            const double xx = (v - t.v1) / t.dv;
+            iv[e][u] = ok ? interp_index(xx, t.nv) : 0;
+            fr[e][u] = ok ? (CT)(xx - iv[e][u]) : (CT)0;
+            sc[e][u] = ok ? (CT)(D[i] * D[b]) : (CT)0;
+            if (ok) { lo = min(lo, iv[e][u]); hi = max(hi, iv[e][u]); }
+            // the chunk's z bins [zm0, zm0 + nzq) with nonzero weight are at
+            // most two consecutive ones (triangular or top-hat weights)
+            int m0 = nzq;
+            CT w0 = 0, w1 = 0;
+            if (ok) {
+                for (int m = 0; m < nzq; ++m) {
+                    const double w = zweight(bins, z, c.zm0 + m);
+                    if (w != 0.0) {
+                        if (m0 == nzq) { m0 = m; w0 = (CT)w; }
+                        else if (m == m0 + 1) w1 = (CT)w;
+                    }
+                }
+            }
+            mlo[e][u] = m0;
+            wlo[e][u] = w0;
+            whi[e][u] = w1;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
         lo = min(lo, __shfl_xor(lo, o, 64));
         hi = max(hi, __shfl_xor(hi, o, 64));
     }
-    if (tid % 64 == 0) { red_lo[rb] = lo; red_hi[rb] = hi; }
+    if (tid % 64 == 0) { red_lo[tid / 64] = lo; red_hi[tid / 64] = hi; }
     __syncthreads();
     lo = min(min(red_lo[0], red_lo[1]), min(red_lo[2], red_lo[3]));
     hi = max(max(red_hi[0], red_hi[1]), max(red_hi[2], red_hi[3]));
     const int len = (hi >= lo) ? hi - lo + 2 : 0;
-    const bool staged = len <= XG_SLICE;
+    const bool staged = len <= SL;
     if (hi < lo) lo = 0;
+    const int nat = nzq * nu * (TC + TR);
 
-    // stage band kn into buffer bf: table slice and the At of members (m, kn)
-    auto stage = [&](int kn, int bf) {
-        if (staged) {
-            const double *row = t.qt + (size_t)kn * t.nv + lo;
-            for (int x = tid; x < len; x += 256) slice[bf][x] = (CT)row[x];
+    // register prefetch of band kn, then its store into LDS buffer bf
+    CT rs[NSL], ra[NAT];
+    auto at_value = [&](int kn, int f) {
+        const int m = f / (nu * (TC + TR)), x = (f / (TC + TR)) % nu, p = f % (TC + TR);
+        const double *At = aux + c.at_off + ((size_t)(m * kbu + kn) * MAX_NU + x) * Np;
+        return (CT)At[p < TC ? b0 + p : i0 + p - TC];
+    };
+    auto fetch = [&](int kn) {
+        const double *row = t.qt + (size_t)kn * t.nv + lo;
+#pragma unroll
+        for (int r = 0; r < NSL; ++r) {
+            const int x = tid + 256 * r;
+            rs[r] = (staged && x < len) ? (CT)row[x] : (CT)0;
         }
-        for (int x = tid; x < nzq * nu * (64 + TR); x += 256) {
-            const int m = x / (nu * (64 + TR)), rem = x % (nu * (64 + TR));
-            const int q = rem / (64 + TR), p = rem % (64 + TR);
-            const double *At = aux + c.at_off + ((size_t)(m * c.kbu + kn) * MAX_NU + q) * Np;
-            if (p < 64) atc[bf][m][q][p] = (CT)At[b0 + p];
-            else atr[bf][m][q][p - 64] = (i0 + p - 64 < Np) ? (CT)At[i0 + p - 64] : (CT)0;
+#pragma unroll
+        for (int r = 0; r < NAT; ++r) {
+            const int f = tid + 256 * r;
+            ra[r] = (f < nat) ? at_value(kn, f) : (CT)0;
         }
     };
-    if (c.kbu > 0) stage(0, 0);
-    for (int kn = 0; kn < c.kbu; ++kn) {
-        const int cur = kn & 1;
-        __syncthreads();
-        if (kn + 1 < c.kbu) stage(kn + 1, cur ^ 1);
-        CT q[EPT];
+    auto put = [&](int kn, int bf) {
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) {
-            CT y0, y1;
-            if (staged) {
-                const int o = ok[e] ? iv[e] - lo : 0;   // in [0, len - 2] for valid elements
-                y0 = slice[cur][o];
-                y1 = slice[cur][o + 1];
-            } else {
-                const double *row = t.qt + (size_t)kn * t.nv;
-                y0 = (CT)row[iv[e]];
-                y1 = (CT)row[iv[e] + 1];
-            }
-            q[e] = y0 * ((CT)1 - fr[e]) + y1 * fr[e];
+        for (int r = 0; r < NSL; ++r) {
+            const int x = tid + 256 * r;
+            if (staged && x < len) slice[bf][x] = rs[r];
         }
 #pragma unroll
-        for (int m = 0; m < DP_ZB; ++m) {
-            if (m >= nzq) break;
-            CT ac[MAX_NU];
+        for (int r = 0; r < NAT; ++r) {
+            const int f = tid + 256 * r;
+            if (f < nat) atv[bf][f] = ra[r];
+        }
+        if (staged && len > 256 * NSL) {
+            const double *row = t.qt + (size_t)kn * t.nv + lo;
+            for (int x = tid + 256 * NSL; x < len; x += 256) slice[bf][x] = (CT)row[x];
+        }
+        for (int f = tid + 256 * NAT; f < nat; f += 256) atv[bf][f] = at_value(kn, f);
+    };
+    if (kbu > 0) {
+        fetch(0);
+        put(0, 0);
+    }
+    __syncthreads();
+    for (int kn = 0; kn < kbu; ++kn) {
+        const int cur = kn & 1;
+        if (kn + 1 < kbu) fetch(kn + 1);
+        CT q[EPT][4];
+        const double *grow = t.qt + (size_t)kn * t.nv;
 #pragma unroll
-            for (int x = 0; x < MAX_NU; ++x) ac[x] = (x < nu) ? atc[cur][m][x][tid % 64] : (CT)0;
-            const int j = m * c.kbu + kn;
+        for (int e = 0; e < EPT; ++e)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                CT y0, y1;
+                if (staged) {
+                    const int o = iv[e][u] - lo;   // in [0, len - 2] for valid elements, 0 otherwise
+                    y0 = slice[cur][o > 0 ? o : 0];
+                    y1 = slice[cur][o > 0 ? o + 1 : 1];
+                } else {
+                    y0 = (CT)grow[iv[e][u]];
+                    y1 = (CT)grow[iv[e][u] + 1];
+                }
+                q[e][u] = y0 * ((CT)1 - fr[e][u]) + y1 * fr[e][u];
+            }
+        for (int m = 0; m < nzq; ++m) {
+            const CT *ab = &atv[cur][m * nu * (TC + TR)];
+            CT ac[4][NUM > 0 ? NUM : 1];
+#pragma unroll
+            for (int x = 0; x < NUM; ++x)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) ac[u][x] = (x < nu) ? ab[x * (TC + TR) + 4 * cg + u] : (CT)0;
+            const int j = m * kbu + kn;
 #pragma unroll
             for (int e = 0; e < EPT; ++e) {
-                CT val = q[e] * sw[e][m];
+                CT ar[NUM > 0 ? NUM : 1];
 #pragma unroll
-                for (int x = 0; x < MAX_NU; ++x)
-                    if (x < nu) val -= ca[e][x] * ac[x] + atr[cur][m][x][rb + 4 * e] * cb[e][x];
-                if (ii[e] < Np) XStore<OutT>::put(Xc, msz, j, (size_t)ii[e] * Np + b, ok[e] ? val : (CT)0);
+                for (int x = 0; x < NUM; ++x) ar[x] = (x < nu) ? ab[x * (TC + TR) + TC + rl + 16 * e] : (CT)0;
+                CT val[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const CT w = (m == mlo[e][u]) ? wlo[e][u] : ((m == mlo[e][u] + 1) ? whi[e][u] : (CT)0);
+                    CT acc = q[e][u] * w;
+#pragma unroll
+                    for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
+                    val[u] = sc[e][u] * acc;
+                }
+                XStore4<OutT>::put(Xc, msz, j, (size_t)(i0 + rl + 16 * e) * Np + bb, val);
             }
         }
+        if (kn + 1 < kbu) put(kn + 1, cur ^ 1);
+        __syncthreads();
     }
-    // the noise (H) and signal (T) members
+    // the noise (H) and signal (T) members, in FP64 arithmetic
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int j = c.n + h;
         const double *At = aux + c.at_off + (size_t)j * MAX_NU * Np;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-            if (ii[e] >= Np) continue;
-            double val = 0.0;
-            if (ok[e]) {
-                const double sc = D[ii[e]] * D[b];
-                const double base = (h == 0) ? ((ii[e] == b) ? Npr[b] : 0.0) : S[(size_t)ii[e] * Np + b];
-                double corr = 0.0;
-                for (int x = 0; x < nu; ++x) {
-                    corr += in[c.in_off + (IN_U0 + x) * (size_t)Np + ii[e]] * At[(size_t)x * Np + b]
-                          + At[(size_t)x * Np + ii[e]] * in[c.in_off + (IN_U0 + x) * (size_t)Np + b];
+            const int i = i0 + rl + 16 * e;
+            CT val[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int b = bb + u;
+                double vv = 0.0;
+                if (i < N && b < N) {
+                    const double base = (h == 0) ? ((i == b) ? Npr[b] : 0.0) : S[(size_t)i * Np + b];
+                    double corr = 0.0;
+                    for (int x = 0; x < nu; ++x)
+                        corr += U[x * (size_t)Np + i] * At[(size_t)x * Np + b] + At[(size_t)x * Np + i] * U[x * (size_t)Np + b];
+                    vv = D[i] * D[b] * (base - corr);
                 }
-                val = sc * (base - corr);
+                val[u] = (CT)vv;
             }
-            XStore<OutT>::put(Xc, msz, j, (size_t)ii[e] * Np + b, (CT)val);
+            XStore4<OutT>::put(Xc, msz, j, (size_t)i * Np + bb, val);
         }
     }
 }
@@ -854,20 +933,24 @@ void launch_dpass(const LaunchCtx &L) {
     k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
 }
 void launch_small(const LaunchCtx &L) { k_small<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj); }
+template <class OutT, class CT>
+static void launch_xgen_t(const LaunchCtx &L, OutT *X) {
+    const int tr = (sizeof(CT) == 4) ? 32 : 16;
+    dim3 grid(L.np_max / 64, L.np_max / tr, L.nchunks);
+    if (L.nu_max <= 0)
+        k_xgen<OutT, CT, 0><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+    else if (L.nu_max <= 2)
+        k_xgen<OutT, CT, 2><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+    else
+        k_xgen<OutT, CT, MAX_NU><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+}
 void launch_xgen(const LaunchCtx &L, void *X, int prec) {
-    if (prec == 0) {
-        dim3 grid(L.np_max / 64, L.np_max / 8, L.nchunks);
-        k_xgen<double, double, 2><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
-                                                             (double *)X);
-    } else {
-        dim3 grid(L.np_max / 64, L.np_max / 16, L.nchunks);
-        if (prec == 1)
-            k_xgen<float, float, 4><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
-                                                               (float *)X);
-        else
-            k_xgen<__bf16, float, 4><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj,
-                                                                (__bf16 *)X);
-    }
+    if (prec == 0)
+        launch_xgen_t<double, double>(L, (double *)X);
+    else if (prec == 1)
+        launch_xgen_t<float, float>(L, (float *)X);
+    else
+        launch_xgen_t<__bf16, float>(L, (__bf16 *)X);
 }
 void launch_convert_li(const LaunchCtx &L, void *lic, int prec) {
     dim3 grid(64, L.nchunks);

@@ -372,27 +372,58 @@ void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Wo
 
 namespace qml {
 
-// ------------------------------------------- split-BF16x3, 128x128 tiles --
+// ------------------------------------------------- split-BF16x3 GEMMs --
 // Operands are stored pre-split: element (r, k) of an operand is hi[r*ld + k]
 // + hi[lo_off + r*ld + k] (two bf16 planes), so the main loop moves bytes
 // and issues MFMAs only.  Every product is A B^T with both operands K-major
-// (X_j is symmetric; Li^T and the Gram are NT by construction).  A
-// workgroup computes a 128x128 tile; each wave a 64x64 quarter of 4x4
-// 16x16 blocks, three v_mfma_f32_16x16x32_bf16 per block and K step
-// (lo*hi, hi*lo, hi*hi).  Waves whose quarter lies outside the matrix, or
-// strictly below the diagonal for the upper-only products, skip their MFMAs.
-constexpr int BT = 128, BK = 32, BLD = 40;
+// (X_j is symmetric; Li^T and the Gram are NT by construction).
+//
+// A workgroup computes a BM x BN tile with one wave per 64x64 quarter
+// (4x4 blocks of 16x16, three v_mfma_f32_16x16x32_bf16 per block and K step:
+// lo*hi, hi*lo, hi*hi).  The whitening products use 128x256 (left) and
+// 256x128 (right) tiles of eight waves: the triangular operand keeps the
+// 128 granularity of its K range, and the bytes staged per MFMA fall by a
+// quarter against 128x128.  The Gram uses 128x128.  Waves whose quarter lies
+// outside the matrix, or strictly below the diagonal for the upper-only
+// products, skip their MFMAs (a separate copy of the loop, so the
+// accumulators stay in AGPRs).
+//
+// LDS: per stage four planes (A hi, A lo, B hi, B lo) of rows x 32 bf16
+// (64 bytes, unpadded).  The four 16-byte chunks of row r sit at slot
+// c ^ ((r >> 1) & 3): any 8 consecutive rows read at one chunk, and the
+// staging writes (two rows' four chunks per 8 lanes), then fall on 8
+// distinct 16-byte slots of a 128-byte bank line.
+//
+// The MFMA is issued as B-fragment x A-fragment, so that each lane ends with
+// four consecutive columns of one output row (8-byte bf16 / 32-byte FP64
+// stores) instead of four rows of one column.
+constexpr int BK = 32;
+
+__device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ ((r >> 1) & 3)) << 3); }
+
+template <int MODE> struct Bf3Shape;
+template <> struct Bf3Shape<MODE_LEFT> { static constexpr int BM = 128, BN = 128; };
+template <> struct Bf3Shape<MODE_RIGHT> { static constexpr int BM = 128, BN = 128; };
+template <> struct Bf3Shape<MODE_GRAM> { static constexpr int BM = 128, BN = 128; };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork,
-                                                  const __bf16 *Abase, const __bf16 *Bbase, void *Cbase) {
+__global__ __launch_bounds__(64 * (Bf3Shape<MODE>::BM / 64) * (Bf3Shape<MODE>::BN / 64))
+void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *Abase, const __bf16 *Bbase,
+                void *Cbase) {
+    constexpr int BM = Bf3Shape<MODE>::BM, BN = Bf3Shape<MODE>::BN;
+    constexpr int WN = BN / 64;                 // waves along N
+    constexpr int NTH = 64 * (BM / 64) * WN;    // threads
+    constexpr int RP = NTH / 4;                 // rows staged per pass
+    constexpr int PA = BM / RP, PB = BN / RP;   // staging passes per plane
+    constexpr int PLA = BM * BK, PLB = BN * BK; // elements per LDS plane
+    constexpr int STG = 2 * (PLA + PLB);        // elements per stage
     const int li = xcd_logical(nwork);
     if (li >= nwork) return;
     const Work wk = work[li];
     const ChunkDev c = cs[wk.c];
     const int Np = c.Np;
     const long long msz = (long long)Np * Np;
-    const int I0 = wk.I * BT, J0 = wk.J * BT;
+    const int I0 = wk.I * BM, J0 = wk.J * BN;
     const __bf16 *Ahi, *Bhi;
     long long lda, ldb, alo, blo;
     int arows, brows, kbeg = 0, kend, Mv, Nv;
@@ -402,7 +433,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
         lda = ldb = Np;
         alo = blo = msz;
         arows = brows = Np;
-        kend = min(Np, I0 + BT);
+        kend = min(Np, I0 + BM);
         Mv = Nv = Np;
     } else if (MODE == MODE_RIGHT) {
         Ahi = Abase + c.stk_off + 2 * wk.j * msz;
@@ -410,7 +441,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
         lda = ldb = Np;
         alo = blo = msz;
         arows = brows = Np;
-        kend = min(Np, J0 + BT);
+        kend = min(Np, J0 + BN);
         Mv = Nv = Np;
     } else {
         const long long ktot = (long long)c.ntu * TILE * TILE;
@@ -422,10 +453,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
         kend = kbeg + TILE * TILE;
         Mv = Nv = c.m;
     }
-    __shared__ __attribute__((aligned(16))) __bf16 sA[2][2][BT][BLD];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[2][2][BT][BLD];
+    __shared__ __attribute__((aligned(16))) __bf16 smem[2 * STG];
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int wm = w / 2, wn = w % 2;
+    const int wm = w / WN, wn = w % WN;
     const int r0 = I0 + 64 * wm, c0 = J0 + 64 * wn;
     bool active = (r0 < Mv) && (c0 < Nv);
     if (MODE != MODE_LEFT) active = active && (c0 >= r0);
@@ -434,69 +464,101 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
-    // global -> register staging: thread t owns row t/2, k-half (t%2)*16
-    const int lr = tid / 2, lk = (tid % 2) * 16;
-    const bool aok = (I0 + lr) < arows, bok = (J0 + lr) < brows;
-    const __bf16 *pa = Ahi + (long long)(aok ? I0 + lr : 0) * lda + lk;
-    const __bf16 *pb = Bhi + (long long)(bok ? J0 + lr : 0) * ldb + lk;
-    const uint4 z4 = {0u, 0u, 0u, 0u};
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define QML_GLOAD(k0)                                                  \
-    ra0 = aok ? *(const uint4 *)(pa + (k0)) : z4;                      \
-    ra1 = aok ? *(const uint4 *)(pa + (k0) + 8) : z4;                  \
-    ra2 = aok ? *(const uint4 *)(pa + alo + (k0)) : z4;                \
-    ra3 = aok ? *(const uint4 *)(pa + alo + (k0) + 8) : z4;            \
-    rb0 = bok ? *(const uint4 *)(pb + (k0)) : z4;                      \
-    rb1 = bok ? *(const uint4 *)(pb + (k0) + 8) : z4;                  \
-    rb2 = bok ? *(const uint4 *)(pb + blo + (k0)) : z4;                \
-    rb3 = bok ? *(const uint4 *)(pb + blo + (k0) + 8) : z4;
-#define QML_SSTORE(st)                                                 \
-    *(uint4 *)&sA[st][0][lr][lk] = ra0;                                \
-    *(uint4 *)&sA[st][0][lr][lk + 8] = ra1;                            \
-    *(uint4 *)&sA[st][1][lr][lk] = ra2;                                \
-    *(uint4 *)&sA[st][1][lr][lk + 8] = ra3;                            \
-    *(uint4 *)&sB[st][0][lr][lk] = rb0;                                \
-    *(uint4 *)&sB[st][0][lr][lk + 8] = rb1;                            \
-    *(uint4 *)&sB[st][1][lr][lk] = rb2;                                \
-    *(uint4 *)&sB[st][1][lr][lk + 8] = rb3;
-    if (kbeg < kend) {
-        QML_GLOAD(kbeg)
-        QML_SSTORE(0)
-        __syncthreads();
-        int st = 0;
-        const int kq = 8 * (lane / 16), rr = lane % 16;
-        for (int k0 = kbeg; k0 < kend; k0 += BK, st ^= 1) {
-            const bool more = k0 + BK < kend;
-            if (more) {
-                QML_GLOAD(k0 + BK)
-            }
-            if (active) {
-                bf8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    ah[x] = *(const bf8 *)&sA[st][0][64 * wm + 16 * x + rr][kq];
-                    al[x] = *(const bf8 *)&sA[st][1][64 * wm + 16 * x + rr][kq];
-                    bh[x] = *(const bf8 *)&sB[st][0][64 * wn + 16 * x + rr][kq];
-                    bl[x] = *(const bf8 *)&sB[st][1][64 * wn + 16 * x + rr][kq];
-                }
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) {
-                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
-                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
-                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
-                    }
-            }
-            if (more) {
-                QML_SSTORE(st ^ 1)
-            }
-            __syncthreads();
-        }
+    // Staging: thread t loads the 16-byte chunk t%4 of rows t/4 + RP*q, so
+    // four consecutive lanes read one contiguous 64-byte row segment.  Rows
+    // past the operand read row 0 instead: for the whitening products Np is
+    // a multiple of 64, so such rows only feed inactive waves; for the Gram
+    // they feed output rows >= m that are never read.  The loads are
+    // therefore unconditional (no exec branches).
+    static_assert(PA >= 1 && PA <= 2 && PB >= 1 && PB <= 2, "staging passes");
+    const int lr = tid / 4, lc = tid % 4;
+    const auto arow = [&](int q) { const int r = I0 + lr + RP * q; return r < arows ? r : 0; };
+    const auto brow = [&](int q) { const int r = J0 + lr + RP * q; return r < brows ? r : 0; };
+    const __bf16 *pa0 = Ahi + (long long)arow(0) * lda + 8 * lc;
+    const __bf16 *pa1 = Ahi + (long long)arow(PA - 1) * lda + 8 * lc;
+    const __bf16 *pb0 = Bhi + (long long)brow(0) * ldb + 8 * lc;
+    const __bf16 *pb1 = Bhi + (long long)brow(PB - 1) * ldb + 8 * lc;
+    const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
+    const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
+    uint4 rah0, ral0, rah1, ral1, rbh0, rbl0, rbh1, rbl1;
+    // fragment reads: row 16x + lane%16 of the wave's 64-row slab, chunk lane/16
+    const int rr = lane % 16, cq = lane / 16;
+    const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
+#define QML_BF3_GLOAD(k0)                                                        \
+    rah0 = *(const uint4 *)(pa0 + (k0));                                         \
+    ral0 = *(const uint4 *)(pa0 + alo + (k0));                                   \
+    rbh0 = *(const uint4 *)(pb0 + (k0));                                         \
+    rbl0 = *(const uint4 *)(pb0 + blo + (k0));                                   \
+    if constexpr (PA > 1) {                                                      \
+        rah1 = *(const uint4 *)(pa1 + (k0));                                     \
+        ral1 = *(const uint4 *)(pa1 + alo + (k0));                               \
+    }                                                                            \
+    if constexpr (PB > 1) {                                                      \
+        rbh1 = *(const uint4 *)(pb1 + (k0));                                     \
+        rbl1 = *(const uint4 *)(pb1 + blo + (k0));                               \
     }
-#undef QML_GLOAD
-#undef QML_SSTORE
+#define QML_BF3_SSTORE(st)                                                       \
+    {                                                                            \
+        __bf16 *s0 = smem + (st) * STG;                                          \
+        *(uint4 *)(s0 + wa0) = rah0;                                             \
+        *(uint4 *)(s0 + PLA + wa0) = ral0;                                       \
+        *(uint4 *)(s0 + 2 * PLA + wb0) = rbh0;                                   \
+        *(uint4 *)(s0 + 2 * PLA + PLB + wb0) = rbl0;                             \
+        if constexpr (PA > 1) {                                                  \
+            *(uint4 *)(s0 + wa1) = rah1;                                         \
+            *(uint4 *)(s0 + PLA + wa1) = ral1;                                   \
+        }                                                                        \
+        if constexpr (PB > 1) {                                                  \
+            *(uint4 *)(s0 + 2 * PLA + wb1) = rbh1;                               \
+            *(uint4 *)(s0 + 2 * PLA + PLB + wb1) = rbl1;                         \
+        }                                                                        \
+    }
+#define QML_BF3_MFMA(st)                                                                              \
+    {                                                                                                 \
+        const __bf16 *s0 = smem + (st) * STG;                                                         \
+        bf8 ah[4], al[4], bh[4], bl[4];                                                               \
+        _Pragma("unroll") for (int x = 0; x < 4; ++x) {                                               \
+            ah[x] = *(const bf8 *)(s0 + aoff + x * 16 * BK);                                          \
+            al[x] = *(const bf8 *)(s0 + PLA + aoff + x * 16 * BK);                                    \
+            bh[x] = *(const bf8 *)(s0 + 2 * PLA + boff + x * 16 * BK);                                \
+            bl[x] = *(const bf8 *)(s0 + 2 * PLA + PLB + boff + x * 16 * BK);                          \
+        }                                                                                             \
+        _Pragma("unroll") for (int x = 0; x < 4; ++x)                                                 \
+        _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                               \
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al[x], acc[x][y], 0, 0, 0);    \
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah[x], acc[x][y], 0, 0, 0);    \
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah[x], acc[x][y], 0, 0, 0);    \
+        }                                                                                             \
+    }
+// One K sweep: stage k0+BK is fetched into registers while stage k0 is
+// multiplied out of LDS, then stored into the other LDS buffer.
+#define QML_BF3_SWEEP(DO_MFMA)                                         \
+    {                                                                  \
+        QML_BF3_GLOAD(kbeg)                                            \
+        QML_BF3_SSTORE(0)                                              \
+        __syncthreads();                                               \
+        int st = 0;                                                    \
+        for (int k0 = kbeg; k0 < kend; k0 += BK, st ^= 1) {            \
+            const bool more = k0 + BK < kend;                          \
+            if (more) {                                                \
+                QML_BF3_GLOAD(k0 + BK)                                 \
+            }                                                          \
+            if (DO_MFMA) QML_BF3_MFMA(st)                              \
+            if (more) QML_BF3_SSTORE(st ^ 1)                           \
+            __syncthreads();                                           \
+        }                                                              \
+    }
+    if (kbeg < kend) {
+        if (active) QML_BF3_SWEEP(true)
+        else QML_BF3_SWEEP(false)
+    }
+#undef QML_BF3_GLOAD
+#undef QML_BF3_SSTORE
+#undef QML_BF3_MFMA
+#undef QML_BF3_SWEEP
     if (!active) return;
+    // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
+    const int orow = lane % 16, ocol = 4 * (lane / 16);
     if (MODE == MODE_GRAM) {
         const int ntile = c.mt * (c.mt + 1) / 2;
         const double wgt = packed_tile_is_diag(wk.j, c.nt) ? 1.0 : 2.0;
@@ -505,10 +567,14 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y)
+            for (int y = 0; y < 4; ++y) {
+                d4 v;
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    C[(16 * x + 4 * (lane / 16) + u) * TILE + 16 * y + lane % 16] = wgt * (double)acc[x][y][u];
+                for (int u = 0; u < 4; ++u) v[u] = wgt * (double)acc[x][y][u];
+                double *o = C + (16 * x + orow) * TILE + 16 * y + ocol;
+                *(double2 *)o = double2{v[0], v[1]};
+                *(double2 *)(o + 2) = double2{v[2], v[3]};
+            }
         return;
     }
     __bf16 *C = (__bf16 *)Cbase;
@@ -523,19 +589,22 @@ __global__ __launch_bounds__(256) void k_gemm_bf3(const ChunkDev *cs, const Work
         ldc = TILE;
         lo = ktot;
     }
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y)
+        for (int y = 0; y < 4; ++y) {
+            bf4 h, l;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const float v = acc[x][y][u];
-                const __bf16 h = (__bf16)v;
-                const __bf16 l = (__bf16)(v - (float)h);
-                const long long o = base + (long long)(16 * x + 4 * (lane / 16) + u) * ldc + 16 * y + lane % 16;
-                C[o] = h;
-                C[o + lo] = l;
+                h[u] = (__bf16)v;
+                l[u] = (__bf16)(v - (float)h[u]);
             }
+            const long long o = base + (long long)(16 * x + orow) * ldc + 16 * y + ocol;
+            *(bf4 *)(C + o) = h;
+            *(bf4 *)(C + o + lo) = l;
+        }
 }
 
 void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,

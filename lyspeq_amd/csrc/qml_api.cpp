@@ -54,8 +54,9 @@ struct Batch {
     ChunkDev *d_chunks = nullptr;
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
-    int np_max = 0, nvu_max = 0;
-    hipEvent_t ev[13]{};
+    int np_max = 0, nvu_max = 0, nu_max = 0;
+    hipEvent_t evA[7]{};   // stage A (assembly .. solves) on stream st
+    hipEvent_t evB[6]{};   // stage B (stack generation .. Gram) on stream st2
 };
 
 double median_sorted(const std::vector<double> &a) {
@@ -117,7 +118,8 @@ struct qml_ctx {
     int device = 0, prec = 0;
     double scratch_gb = 32.0;
     int max_batch = 256;
-    hipStream_t st = nullptr;
+    int pipeline = 1;
+    hipStream_t st = nullptr, st2 = nullptr;
     bool bins_set = false, committed = false;
     std::vector<double> kedges, kcenters, zcenters;
     qml_bins bins{};
@@ -132,8 +134,12 @@ struct qml_ctx {
     ChunkOut *d_outs = nullptr;
     double *d_fc = nullptr, *d_dbt = nullptr;
     int *d_status = nullptr;
-    double *d_mat = nullptr, *d_vec = nullptr, *d_pj = nullptr, *d_sj = nullptr, *d_gram = nullptr;
-    void *d_X = nullptr, *d_Y = nullptr, *d_lic = nullptr;
+    // stage-A arenas are double-buffered so that batch b+1 is factorised
+    // while batch b runs its dense products
+    double *d_mat[2] = {}, *d_vec[2] = {}, *d_pj[2] = {}, *d_sj[2] = {};
+    void *d_lic[2] = {};
+    double *d_gram = nullptr;
+    void *d_X = nullptr, *d_Y = nullptr;
     double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
     std::vector<Batch> batches;
     hipEvent_t ev_begin{}, ev_end{}, ev_acc{};
@@ -167,6 +173,7 @@ int qml_create(qml_ctx **out, const qml_opts *opts) {
         return fail(QML_ERR_ARG, "qml_create: device ordinal out of range");
     }
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+        hipStreamCreate(&c->st2) != hipSuccess ||
         hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
         hipEventCreate(&c->ev_acc) != hipSuccess) {
         delete c;
@@ -180,18 +187,25 @@ int qml_destroy(qml_ctx *c) {
     if (!c) return QML_OK;
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
+    if (c->st2) hipStreamSynchronize(c->st2);
     for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_s); }
     for (auto &b : c->batches) {
         hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
-        for (auto &e : b.ev) if (e) hipEventDestroy(e);
+        for (auto &e : b.evA) if (e) hipEventDestroy(e);
+        for (auto &e : b.evB) if (e) hipEventDestroy(e);
     }
-    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_mat, c->d_vec,
-                    c->d_pj, c->d_sj, c->d_gram, c->d_X, c->d_Y, c->d_lic, c->d_theta, c->d_F, c->d_dbtsum};
+    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_gram,
+                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum};
     for (void *p : ptrs) if (p) hipFree(p);
+    for (int k = 0; k < 2; ++k) {
+        void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
+        for (void *p : set) if (p) hipFree(p);
+    }
     if (c->ev_begin) hipEventDestroy(c->ev_begin);
     if (c->ev_end) hipEventDestroy(c->ev_end);
     if (c->ev_acc) hipEventDestroy(c->ev_acc);
     if (c->st) hipStreamDestroy(c->st);
+    if (c->st2) hipStreamDestroy(c->st2);
     delete c;
     return QML_OK;
 }
@@ -479,7 +493,8 @@ int qml_commit(qml_ctx *c) {
         return n;
     };
     auto bytes = [&](const Need &n) {
-        return 8 * (n.mat + n.vec + n.pj + n.sj + n.gram) + es * (2 * n.stk + n.lic);
+        // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
+        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
     };
     std::vector<int> order(nch);
     std::iota(order.begin(), order.end(), 0);
@@ -555,15 +570,17 @@ int qml_commit(qml_ctx *c) {
         mx.gram = std::max(mx.gram, acc.gram);
         devs.push_back(cd);
     }
-    HIPCHK(hipMalloc(&c->d_mat, std::max<size_t>(mx.mat, 1) * 8));
-    HIPCHK(hipMalloc(&c->d_vec, std::max<size_t>(mx.vec, 1) * 8));
-    HIPCHK(hipMalloc(&c->d_pj, std::max<size_t>(mx.pj, 1) * 8));
-    HIPCHK(hipMalloc(&c->d_sj, std::max<size_t>(mx.sj, 1) * 8));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipMalloc(&c->d_mat[k], std::max<size_t>(mx.mat, 1) * 8));
+        HIPCHK(hipMalloc(&c->d_vec[k], std::max<size_t>(mx.vec, 1) * 8));
+        HIPCHK(hipMalloc(&c->d_pj[k], std::max<size_t>(mx.pj, 1) * 8));
+        HIPCHK(hipMalloc(&c->d_sj[k], std::max<size_t>(mx.sj, 1) * 8));
+        if (c->prec != 0) HIPCHK(hipMalloc(&c->d_lic[k], std::max<size_t>(mx.lic, 1) * es));
+        HIPCHK(hipMemset(c->d_vec[k], 0, std::max<size_t>(mx.vec, 1) * 8));
+    }
     HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
     HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
     HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
-    if (c->prec != 0) HIPCHK(hipMalloc(&c->d_lic, std::max<size_t>(mx.lic, 1) * es));
-    HIPCHK(hipMemset(c->d_vec, 0, std::max<size_t>(mx.vec, 1) * 8));
 
     for (size_t bi = 0; bi < groups.size(); ++bi) {
         Batch B;
@@ -574,7 +591,8 @@ int qml_commit(qml_ctx *c) {
             const ChunkDev &d = cd[li];
             B.np_max = std::max(B.np_max, d.Np);
             B.nvu_max = std::max(B.nvu_max, d.nvu);
-            // tiles of 64 (FP64, FP32) or 128 (split BF16) output rows/columns
+            B.nu_max = std::max(B.nu_max, d.nu);
+            // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16.
             const int g = (c->prec == 2) ? 2 : 1;
             const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
             for (int j = 0; j < d.m; ++j)
@@ -598,7 +616,8 @@ int qml_commit(qml_ctx *c) {
         if (!wl.empty()) HIPCHK(hipMemcpy(B.d_wl, wl.data(), wl.size() * sizeof(Work), hipMemcpyHostToDevice));
         if (!wr.empty()) HIPCHK(hipMemcpy(B.d_wr, wr.data(), wr.size() * sizeof(Work), hipMemcpyHostToDevice));
         if (!wg.empty()) HIPCHK(hipMemcpy(B.d_wg, wg.data(), wg.size() * sizeof(Work), hipMemcpyHostToDevice));
-        for (auto &e : B.ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : B.evA) HIPCHK(hipEventCreate(&e));
+        for (auto &e : B.evB) HIPCHK(hipEventCreate(&e));
         c->batches.push_back(B);
     }
     c->hF.assign((size_t)c->T * c->T, 0.0);
@@ -612,68 +631,86 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
     if (!theta || !fisher_sum || !dbt_sum) return fail(QML_ERR_ARG, "qml_iterate: null argument");
     HIPCHK(hipSetDevice(c->device));
     const int T = c->T;
+    hipStream_t sB = c->pipeline ? c->st2 : c->st;   // stream of the dense-product stage
     HIPCHK(hipEventRecord(c->ev_begin, c->st));
     HIPCHK(hipMemcpyAsync(c->d_theta, theta, T * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemsetAsync(c->d_F, 0, (size_t)T * T * sizeof(double), c->st));
-    HIPCHK(hipMemsetAsync(c->d_dbtsum, 0, (size_t)3 * T * sizeof(double), c->st));
-    for (auto &B : c->batches) {
+    HIPCHK(hipMemsetAsync(c->d_F, 0, (size_t)T * T * sizeof(double), sB));
+    HIPCHK(hipMemsetAsync(c->d_dbtsum, 0, (size_t)3 * T * sizeof(double), sB));
+    for (size_t bi = 0; bi < c->batches.size(); ++bi) {
+        Batch &B = c->batches[bi];
+        const int set = (int)(bi % 2);
         LaunchCtx L;
         L.stream = c->st;
         L.chunks = B.d_chunks;
         L.nchunks = (int)B.ids.size();
         L.np_max = B.np_max;
         L.nvu_max = B.nvu_max;
+        L.nu_max = B.nu_max;
         L.in = c->d_in;
         L.tables = c->d_tables;
         L.bins = c->binsdev;
-        L.mat = c->d_mat;
-        L.vec = c->d_vec;
-        L.pj = c->d_pj;
-        L.sj = c->d_sj;
-        const void *li = (c->prec == 0) ? (const void *)c->d_mat : c->d_lic;
-        HIPCHK(hipEventRecord(B.ev[0], c->st));
+        L.mat = c->d_mat[set];
+        L.vec = c->d_vec[set];
+        L.pj = c->d_pj[set];
+        L.sj = c->d_sj[set];
+        // stage A on st: this arena set was last read by stage B of batch bi - 2
+        if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
+        HIPCHK(hipEventRecord(B.evA[0], c->st));
         launch_assemble(L, c->d_theta);
-        HIPCHK(hipEventRecord(B.ev[1], c->st));
+        HIPCHK(hipEventRecord(B.evA[1], c->st));
         launch_cholesky(L, c->d_status);
-        HIPCHK(hipEventRecord(B.ev[2], c->st));
+        HIPCHK(hipEventRecord(B.evA[2], c->st));
         launch_trinv(L);
-        HIPCHK(hipEventRecord(B.ev[3], c->st));
+        HIPCHK(hipEventRecord(B.evA[3], c->st));
         launch_solve(L);
-        HIPCHK(hipEventRecord(B.ev[4], c->st));
+        HIPCHK(hipEventRecord(B.evA[4], c->st));
         launch_dpass(L);
-        HIPCHK(hipEventRecord(B.ev[5], c->st));
+        HIPCHK(hipEventRecord(B.evA[5], c->st));
         launch_small(L);
-        HIPCHK(hipEventRecord(B.ev[6], c->st));
-        launch_xgen(L, c->d_X, c->prec);
-        if (c->prec != 0) launch_convert_li(L, c->d_lic, c->prec);
-        HIPCHK(hipEventRecord(B.ev[7], c->st));
-        launch_gemm_whiten_left(c->st, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
-        HIPCHK(hipEventRecord(B.ev[8], c->st));
-        launch_gemm_whiten_right(c->st, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
-        HIPCHK(hipEventRecord(B.ev[9], c->st));
-        launch_gemm_gram(c->st, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
-        HIPCHK(hipEventRecord(B.ev[10], c->st));
-        launch_gram_finalize(L, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
-        HIPCHK(hipEventRecord(B.ev[11], c->st));
+        if (c->prec != 0) launch_convert_li(L, c->d_lic[set], c->prec);
+        HIPCHK(hipEventRecord(B.evA[6], c->st));
+        HIPCHK(hipGetLastError());
+        // stage B on sB, after stage A of this batch
+        LaunchCtx L2 = L;
+        L2.stream = sB;
+        const void *li = (c->prec == 0) ? (const void *)c->d_mat[set] : c->d_lic[set];
+        HIPCHK(hipStreamWaitEvent(sB, B.evA[6], 0));
+        HIPCHK(hipEventRecord(B.evB[0], sB));
+        launch_xgen(L2, c->d_X, c->prec);
+        HIPCHK(hipEventRecord(B.evB[1], sB));
+        launch_gemm_whiten_left(sB, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
+        HIPCHK(hipEventRecord(B.evB[2], sB));
+        launch_gemm_whiten_right(sB, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
+        HIPCHK(hipEventRecord(B.evB[3], sB));
+        launch_gemm_gram(sB, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
+        HIPCHK(hipEventRecord(B.evB[4], sB));
+        launch_gram_finalize(L2, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
+        HIPCHK(hipEventRecord(B.evB[5], sB));
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipEventRecord(c->ev_acc, c->st));
-    launch_accumulate(c->st, c->d_outs, c->d_status, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
+    HIPCHK(hipEventRecord(c->ev_acc, sB));
+    launch_accumulate(sB, c->d_outs, c->d_status, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
                       c->d_dbtsum);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev_end, c->st));
-    HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(c->hdbt.data(), c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, sB));
+    HIPCHK(hipMemcpyAsync(c->hdbt.data(), c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, sB));
+    HIPCHK(hipEventRecord(c->ev_end, sB));
     HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
     for (int i = 0; i < T; ++i)
         for (int j = i; j < T; ++j) fisher_sum[(size_t)i * T + j] += c->hF[(size_t)i * T + j];
     for (int k = 0; k < 3 * T; ++k) dbt_sum[k] += c->hdbt[k];
     qml_timings tm{};
     for (auto &B : c->batches) {
-        for (int k = 0; k < 11; ++k) {
+        for (int k = 0; k < 6; ++k) {
             float ms = 0;
-            hipEventElapsedTime(&ms, B.ev[k], B.ev[k + 1]);
+            hipEventElapsedTime(&ms, B.evA[k], B.evA[k + 1]);
             tm.kernel_ms[k] += ms;
+        }
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, B.evB[k], B.evB[k + 1]);
+            tm.kernel_ms[6 + k] += ms;
         }
     }
     {
@@ -710,6 +747,12 @@ int qml_get_chunk_result(qml_ctx *c, int id, double *fisher, double *dbt, int *s
 int qml_get_timings(qml_ctx *c, qml_timings *t) {
     if (!c || !t) return fail(QML_ERR_ARG, "qml_get_timings: null argument");
     *t = c->tm;
+    return QML_OK;
+}
+
+int qml_set_pipeline(qml_ctx *c, int on) {
+    if (!c) return fail(QML_ERR_ARG, "qml_set_pipeline: null context");
+    c->pipeline = on ? 1 : 0;
     return QML_OK;
 }
 

@@ -11,6 +11,7 @@ struct LaunchCtx {
     int nchunks;
     int np_max;               // largest Np in the batch
     int nvu_max;              // largest theta-table length in the batch
+    int nu_max;               // most continuum templates of a chunk in the batch
     const double *in;         // per-chunk input vectors
     const TableDev *tables;   // device table descriptors
     BinsDev bins;

@@ -267,84 +267,127 @@ __device__ double block_sum(double v, double *red) {
     return s;
 }
 
-// out = Li v (lower), one wave per row.
-__device__ void lower_mv(const double *Li, int Np, const double *v, double *out) {
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    for (int i = w; i < Np; i += 4) {
-        const double *row = Li + (size_t)i * Np;
-        double s = 0.0;
-        for (int k = l; k <= i; k += 64) s += row[k] * v[k];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (l == 0) out[i] = s;
-    }
-    __syncthreads();
-}
-
-// out = Li^T v, one thread per column.
-__device__ void lower_tmv(const double *Li, int Np, const double *v, double *out) {
-    for (int k = threadIdx.x; k < Np; k += 256) {
-        double s = 0.0;
-        for (int i = k; i < Np; ++i) s += Li[(size_t)i * Np + k] * v[i];
-        out[k] = s;
-    }
-    __syncthreads();
-}
-
-// out = A v (dense), one wave per row.
-__device__ void dense_mv(const double *A, int Np, const double *v, double *out) {
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    for (int i = w; i < Np; i += 4) {
-        const double *row = A + (size_t)i * Np;
-        double s = 0.0;
-        for (int k = l; k < Np; k += 64) s += row[k] * v[k];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (l == 0) out[i] = s;
-    }
-    __syncthreads();
-}
-
 // ------------------------------------------------------------------ solves --
 // x = C^-1 delta (one FP64 refinement step on A), W = C^-1 U (U^T C^-1 U)^-1,
-// y = C_marg^-1 delta = x - W U^T x.
-__global__ __launch_bounds__(256) void k_solve(const ChunkDev *cs, const double *in,
-                                               const double *mat, double *vec) {
-    const ChunkDev c = cs[blockIdx.x];
-    const int Np = c.Np, tid = threadIdx.x;
-    const double *A = mat + c.mat_off;
-    const double *Li = A + 2 * (size_t)Np * Np;
+// y = C_marg^-1 delta = x - W U^T x.  With C = D^-1 A D^-1 and A^-1 = Li^T Li,
+// C^-1 v = D Li^T Li D v.  The mat-vecs run over the whole batch, several
+// workgroups per chunk and all right-hand sides (D delta and the D U_t) at
+// once, as a sequence of launches:
+//   R = [D delta, D U]; T = Li R; Z = Li^T T; E = R_0 - A Z_0; F = Li E;
+//   G = Li^T F; x = D (Z_0 + G); W_t = D Z_t; then the small projection.
+
+__global__ __launch_bounds__(256) void k_solve_prep(const ChunkDev *cs, const double *in, double *vec) {
+    const ChunkDev c = cs[blockIdx.y];
+    const int Np = c.Np;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Np) return;
     const double *D = in + c.in_off + IN_D * (size_t)Np;
-    const double *delta = in + c.in_off + IN_DELTA * (size_t)Np;
+    double *V = vec + c.vec_off;
+    V[V_R0 * (size_t)Np + i] = D[i] * in[c.in_off + IN_DELTA * (size_t)Np + i];
+    for (int t = 0; t < c.nu; ++t)
+        V[(V_R0 + 1 + t) * (size_t)Np + i] = D[i] * in[c.in_off + (IN_U0 + t) * (size_t)Np + i];
+}
+
+// out_r = M in_r for the chunk's right-hand sides r (1 + nu of them, or one
+// if single), M = Li (lower triangular) or the dense A.  A workgroup takes 16
+// rows, a wave four of them; lanes stride along the rows (coalesced), with
+// the four rows' loads independent.  With sub >= 0 the result is
+// vec[sub] - M in_0 (the refinement residual).
+template <bool DENSE>
+__global__ __launch_bounds__(256) void k_mv_rows(const ChunkDev *cs, const double *mat, double *vec, int in_slot,
+                                                 int out_slot, int single, int sub) {
+    constexpr int NR = 1 + MAX_NU;
+    const ChunkDev c = cs[blockIdx.y];
+    const int Np = c.Np, i0 = blockIdx.x * 16;
+    if (i0 >= Np) return;
+    const int nr = single ? 1 : 1 + c.nu;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const double *M = mat + c.mat_off + (DENSE ? 0 : 2 * (size_t)Np * Np);
+    double *V = vec + c.vec_off;
+    const double *R = V + in_slot * (size_t)Np;
+    const int r0 = i0 + 4 * w;
+    // rows of Li are zero right of the diagonal
+    const int kend = DENSE ? Np : min(Np, i0 + 16);
+    double s[4][NR];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[q][r] = 0.0;
+    for (int k = l; k < kend; k += 64) {
+        double a[4], v[NR];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = M[(size_t)(r0 + q) * Np + k];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) v[r] = (r < nr) ? R[r * (size_t)Np + k] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s[q][r] += a[q] * v[r];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r >= nr) continue;
+            double t = s[q][r];
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (l == 0) {
+                const size_t o = (size_t)(out_slot + r) * Np + r0 + q;
+                V[o] = (sub >= 0) ? V[sub * (size_t)Np + r0 + q] - t : t;
+            }
+        }
+}
+
+// out_r = Li^T in_r: a workgroup takes 64 columns (lane = column, coalesced
+// row segments); the four waves split the rows i >= k0 and are combined in
+// LDS.
+__global__ __launch_bounds__(256) void k_mv_cols(const ChunkDev *cs, const double *mat, double *vec, int in_slot,
+                                                 int out_slot, int single) {
+    constexpr int NR = 1 + MAX_NU;
+    const ChunkDev c = cs[blockIdx.y];
+    const int Np = c.Np, k0 = blockIdx.x * 64;
+    if (k0 >= Np) return;
+    const int nr = single ? 1 : 1 + c.nu;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const double *Li = mat + c.mat_off + 2 * (size_t)Np * Np;
+    double *V = vec + c.vec_off;
+    const double *R = V + in_slot * (size_t)Np;
+    __shared__ double red[4][NR][64];
+    double s[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] = 0.0;
+    // Li[i][k] is zero for i < k, so rows from k0 on cover the block
+#pragma unroll 4
+    for (int i = k0 + w; i < Np; i += 4) {
+        const double a = Li[(size_t)i * Np + k0 + l];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < nr) s[r] += a * R[r * (size_t)Np + i];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) red[w][r][l] = s[r];
+    __syncthreads();
+    if (w == 0) {
+        for (int r = 0; r < nr; ++r)
+            V[(size_t)(out_slot + r) * Np + k0 + l] = (red[0][r][l] + red[1][r][l]) + (red[2][r][l] + red[3][r][l]);
+    }
+}
+
+// x = D (Z_0 + G), W_t = D Z_t, then M = U^T W, Mi = M^-1 (Gauss-Jordan),
+// W <- W Mi, y = x - W (U^T x).  One workgroup per chunk, O(nu Np) work.
+__global__ __launch_bounds__(256) void k_solve_finish(const ChunkDev *cs, const double *in, double *vec) {
+    const ChunkDev c = cs[blockIdx.x];
+    const int Np = c.Np, tid = threadIdx.x, nu = c.nu;
+    const double *D = in + c.in_off + IN_D * (size_t)Np;
     double *V = vec + c.vec_off;
     double *x = V + V_X * (size_t)Np, *y = V + V_Y * (size_t)Np;
-    double *t0 = V + V_T0 * (size_t)Np, *t1 = V + V_T1 * (size_t)Np, *t2 = V + V_T2 * (size_t)Np;
     __shared__ double red[4];
     __shared__ double M[MAX_NU][MAX_NU], Mi[MAX_NU][MAX_NU], ux[MAX_NU];
-
-    // u = A^-1 (D delta), refined once
-    for (int i = tid; i < Np; i += 256) t0[i] = D[i] * delta[i];
-    __syncthreads();
-    lower_mv(Li, Np, t0, t1);
-    lower_tmv(Li, Np, t1, x);           // x holds u
-    dense_mv(A, Np, x, t2);             // A u
-    for (int i = tid; i < Np; i += 256) t2[i] = t0[i] - t2[i];
-    __syncthreads();
-    lower_mv(Li, Np, t2, t1);
-    lower_tmv(Li, Np, t1, t2);
-    for (int i = tid; i < Np; i += 256) x[i] = D[i] * (x[i] + t2[i]);
-    __syncthreads();
-
-    const int nu = c.nu;
-    // C^-1 U_t into the W slots
-    for (int t = 0; t < nu; ++t) {
-        const double *Ut = in + c.in_off + (IN_U0 + t) * (size_t)Np;
-        double *Wt = V + (V_W0 + t) * (size_t)Np;
-        for (int i = tid; i < Np; i += 256) t0[i] = D[i] * Ut[i];
-        __syncthreads();
-        lower_mv(Li, Np, t0, t1);
-        lower_tmv(Li, Np, t1, t2);
-        for (int i = tid; i < Np; i += 256) Wt[i] = D[i] * t2[i];
-        __syncthreads();
+    for (int i = tid; i < Np; i += 256) {
+        x[i] = D[i] * (V[V_Z0 * (size_t)Np + i] + V[V_G * (size_t)Np + i]);
+        for (int t = 0; t < nu; ++t) V[(V_W0 + t) * (size_t)Np + i] = D[i] * V[(V_Z0 + 1 + t) * (size_t)Np + i];
     }
+    __syncthreads();
     for (int t = 0; t < nu; ++t) {
         const double *Ut = in + c.in_off + (IN_U0 + t) * (size_t)Np;
         for (int s = 0; s < nu; ++s) {
@@ -407,6 +450,7 @@ __global__ __launch_bounds__(256) void k_solve(const ChunkDev *cs, const double 
 // time are staged in LDS per wave.
 constexpr int DP_ZB = 8, DQ_RB = 4, DQ_CB = 3;
 typedef double dq4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double *in,
                                                const TableDev *tabs, BinsDev bins,
                                                const double *mat, const double *vec, double *P) {
@@ -927,7 +971,16 @@ void launch_cholesky(const LaunchCtx &L, int *status) {
     k_cholesky<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat, status);
 }
 void launch_trinv(const LaunchCtx &L) { k_trinv<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat); }
-void launch_solve(const LaunchCtx &L) { k_solve<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.mat, L.vec); }
+void launch_solve(const LaunchCtx &L) {
+    const dim3 gp((L.np_max + 255) / 256, L.nchunks), gr(L.np_max / 16, L.nchunks), gc(L.np_max / 64, L.nchunks);
+    k_solve_prep<<<gp, 256, 0, L.stream>>>(L.chunks, L.in, L.vec);
+    k_mv_rows<false><<<gr, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_R0, V_T0, 0, -1);
+    k_mv_cols<<<gc, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_T0, V_Z0, 0);
+    k_mv_rows<true><<<gr, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_Z0, V_E, 1, V_R0);
+    k_mv_rows<false><<<gr, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_E, V_F, 1, -1);
+    k_mv_cols<<<gc, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_F, V_G, 1);
+    k_solve_finish<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec);
+}
 void launch_dpass(const LaunchCtx &L) {
     dim3 grid((L.np_max + 3) / 4, L.nchunks);
     k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);

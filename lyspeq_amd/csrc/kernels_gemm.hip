@@ -577,6 +577,13 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
             }
         return;
     }
+    // Split the 64x64 FP32 block into bf16 hi/lo planes and write them out
+    // through LDS: the wave stages one plane (64 rows of 128 bytes at a row
+    // pitch of 144 bytes, against bank conflicts) in its own LDS region, then
+    // each store instruction writes eight full 128-byte rows, 16 bytes per
+    // lane.  The main loop ended on a barrier, so the stage buffers are free;
+    // the region is private to the wave and one wave's LDS operations
+    // complete in order, so no further barrier is needed.
     __bf16 *C = (__bf16 *)Cbase;
     long long base, ldc, lo;
     if (MODE == MODE_LEFT) {
@@ -589,22 +596,32 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         ldc = TILE;
         lo = ktot;
     }
+    constexpr int EP = 72;   // row pitch of the staged block, in bf16
+    static_assert(4 * 64 * EP <= 2 * STG, "epilogue staging fits the stage buffers");
+    __bf16 *ep = smem + w * 64 * EP;
     typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    // hi plane first; acc then keeps the residual v - hi for the lo plane
+#pragma unroll 1
+    for (int plane = 0; plane < 2; ++plane) {
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+        for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            bf4 h, l;
+            for (int y = 0; y < 4; ++y) {
+                bf4 h;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float v = acc[x][y][u];
-                h[u] = (__bf16)v;
-                l[u] = (__bf16)(v - (float)h[u]);
+                for (int u = 0; u < 4; ++u) {
+                    h[u] = (__bf16)acc[x][y][u];
+                    acc[x][y][u] -= (float)h[u];
+                }
+                *(bf4 *)(ep + (16 * x + orow) * EP + 16 * y + ocol) = h;
             }
-            const long long o = base + (long long)(16 * x + orow) * ldc + 16 * y + ocol;
-            *(bf4 *)(C + o) = h;
-            *(bf4 *)(C + o + lo) = l;
+        __bf16 *Cp = C + base + plane * lo;
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8) {
+            const int r = 8 * s8 + lane / 8, ch = lane % 8;
+            *(uint4 *)(Cp + (long long)r * ldc + 8 * ch) = *(const uint4 *)(ep + r * EP + 8 * ch);
         }
+    }
 }
 
 void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,

@@ -10,7 +10,7 @@ constexpr int TILE = 64;          // GEMM tile and padding granularity
 constexpr int MAX_NU = 4;         // continuum templates handled on the device
 constexpr int MAX_ZB = 64;        // redshift bins
 constexpr int MAX_NK = 64;        // k bands
-constexpr int VEC_SLOTS = 8 + 2 * MAX_NU;  // FP64 vectors of length Np per chunk
+constexpr int VEC_SLOTS = 8 + 4 * MAX_NU;  // FP64 vectors of length Np per chunk
 
 constexpr double SPEED_OF_LIGHT = 299792.458;
 constexpr double LYA_REST = 1215.67;
@@ -18,8 +18,12 @@ constexpr double LYA_REST = 1215.67;
 // Per-chunk input vectors, each Np long, at ChunkDev::in_off.
 enum InSlot { IN_WAVE = 0, IN_DELTA = 1, IN_IVAR = 2, IN_D = 3, IN_NPRIME = 4, IN_U0 = 5,
               IN_SLOTS = 5 + MAX_NU };
-// Per-chunk scratch vectors, each Np long, at ChunkDev::vec_off.
-enum VecSlot { V_X = 0, V_Y = 1, V_T0 = 2, V_T1 = 3, V_T2 = 4, V_W0 = 5 };
+// Per-chunk scratch vectors, each Np long, at ChunkDev::vec_off: x, y, the
+// W_t, then for the solves the right-hand sides R (D delta, D U_t), T = Li R,
+// Z = Li^T T, the residual E, F = Li E and G = Li^T F.
+enum VecSlot { V_X = 0, V_Y = 1, V_W0 = 2, V_R0 = 2 + MAX_NU, V_T0 = 3 + 2 * MAX_NU, V_Z0 = 4 + 3 * MAX_NU,
+               V_E = 5 + 4 * MAX_NU, V_F = 6 + 4 * MAX_NU, V_G = 7 + 4 * MAX_NU };
+static_assert(V_G < VEC_SLOTS, "vector slots");
 
 struct BinsDev {
     double zc[MAX_ZB];

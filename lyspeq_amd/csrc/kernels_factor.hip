@@ -1,0 +1,221 @@
+// Batched blocked Cholesky factorisation A = L L^T and triangular inverse
+// Li = L^-1 of every chunk of a batch (gfx950, FP64).
+//
+// Right-looking with 64x64 blocks (Np is a multiple of 64).  The L slot of
+// the matrix arena starts as a copy of A and is factorised in place.  Block
+// column p takes three launches over the whole batch:
+//   diag   : one workgroup per chunk factors the updated block (p,p) in LDS,
+//            inverts it (Li[p,p] = L[p,p]^-1) and zeroes block row p right of
+//            the diagonal in L and Li;
+//   panel  : TRSM tiles L[i,p] = A[i,p] Li[p,p]^T for i > p, and the tiles of
+//            block row p of the inverse, Li[p,j] = -Li[p,p] sum_{k=j}^{p-1}
+//            L[p,k] Li[k,j] for j < p (rows k < p of Li are final by then);
+//   update : trailing tiles A[i,j] -= L[i,p] L[j,p]^T for p < j <= i.
+// The 64x64 tile products run on v_mfma_f64_16x16x4f64, four waves per
+// workgroup, each owning a 32x32 quarter.  A non-positive pivot marks the
+// chunk failed (status) and is replaced by 1, as in the reference's LAPACK
+// error path the chunk's results are then discarded.
+#include "qml_common.hpp"
+#include "qml_kernels.hpp"
+
+namespace qml {
+
+typedef double fd4 __attribute__((ext_vector_type(4)));
+constexpr int FB = 64, FLD = FB + 1;   // block size, LDS row pitch (doubles)
+
+// acc += S_a S_b^T over k in [0, 64) for the wave's 32x32 quarter, with
+// S_a[r][k] = sa[r * FLD + k] and S_b[c][k] = sb[c * FLD + k].
+__device__ __forceinline__ void tile_nt(const double *sa, const double *sb, fd4 (&acc)[2][2]) {
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int wm = w / 2, wn = w % 2;
+#pragma unroll 4
+    for (int k = 0; k < FB; k += 4) {
+        double a[2], b[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            a[q] = sa[(wm * 32 + q * 16 + lane % 16) * FLD + k + lane / 16];
+            b[q] = sb[(wn * 32 + q * 16 + lane % 16) * FLD + k + lane / 16];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+}
+
+// Element (r, c) of the 64x64 tile held in acc by this lane: for u = 0..3,
+// r = 32 wm + 16 x + 4 u + lane/16, c = 32 wn + 16 y + lane%16.
+template <class F>
+__device__ __forceinline__ void tile_each(const fd4 (&acc)[2][2], F f) {
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int wm = w / 2, wn = w % 2;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                f(wm * 32 + x * 16 + 4 * u + lane / 16, wn * 32 + y * 16 + lane % 16, acc[x][y][u]);
+}
+
+__device__ __forceinline__ void tile_zero(fd4 (&acc)[2][2]) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = fd4{0.0, 0.0, 0.0, 0.0};
+}
+
+// s[r * FLD + c] = M[(r0 + r) * Np + c0 + c] (or transposed) for a 64x64 block
+__device__ __forceinline__ void load_block(double *s, const double *M, int Np, int r0, int c0, bool transpose) {
+    for (int e = threadIdx.x; e < FB * FB; e += 256) {
+        const int r = e / FB, c = e % FB;
+        const double v = M[(size_t)(r0 + r) * Np + c0 + c];
+        if (transpose) s[c * FLD + r] = v;
+        else s[r * FLD + c] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chol_init(const ChunkDev *cs, double *mat) {
+    const ChunkDev c = cs[blockIdx.y];
+    const size_t n = (size_t)c.Np * c.Np;
+    const double *A = mat + c.mat_off;
+    double *L = mat + c.mat_off + n;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) L[e] = A[e];
+}
+
+__global__ __launch_bounds__(256) void k_chol_diag(const ChunkDev *cs, double *mat, int *status, int p) {
+    const ChunkDev c = cs[blockIdx.x];
+    const int Np = c.Np, tid = threadIdx.x, p0 = p * FB;
+    if (p0 >= Np) return;
+    double *L = mat + c.mat_off + (size_t)Np * Np;
+    double *Li = mat + c.mat_off + 2 * (size_t)Np * Np;
+    __shared__ double a[FB * FLD];
+    __shared__ double v[FB * FLD];
+    __shared__ int fail;
+    if (tid == 0) fail = 0;
+    load_block(a, L, Np, p0, p0, false);
+    __syncthreads();
+    // unblocked right-looking factorisation of the 64x64 block
+    for (int k = 0; k < FB; ++k) {
+        if (tid == 0) {
+            const double d = a[k * FLD + k];
+            if (!(d > 0.0)) fail = 1;
+            a[k * FLD + k] = (d > 0.0) ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        if (tid > k && tid < FB) a[tid * FLD + k] /= a[k * FLD + k];
+        __syncthreads();
+        for (int e = tid; e < FB * FB; e += 256) {
+            const int i = e / FB, j = e % FB;
+            if (j > k && j <= i) a[i * FLD + j] -= a[i * FLD + k] * a[j * FLD + k];
+        }
+        __syncthreads();
+    }
+    // inverse by column: thread j solves L x = e_j for rows j..63
+    if (tid < FB) {
+        const int j = tid;
+        for (int i = 0; i < j; ++i) v[i * FLD + j] = 0.0;
+        for (int i = j; i < FB; ++i) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = j; k < i; ++k) s -= a[i * FLD + k] * v[k * FLD + j];
+            v[i * FLD + j] = s / a[i * FLD + i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < FB * FB; e += 256) {
+        const int i = e / FB, j = e % FB;
+        const size_t o = (size_t)(p0 + i) * Np + p0 + j;
+        L[o] = (j <= i) ? a[i * FLD + j] : 0.0;
+        Li[o] = v[i * FLD + j];
+    }
+    // block row p right of the diagonal is zero in L and Li
+    const int wr = Np - p0 - FB;
+    for (size_t e = tid; e < (size_t)FB * wr; e += 256) {
+        const size_t o = (size_t)(p0 + e / wr) * Np + p0 + FB + e % wr;
+        L[o] = 0.0;
+        Li[o] = 0.0;
+    }
+    if (tid == 0) {
+        if (p == 0) status[c.gidx] = fail;
+        else if (fail) status[c.gidx] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chol_panel(const ChunkDev *cs, double *mat, int p, int ntmax) {
+    const ChunkDev c = cs[blockIdx.y];
+    const int Np = c.Np, nt = Np / FB, p0 = p * FB, bx = blockIdx.x;
+    if (p >= nt) return;
+    double *L = mat + c.mat_off + (size_t)Np * Np;
+    double *Li = mat + c.mat_off + 2 * (size_t)Np * Np;
+    __shared__ double sa[FB * FLD];
+    __shared__ double sb[FB * FLD];
+    fd4 acc[2][2];
+    tile_zero(acc);
+    if (bx < ntmax) {
+        // TRSM tile: L[i,p] = A[i,p] Li[p,p]^T
+        const int i = p + 1 + bx;
+        if (i >= nt) return;
+        load_block(sa, L, Np, i * FB, p0, false);
+        load_block(sb, Li, Np, p0, p0, false);
+        __syncthreads();
+        tile_nt(sa, sb, acc);
+        tile_each(acc, [&](int r, int q, double x) { L[(size_t)(i * FB + r) * Np + p0 + q] = x; });
+        return;
+    }
+    // block row p of the inverse
+    const int j = bx - ntmax;
+    if (j >= p) return;
+    for (int k = j; k < p; ++k) {
+        load_block(sa, L, Np, p0, k * FB, false);       // L[p,k]
+        load_block(sb, Li, Np, k * FB, j * FB, true);   // Li[k,j]^T
+        __syncthreads();
+        tile_nt(sa, sb, acc);
+        __syncthreads();
+    }
+    // T = acc -> LDS (transposed, as the B operand), then Li[p,j] = -Li[p,p] T
+    tile_each(acc, [&](int r, int q, double x) { sb[q * FLD + r] = x; });
+    load_block(sa, Li, Np, p0, p0, false);
+    __syncthreads();
+    tile_zero(acc);
+    tile_nt(sa, sb, acc);
+    tile_each(acc, [&](int r, int q, double x) { Li[(size_t)(p0 + r) * Np + j * FB + q] = -x; });
+}
+
+__global__ __launch_bounds__(256) void k_chol_update(const ChunkDev *cs, double *mat, int p) {
+    const ChunkDev c = cs[blockIdx.y];
+    const int Np = c.Np, nt = Np / FB, p0 = p * FB;
+    const int m = nt - p - 1;                 // trailing block rows of this chunk
+    int t = blockIdx.x;
+    if (m <= 0 || t >= m * (m + 1) / 2) return;
+    // t -> (i, j), p < j <= i < nt, row-major over the lower triangle
+    int i = 0;
+    while (t > i) { t -= i + 1; ++i; }
+    const int bi = p + 1 + i, bj = p + 1 + t;
+    double *L = mat + c.mat_off + (size_t)Np * Np;
+    __shared__ double sa[FB * FLD];
+    __shared__ double sb[FB * FLD];
+    fd4 acc[2][2];
+    tile_zero(acc);
+    load_block(sa, L, Np, bi * FB, p0, false);
+    load_block(sb, L, Np, bj * FB, p0, false);
+    __syncthreads();
+    tile_nt(sa, sb, acc);
+    tile_each(acc, [&](int r, int q, double x) { L[(size_t)(bi * FB + r) * Np + bj * FB + q] -= x; });
+}
+
+void launch_cholesky(const LaunchCtx &L, int *status) {
+    const int ntmax = L.np_max / FB;
+    k_chol_init<<<dim3(64, L.nchunks), 256, 0, L.stream>>>(L.chunks, L.mat);
+    for (int p = 0; p < ntmax; ++p) {
+        k_chol_diag<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.mat, status, p);
+        const int gx = ntmax + p;   // TRSM tiles (< ntmax) and inverse tiles j < p
+        k_chol_panel<<<dim3(gx, L.nchunks), 256, 0, L.stream>>>(L.chunks, L.mat, p, ntmax);
+        const int m = ntmax - p - 1;
+        if (m > 0) k_chol_update<<<dim3(m * (m + 1) / 2, L.nchunks), 256, 0, L.stream>>>(L.chunks, L.mat, p);
+    }
+}
+
+// The inverse is formed inside the blocked factorisation.
+void launch_trinv(const LaunchCtx &) {}
+
+}  // namespace qml

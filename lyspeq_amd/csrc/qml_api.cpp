@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -119,6 +120,7 @@ struct qml_ctx {
     double scratch_gb = 32.0;
     int max_batch = 256;
     int pipeline = 1;
+    int poison = 0;   // LYSPEQ_QML_POISON=1 at commit: NaN-filled scratch and LDS
     hipStream_t st = nullptr, st2 = nullptr;
     bool bins_set = false, committed = false;
     std::vector<double> kedges, kcenters, zcenters;
@@ -581,6 +583,22 @@ int qml_commit(qml_ctx *c) {
     HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
     HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
     HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
+    // LYSPEQ_QML_POISON=1 fills every scratch arena with all-ones bytes (NaN
+    // in FP64, FP32 and bf16), so that a read of memory the iteration did
+    // not write shows up in the results.  The vector arena keeps its zeros.
+    // LDS is also filled with NaN bytes before each stage of every batch.
+    if (const char *pz = std::getenv("LYSPEQ_QML_POISON"); pz && pz[0] == '1') {
+        c->poison = 1;
+        for (int k = 0; k < 2; ++k) {
+            HIPCHK(hipMemset(c->d_mat[k], 0xff, std::max<size_t>(mx.mat, 1) * 8));
+            HIPCHK(hipMemset(c->d_pj[k], 0xff, std::max<size_t>(mx.pj, 1) * 8));
+            HIPCHK(hipMemset(c->d_sj[k], 0xff, std::max<size_t>(mx.sj, 1) * 8));
+            if (c->d_lic[k]) HIPCHK(hipMemset(c->d_lic[k], 0xff, std::max<size_t>(mx.lic, 1) * es));
+        }
+        HIPCHK(hipMemset(c->d_gram, 0xff, std::max<size_t>(mx.gram, 1) * 8));
+        HIPCHK(hipMemset(c->d_X, 0xff, std::max<size_t>(mx.stk, 1) * es));
+        HIPCHK(hipMemset(c->d_Y, 0xff, std::max<size_t>(mx.stk, 1) * es));
+    }
 
     for (size_t bi = 0; bi < groups.size(); ++bi) {
         Batch B;
@@ -655,6 +673,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.sj = c->d_sj[set];
         // stage A on st: this arena set was last read by stage B of batch bi - 2
         if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
+        if (c->poison) launch_poison_lds(c->st);
         HIPCHK(hipEventRecord(B.evA[0], c->st));
         launch_assemble(L, c->d_theta);
         HIPCHK(hipEventRecord(B.evA[1], c->st));
@@ -675,6 +694,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L2.stream = sB;
         const void *li = (c->prec == 0) ? (const void *)c->d_mat[set] : c->d_lic[set];
         HIPCHK(hipStreamWaitEvent(sB, B.evA[6], 0));
+        if (c->poison) launch_poison_lds(sB);
         HIPCHK(hipEventRecord(B.evB[0], sB));
         launch_xgen(L2, c->d_X, c->prec);
         HIPCHK(hipEventRecord(B.evB[1], sB));

@@ -23,6 +23,8 @@ struct LaunchCtx {
 
 // kernels_chunk.hip
 void launch_assemble(const LaunchCtx &L, const double *theta);
+// kernels_factor.hip: L = chol(A) and Li = L^-1 (both formed by
+// launch_cholesky; launch_trinv is kept for the timing slot and is empty)
 void launch_cholesky(const LaunchCtx &L, int *status);
 void launch_trinv(const LaunchCtx &L);
 void launch_solve(const LaunchCtx &L);
@@ -32,6 +34,9 @@ void launch_xgen(const LaunchCtx &L, void *X, int prec);
 void launch_convert_li(const LaunchCtx &L, void *lic, int prec);
 void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double *gram, const int *status,
                           double *fc, double *dbt);
+// Debug aid: fills the LDS of every CU with all-ones bytes (NaN in FP64,
+// FP32 and bf16), so that a kernel reading LDS it did not write shows it.
+void launch_poison_lds(hipStream_t st);
 void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, int nchunks, int T,
                        const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
 

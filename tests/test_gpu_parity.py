@@ -124,3 +124,29 @@ def test_mixed_precision_against_fp64(precision, tol):
     print(f"{precision}: F {ef:.2e} b {eb:.2e} t {et:.2e} d {P.rel_err(dm[0], d64[0]):.2e}")
     assert P.rel_err(dm[0], d64[0]) < 1e-12        # d_k stays FP64
     assert ef < tol and eb < tol and et < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_poisoned_scratch_gives_identical_results(precision, monkeypatch):
+    """Scratch arenas and LDS filled with NaN bytes (LYSPEQ_QML_POISON=1)
+    must not change a single bit of the results: every value the iteration
+    reads is one it wrote (chunks of the sample have all-padding tiles, whose
+    LDS slices are never staged)."""
+    _print_case(f"poisoned scratch {precision}")
+    w = synth.desi_config(nchunks=24)
+    T = w.binning.total_kz_bins
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 5)
+    idx = list(range(len(w.chunks)))
+    eng0, out0, F0, d0 = P.run_engine(w, th, idx, precision=precision)
+    eng0.close()
+    monkeypatch.setenv("LYSPEQ_QML_POISON", "1")
+    eng1, out1, F1, d1 = P.run_engine(w, th, idx, precision=precision)
+    F2, d2 = np.zeros((T, T)), np.zeros((3, T))
+    eng1.set_pipeline(False)
+    eng1.iterate(th, F2, d2)
+    eng1.close()
+    assert np.isfinite(F1).all() and np.isfinite(d1).all()
+    assert np.array_equal(F0, F1) and np.array_equal(d0, d1)
+    assert np.array_equal(F0, F2) and np.array_equal(d0, d2)
+    for i in out0:
+        assert np.array_equal(out0[i][0], out1[i][0]) and np.array_equal(out0[i][1], out1[i][1])

@@ -173,6 +173,12 @@ int qml_get_chunk_result(qml_ctx *ctx, int chunk_id, double *fisher, double *dbt
 
 int qml_get_timings(qml_ctx *ctx, qml_timings *t);
 
+/* Leaves chunk chunk_id out of (active = 0) or back in (1) the sums that
+ * qml_iterate accumulates; its own results are still computed.  The
+ * counterpart of the reference dropping a Chunk object from
+ * OneQSOEstimate::chunks (core/one_qso_estimate.cpp:176-197). */
+int qml_set_chunk_active(qml_ctx *ctx, int chunk_id, int active);
+
 /* Stage overlap of qml_iterate (default on): the factorisation and solves of
  * batch b+1 run on one HIP stream while the dense products of batch b run on
  * a second.  Off serialises both stages on one stream, so that the per-kernel

@@ -50,7 +50,38 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_LIB = os.path.join(HERE, "libqmlhost.so")
+HOST_DRIVER = os.path.join(ROOT, "tests", "cpp", "host_driver")
+
+
+def _run(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{r.stderr}")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer (Chunk / OneQSOEstimate / Engine over the C ABI)
+    as libqmlhost.so, and the C++ test driver that exercises it."""
+    rocm_lib = "/opt/rocm/lib"
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    link = ["-L", HERE, "-lqmlhip", "-L", rocm_lib, f"-Wl,-rpath-link,{rocm_lib}"]
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
+          os.path.join(HERE, "host", "qml_host.cpp"), *link, "-Wl,-rpath,$ORIGIN", "-o", tmp],
+         "host library build", verbose)
+    os.replace(tmp, HOST_LIB)
+    src = os.path.join(ROOT, "tests", "cpp", "host_driver.cpp")
+    if os.path.exists(src):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", *inc, src, "-L", HERE, "-lqmlhost", *link,
+              "-Wl,-rpath,$ORIGIN/../../lyspeq_amd", "-o", HOST_DRIVER], "host driver build", verbose)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

@@ -763,7 +763,8 @@ __global__ __launch_bounds__(256) void k_gram_finalize(const ChunkDev *cs, const
 // --------------------------------------------------------------- accumulate --
 // fisher_sum[(T+1) fis + j + i T] += F_chunk[j + i nq] for j >= i, and the
 // d/b/t vectors, in chunk-id order (core/chunk_estimate.cpp:893-903).
-__global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const int *status, int nchunks,
+__global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const int *status, const int *skip,
+                                                    int nchunks,
                                                     int T, const double *fc, const double *dbt,
                                                     double *Fsum, double *dbt_sum) {
     // each thread owns fixed columns of row a, so the sums run in chunk order
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
         double s = Fsum[(size_t)a * T + b];
         for (int ci = 0; ci < nchunks; ++ci) {
             const ChunkOut o = outs[ci];
-            if (status[ci] || a < o.fis || b >= o.fis + o.nq) continue;
+            if (status[ci] || skip[ci] || a < o.fis || b >= o.fis + o.nq) continue;
             s += fc[o.fc_off + (size_t)(a - o.fis) * o.nq + (b - o.fis)];
         }
         Fsum[(size_t)a * T + b] = s;
@@ -782,7 +783,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
         double s = dbt_sum[k * T + a];
         for (int ci = 0; ci < nchunks; ++ci) {
             const ChunkOut o = outs[ci];
-            if (status[ci] || a < o.fis || a >= o.fis + o.nq) continue;
+            if (status[ci] || skip[ci] || a < o.fis || a >= o.fis + o.nq) continue;
             s += dbt[o.dbt_off + k * o.nq + (a - o.fis)];
         }
         dbt_sum[k * T + a] = s;
@@ -848,9 +849,9 @@ void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double
                           double *fc, double *dbt) {
     k_gram_finalize<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, outs, gram, L.sj, status, L.bins.nk, fc, dbt);
 }
-void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, int nchunks, int T,
-                       const double *fc, const double *dbt, double *Fsum, double *dbt_sum) {
-    k_accumulate<<<T, 256, 0, st>>>(outs, status, nchunks, T, fc, dbt, Fsum, dbt_sum);
+void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, const int *skip, int nchunks,
+                       int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum) {
+    k_accumulate<<<T, 256, 0, st>>>(outs, status, skip, nchunks, T, fc, dbt, Fsum, dbt_sum);
 }
 
 }  // namespace qml

@@ -136,6 +136,7 @@ struct qml_ctx {
     ChunkOut *d_outs = nullptr;
     double *d_fc = nullptr, *d_dbt = nullptr;
     int *d_status = nullptr;
+    int *d_skip = nullptr;            // chunks left out of the accumulation (qml_set_chunk_active)
     // stage-A arenas are double-buffered so that batch b+1 is factorised
     // while batch b runs its dense products
     double *d_mat[2] = {}, *d_vec[2] = {}, *d_pj[2] = {}, *d_sj[2] = {};
@@ -196,7 +197,7 @@ int qml_destroy(qml_ctx *c) {
         for (auto &e : b.evA) if (e) hipEventDestroy(e);
         for (auto &e : b.evB) if (e) hipEventDestroy(e);
     }
-    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_gram,
+    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip, c->d_gram,
                     c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < 2; ++k) {
@@ -475,6 +476,8 @@ int qml_commit(qml_ctx *c) {
     HIPCHK(hipMalloc(&c->d_dbt, std::max<size_t>(dbt_total, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&c->d_status, std::max(nch, 1) * sizeof(int)));
     HIPCHK(hipMemset(c->d_status, 0, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_skip, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMemset(c->d_skip, 0, std::max(nch, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&c->d_theta, std::max(c->T, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&c->d_F, (size_t)c->T * c->T * sizeof(double)));
     HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
@@ -709,7 +712,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(c->ev_acc, sB));
-    launch_accumulate(sB, c->d_outs, c->d_status, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
+    launch_accumulate(sB, c->d_outs, c->d_status, c->d_skip, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
                       c->d_dbtsum);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, sB));
@@ -773,6 +776,17 @@ int qml_get_timings(qml_ctx *c, qml_timings *t) {
 int qml_set_pipeline(qml_ctx *c, int on) {
     if (!c) return fail(QML_ERR_ARG, "qml_set_pipeline: null context");
     c->pipeline = on ? 1 : 0;
+    return QML_OK;
+}
+
+int qml_set_chunk_active(qml_ctx *c, int id, int active) {
+    if (!c || !c->committed || id < 0 || id >= (int)c->chunks.size())
+        return fail(QML_ERR_ARG, "qml_set_chunk_active: bad chunk id or not committed");
+    HIPCHK(hipSetDevice(c->device));
+    const int skip = active ? 0 : 1;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    HIPCHK(hipMemcpy(c->d_skip + id, &skip, sizeof(int), hipMemcpyHostToDevice));
     return QML_OK;
 }
 

@@ -37,8 +37,9 @@ void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double
 // Debug aid: fills the LDS of every CU with all-ones bytes (NaN in FP64,
 // FP32 and bf16), so that a kernel reading LDS it did not write shows it.
 void launch_poison_lds(hipStream_t st);
-void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, int nchunks, int T,
-                       const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
+// Adds the results of chunks with status 0 and skip 0, in chunk-id order.
+void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, const int *skip, int nchunks,
+                       int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
 
 // kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3.
 // Y_j = Li X_j for every member (lower-triangular Li).

@@ -1,0 +1,209 @@
+// C++ host layer over the QML engine's C ABI; see qml_host.hpp.
+#include "qml_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+namespace lyspeq_amd {
+
+namespace {
+constexpr double SPEED_OF_LIGHT = 299792.458;
+}
+
+// ------------------------------------------------------------------ Engine --
+Engine::Engine(const Binning &bins, int precision, int device, double scratch_gb, int max_batch_chunks)
+    : bins_(bins) {
+    if (bins_.nk() < 1 || bins_.nz() < 1) throw QMLError(QML_ERR_ARG, "Engine: empty binning");
+    qml_opts o{device, precision, scratch_gb, max_batch_chunks};
+    check(qml_create(&ctx_, &o));
+    qml_bins b{};
+    b.nk = bins_.nk();
+    b.nz = bins_.nz();
+    b.kband_edges = bins_.kband_edges.data();
+    b.zbin_centers = bins_.zbin_centers.data();
+    b.z_bin_width = bins_.z_bin_width;
+    b.z_binning_method = bins_.z_binning_method;
+    b.cont_logl_order = bins_.cont_logl_order;
+    b.cont_lam_order = bins_.cont_lam_order;
+    b.turn_off_sfid = bins_.turn_off_sfid ? 1 : 0;
+    b.max_forest_length_v = bins_.maxForestLengthPerChunk();
+    b.max_pixel_length_v = bins_.max_pixel_length_v;
+    const int rc = qml_set_bins(ctx_, &b);
+    if (rc != QML_OK) {
+        const std::string msg = qml_last_error();
+        qml_destroy(ctx_);
+        ctx_ = nullptr;
+        throw QMLError(rc, msg);
+    }
+}
+
+Engine::~Engine() {
+    for (Chunk *c : chunks_)
+        if (c) c->eng_ = nullptr;
+    if (ctx_) qml_destroy(ctx_);
+}
+
+void Engine::check(int code) const {
+    if (code != QML_OK) throw QMLError(code, qml_last_error());
+}
+
+int Engine::addTable(int R_fwhm, double dv_kms, int nv, double v_length, int nz, const double *deriv,
+                     const double *signal) {
+    int r = -1;
+    check(qml_add_table(ctx_, R_fwhm, dv_kms, nv, v_length, nz, deriv, signal, &r));
+    return r;
+}
+
+int Engine::addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info) {
+    if (committed_) throw QMLError(QML_ERR_ARG, "Chunk: the engine has already run; construct chunks first");
+    if (i1 < 0 || i2 > q.size() || i2 <= i1) throw std::out_of_range("Chunk: pixel range outside the spectrum");
+    qml_chunk_in in{q.wave.data() + i1, q.delta.data() + i1, q.ivar.data() + i1, i2 - i1, q.R_fwhm};
+    int id = -1;
+    const int rc = qml_add_chunk(ctx_, &in, info, &id);
+    if (rc != QML_OK) {
+        // the reference constructor throws; OneQSOEstimate skips the chunk
+        throw std::runtime_error(std::string("Chunk::Chunk::") + qml_last_error());
+    }
+    return id;
+}
+
+void Engine::enqueue(Chunk *c, const double *ps, std::vector<std::unique_ptr<double[]>> *dbt, double *fisher) {
+    if (!ps || !dbt || !fisher || dbt->size() < 3)
+        throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: null estimate or sums");
+    if (queue_.empty()) {
+        ps_ = ps;
+        dbt_ = dbt;
+        fisher_ = fisher;
+    } else if (ps != ps_ || dbt != dbt_ || fisher != fisher_) {
+        throw QMLError(QML_ERR_ARG,
+                       "Chunk::oneQSOiteration: all chunks of one flush share the estimate and the sums");
+    }
+    if (std::find(queue_.begin(), queue_.end(), c) == queue_.end()) queue_.push_back(c);
+}
+
+void Engine::forget(Chunk *c) {
+    if (c->id_ >= 0 && c->id_ < (int)chunks_.size()) chunks_[c->id_] = nullptr;
+    queue_.erase(std::remove(queue_.begin(), queue_.end(), c), queue_.end());
+}
+
+void Engine::flush() {
+    if (queue_.empty()) return;
+    if (!committed_) {
+        check(qml_commit(ctx_));
+        committed_ = true;
+        active_.assign(chunks_.size(), 1);
+    }
+    // only enqueued chunks enter the sums
+    std::vector<int> want(chunks_.size(), 0);
+    for (Chunk *c : queue_) want[c->id_] = 1;
+    for (size_t id = 0; id < chunks_.size(); ++id)
+        if (want[id] != active_[id]) {
+            check(qml_set_chunk_active(ctx_, (int)id, want[id]));
+            active_[id] = want[id];
+        }
+    const int T = bins_.totalKzBins();
+    std::vector<double> dbt((size_t)3 * T, 0.0);
+    check(qml_iterate(ctx_, ps_, fisher_, dbt.data()));
+    for (int k = 0; k < 3; ++k) {
+        double *dst = (*dbt_)[k].get();
+        for (int i = 0; i < T; ++i) dst[i] += dbt[(size_t)k * T + i];
+    }
+    std::vector<double> flat;
+    for (Chunk *c : queue_) {
+        const int nq = c->N_Q_MATRICES;
+        flat.assign((size_t)3 * nq, 0.0);
+        int st = 0;
+        check(qml_get_chunk_result(ctx_, c->id_, c->fisher_matrix.get(), flat.data(), &st));
+        for (int k = 0; k < 3; ++k)
+            std::copy(flat.begin() + (size_t)k * nq, flat.begin() + (size_t)(k + 1) * nq,
+                      c->dbt_estimate_before_fisher_vector[k].get());
+        c->status_ = st;
+        if (st != QML_CHUNK_OK)
+            std::fprintf(stderr, "Chunk::oneQSOiteration: covariance of chunk %d not positive definite; "
+                                 "skipping its accumulation.\n", c->id_);
+    }
+    queue_.clear();
+    ps_ = nullptr;
+    dbt_ = nullptr;
+    fisher_ = nullptr;
+}
+
+void Engine::timings(qml_timings *t) { check(qml_get_timings(ctx_, t)); }
+
+// ------------------------------------------------------------------- Chunk --
+Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2) : eng_(&eng) {
+    id_ = eng.addChunk(q, i1, i2, &info_);
+    if ((int)eng.chunks_.size() <= id_) eng.chunks_.resize(id_ + 1, nullptr);
+    eng.chunks_[id_] = this;
+    fisher_index_start = info_.fisher_index_start;
+    N_Q_MATRICES = info_.n_q_matrices;
+    ZBIN = info_.zbin;
+    ZBIN_LOW = info_.zbin_low;
+    ZBIN_UPP = info_.zbin_upp;
+    KBIN_UPP = info_.kbin_upp;
+    for (int k = 0; k < 3; ++k) {
+        dbt_estimate_before_fisher_vector.push_back(std::make_unique<double[]>(N_Q_MATRICES));
+        std::fill_n(dbt_estimate_before_fisher_vector.back().get(), N_Q_MATRICES, 0.0);
+    }
+    fisher_matrix = std::make_unique<double[]>((size_t)N_Q_MATRICES * N_Q_MATRICES);
+    std::fill_n(fisher_matrix.get(), (size_t)N_Q_MATRICES * N_Q_MATRICES, 0.0);
+}
+
+Chunk::~Chunk() {
+    if (eng_) eng_->forget(this);
+}
+
+void Chunk::oneQSOiteration(const double *ps_estimate, std::vector<std::unique_ptr<double[]>> &dbt_sum_vector,
+                            double *fisher_sum) {
+    if (!eng_) throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: the engine is gone");
+    eng_->enqueue(this, ps_estimate, &dbt_sum_vector, fisher_sum);
+}
+
+// ---------------------------------------------------------- OneQSOEstimate --
+std::vector<int> OneQSOEstimate::decideIndices(int size, const double *wave, const Binning &bins) {
+    std::vector<int> indices{0};
+    if (size <= 0) return indices;
+    std::vector<double> vel(size);
+    for (int i = 0; i < size; ++i) vel[i] = SPEED_OF_LIGHT * std::log(wave[i] / wave[0]);
+    double vmax = vel[size - 1];
+    int nchunks = 1;
+    const double mfl = bins.maxForestLengthPerChunk();
+    if (bins.number_of_chunks > 1) {
+        nchunks = (int)(nchunks + vmax / mfl);   // int += double truncates the sum
+        vmax = std::min(mfl, vmax / nchunks);
+    }
+    int lo = 0;
+    for (int i = 0; i < nchunks; ++i) {
+        const int jj = (int)(std::upper_bound(vel.begin() + lo, vel.end(), vmax) - vel.begin());
+        indices.push_back(jj);
+        if (jj == size) break;
+        lo = jj;
+        const double v0 = vel[lo];
+        for (int k = lo; k < size; ++k) vel[k] -= v0;
+    }
+    return indices;
+}
+
+OneQSOEstimate::OneQSOEstimate(Engine &eng, const QSOSpectrum &q) {
+    const std::vector<int> idx = decideIndices(q.size(), q.wave.data(), eng.binning());
+    const int n = (int)idx.size() - 1;
+    chunks.reserve(n);
+    for (int nc = 0; nc < n; ++nc) {
+        try {
+            chunks.push_back(std::make_unique<Chunk>(eng, q, idx[nc], idx[nc + 1]));
+        } catch (QMLError &) {
+            throw;   // engine failures are not chunk rejections
+        } catch (std::exception &e) {
+            std::fprintf(stderr, "OneQSOEstimate::OneQSOEstimate::%s Skipping chunk %d/%d of %s.\n", e.what(),
+                         nc + 1, n, q.fname.c_str());
+        }
+    }
+}
+
+void OneQSOEstimate::oneQSOiteration(const double *ps_estimate,
+                                     std::vector<std::unique_ptr<double[]>> &dbt_sum_vector, double *fisher_sum) {
+    for (auto &c : chunks) c->oneQSOiteration(ps_estimate, dbt_sum_vector, fisher_sum);
+}
+
+}  // namespace lyspeq_amd

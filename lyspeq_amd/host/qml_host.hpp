@@ -1,0 +1,167 @@
+// C++ host layer over the QML engine's C ABI (include/lyspeq_qml.h).
+//
+// Mirrors the surface the reference's driver uses for the per-chunk QML
+// step, so that a caller written against core/chunk_estimate.hpp and
+// core/one_qso_estimate.hpp keeps its shape:
+//
+//   Chunk            core/chunk_estimate.hpp:46-119  (constructor bookkeeping,
+//                    fisher_index_start, N_Q_MATRICES, ZBIN*, KBIN_UPP,
+//                    dbt_estimate_before_fisher_vector, fisher_matrix,
+//                    size(), realSize(), oneQSOiteration)
+//   OneQSOEstimate   core/one_qso_estimate.hpp:16-61  (decideIndices, chunks,
+//                    oneQSOiteration)
+//
+// The difference is when the work happens: Chunk::oneQSOiteration enqueues
+// the chunk, and Engine::flush(), called once after the driver's chunk loop
+// (before the 0.5 scaling of the Fisher sum, core/quadratic_estimate.cpp:456),
+// runs the whole batch on the GPU and adds every chunk's F and d, b, t into
+// the sums the chunks were enqueued with.  After flush() each Chunk's public
+// result arrays hold its own F and d, b, t, as after the reference call.
+//
+// Errors: constructor failures throw std::runtime_error (as the reference
+// constructor does, so OneQSOEstimate skips the chunk); engine failures throw
+// QMLError, which carries the C-ABI status code.  A chunk whose covariance
+// is not positive definite keeps zero results and is left out of the sums,
+// as the reference's caught LAPACK failure (core/chunk_estimate.cpp:905-913).
+#pragma once
+
+#include "lyspeq_qml.h"
+
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lyspeq_amd {
+
+class QMLError : public std::runtime_error {
+public:
+    QMLError(int code, const std::string &what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+// The part of qio::QSOFile (io/qso_file.hpp) the chunk step reads, after
+// the reader's preprocessing (flux to delta, boundary cut): wavelengths in
+// Angstrom, delta, inverse variance, the resolution R_fwhm and the pixel
+// width dv in km/s (table selection rounds it to 0.1 km/s).
+struct QSOSpectrum {
+    std::vector<double> wave, delta, ivar;
+    int R_fwhm = 0;
+    std::string fname;
+    int size() const { return (int)wave.size(); }
+};
+
+// Binning and the specifics the chunk step depends on (bins:: and
+// specifics:: globals of core/global_numbers.cpp).
+struct Binning {
+    std::vector<double> kband_edges;     // NUMBER_OF_K_BANDS + 1
+    std::vector<double> zbin_centers;    // NUMBER_OF_Z_BINS
+    double z_bin_width = 0.2;
+    int z_binning_method = 1;            // 1 triangular, 0 top-hat
+    int cont_logl_order = 1, cont_lam_order = -1;
+    bool turn_off_sfid = false;
+    double max_forest_length_v = 3.5e4;  // MaximumForestLength (before division)
+    int number_of_chunks = 1;            // DynamicChunkNumber
+    double max_pixel_length_v = 120.0;   // MaximumMeanPixelSize
+    int nk() const { return (int)kband_edges.size() - 1; }
+    int nz() const { return (int)zbin_centers.size(); }
+    int totalKzBins() const { return nk() * nz(); }
+    double maxForestLengthPerChunk() const { return max_forest_length_v / std::max(1, number_of_chunks); }
+};
+
+class Chunk;
+
+// One device context: tables, chunks, and the batched iteration.
+class Engine {
+public:
+    Engine(const Binning &bins, int precision = QML_PREC_FP64, int device = 0, double scratch_gb = 0.0,
+           int max_batch_chunks = 0);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // readSQforR counterpart: derivative table nk x nv (v contiguous per
+    // band), signal table nv x nz (z contiguous), both on the reference grid.
+    // Returns RES_INDEX.
+    int addTable(int R_fwhm, double dv_kms, int nv, double v_length, int nz, const double *deriv,
+                 const double *signal);
+
+    // Runs the batch for every chunk enqueued since the last flush and adds
+    // its F and d, b, t into the sums they were enqueued with.  The first
+    // call commits the engine (no chunk can be added after it).
+    void flush();
+
+    int totalKzBins() const { return bins_.totalKzBins(); }
+    const Binning &binning() const { return bins_; }
+    qml_ctx *ctx() { return ctx_; }
+    void timings(qml_timings *t);
+
+private:
+    friend class Chunk;
+    int addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info);
+    void enqueue(Chunk *c, const double *ps, std::vector<std::unique_ptr<double[]>> *dbt, double *fisher);
+    void forget(Chunk *c);
+    void check(int code) const;
+
+    Binning bins_;
+    qml_ctx *ctx_ = nullptr;
+    bool committed_ = false;
+    std::vector<Chunk *> chunks_;          // by engine chunk id; null once destroyed
+    std::vector<int> active_;              // accumulation flag last sent to the device
+    std::vector<Chunk *> queue_;
+    const double *ps_ = nullptr;
+    std::vector<std::unique_ptr<double[]>> *dbt_ = nullptr;
+    double *fisher_ = nullptr;
+};
+
+class Chunk {
+public:
+    // Chunk(const qio::QSOFile&, int i1, int i2): pixels [i1, i2) of q.
+    Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2);
+    ~Chunk();
+    Chunk(const Chunk &) = delete;
+    Chunk(Chunk &&) = delete;
+
+    int fisher_index_start, N_Q_MATRICES;
+    int ZBIN, ZBIN_LOW, ZBIN_UPP, KBIN_UPP;
+    // 3 x N_Q_MATRICES: d, b, t; N_Q_MATRICES^2 Fisher (upper, unhalved)
+    std::vector<std::unique_ptr<double[]>> dbt_estimate_before_fisher_vector;
+    std::unique_ptr<double[]> fisher_matrix;
+
+    int size() const { return info_.size; }
+    int realSize() const { return info_.real_size; }
+    const qml_chunk_info &info() const { return info_; }
+    int engineId() const { return id_; }
+    // QML_OK, or nonzero if the last flush found the covariance not positive definite
+    int lastStatus() const { return status_; }
+
+    // Enqueues this chunk for the next Engine::flush(): ps_estimate is the
+    // full TOTAL_KZ_BINS vector, the sums are TOTAL_KZ_BINS^2 (upper
+    // triangle) and 3 x TOTAL_KZ_BINS.
+    void oneQSOiteration(const double *ps_estimate, std::vector<std::unique_ptr<double[]>> &dbt_sum_vector,
+                         double *fisher_sum);
+
+private:
+    friend class Engine;
+    Engine *eng_;
+    int id_ = -1;
+    int status_ = 0;
+    qml_chunk_info info_{};
+};
+
+class OneQSOEstimate {
+public:
+    // Splits q by decideIndices and constructs a Chunk for each piece; a
+    // piece whose constructor throws is skipped (core/one_qso_estimate.cpp:97-126).
+    OneQSOEstimate(Engine &eng, const QSOSpectrum &q);
+
+    static std::vector<int> decideIndices(int size, const double *wave, const Binning &bins);
+
+    std::vector<std::unique_ptr<Chunk>> chunks;
+
+    void oneQSOiteration(const double *ps_estimate, std::vector<std::unique_ptr<double[]>> &dbt_sum_vector,
+                         double *fisher_sum);
+};
+
+}  // namespace lyspeq_amd

@@ -98,7 +98,8 @@ _lib = None
 
 EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
            "qml_add_chunk", "qml_commit", "qml_iterate", "qml_get_chunk_result",
-           "qml_get_timings", "qml_set_pipeline", "qml_num_chunks", "qml_total_kz_bins"]
+           "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
+           "qml_total_kz_bins"]
 
 
 def load_library(path: str = LIB_PATH):
@@ -123,6 +124,7 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_chunk_result.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_int)]
     lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
     lib.qml_set_pipeline.argtypes = [vp, C.c_int]
+    lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]
     lib.qml_total_kz_bins.argtypes = [vp]
     _lib = lib
@@ -281,6 +283,10 @@ class QMLEngine:
         """Stage overlap across two streams (default on); off gives per-kernel
         durations free of contention, with identical results."""
         _check(self._lib.qml_set_pipeline(self._ctx, int(bool(on))), self._lib)
+
+    def set_chunk_active(self, cid: int, active: bool):
+        """Leaves a chunk out of (or back in) the accumulated sums."""
+        _check(self._lib.qml_set_chunk_active(self._ctx, int(cid), int(bool(active))), self._lib)
 
     def timings(self) -> dict:
         t = Timings()

@@ -1,0 +1,180 @@
+"""The C++ host layer (lyspeq_amd/host/qml_host.hpp: Chunk, OneQSOEstimate,
+Engine::flush) driven the way the reference's driver drives Chunk and
+OneQSOEstimate (core/quadratic_estimate.cpp:437-457), through the compiled
+test driver tests/cpp/host_driver.  Needs an MI355X.
+
+The C++ path must reproduce the Python binding's results bit for bit (same
+library, same chunks) and the oracle to the FP64 bar, including spectra
+that OneQSOEstimate::decideIndices splits into several chunks."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import parity as P
+from lyspeq_amd import synth
+from lyspeq_amd.build import HOST_DRIVER
+from lyspeq_amd.qml import PRECISIONS
+from oracle.chunk import Chunk as OracleChunk, ChunkError, decide_indices
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _write_workload(path, w, theta, precision, iterations=1):
+    b = w.binning
+    edges, zc = b.kband_edges(), b.zbin_centers()
+    nk, nz = len(edges) - 1, len(zc)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iii", 0x574C4D51, nk, nz))
+        f.write(np.asarray(edges, "<f8").tobytes())
+        f.write(np.asarray(zc, "<f8").tobytes())
+        f.write(struct.pack("<d", b.z_bin_width))
+        f.write(struct.pack("<iiii", b.method, b.cont_logl_order, b.cont_lam_order, int(bool(b.turn_off_sfid))))
+        f.write(struct.pack("<did", b.max_forest_length_v, b.number_of_chunks, b.max_pixel_length_v))
+        f.write(struct.pack("<ii", PRECISIONS[precision], iterations))
+        f.write(struct.pack("<i", len(w.pairs)))
+        for i, (R, dv) in enumerate(w.pairs):
+            deriv = np.ascontiguousarray(w.deriv[i], "<f8")
+            sig = np.ascontiguousarray(w.signal[i], "<f8")
+            f.write(struct.pack("<idid", int(R), float(dv), deriv.shape[1], float(w.v_length)))
+            f.write(struct.pack("<i", sig.shape[1]))
+            f.write(deriv.tobytes())
+            f.write(sig.tobytes())
+        f.write(struct.pack("<i", len(w.chunks)))
+        for c in w.chunks:
+            n = len(c["wave"])
+            f.write(struct.pack("<ii", int(c["R"]), n))
+            for key in ("wave", "delta", "ivar"):
+                f.write(np.ascontiguousarray(c[key], "<f8").tobytes())
+        f.write(np.ascontiguousarray(theta, "<f8").tobytes())
+
+
+def _read_output(path):
+    with open(path, "rb") as f:
+        buf = f.read()
+    T = struct.unpack_from("<i", buf, 0)[0]
+    o = 4
+    F = np.frombuffer(buf, "<f8", T * T, o).reshape(T, T)
+    o += 8 * T * T
+    dbt = np.frombuffer(buf, "<f8", 3 * T, o).reshape(3, T)
+    o += 8 * 3 * T
+    nch = struct.unpack_from("<i", buf, o)[0]
+    o += 4
+    chunks = []
+    for _ in range(nch):
+        s, fis, nq, st = struct.unpack_from("<iiii", buf, o)
+        o += 16
+        fc = np.frombuffer(buf, "<f8", nq * nq, o).reshape(nq, nq)
+        o += 8 * nq * nq
+        d = np.frombuffer(buf, "<f8", 3 * nq, o).reshape(3, nq)
+        o += 8 * 3 * nq
+        chunks.append((s, fis, nq, st, fc, d))
+    return F, dbt, chunks
+
+
+def _run_driver(tmp_path, w, theta, precision, iterations=1):
+    if not os.path.exists(HOST_DRIVER):
+        pytest.fail(f"{HOST_DRIVER} is missing: build it with `python -m lyspeq_amd.build`")
+    wl, out = tmp_path / "workload.bin", tmp_path / "out.bin"
+    _write_workload(wl, w, theta, precision, iterations)
+    r = subprocess.run([HOST_DRIVER, str(wl), str(out)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    return _read_output(out)
+
+
+def _pieces(w):
+    """Chunk pieces (spectrum index, i1, i2) by the oracle's decideIndices."""
+    b = P.oracle_bins(w.binning)
+    out = []
+    for s, c in enumerate(w.chunks):
+        idx = decide_indices(np.asarray(c["wave"]), b)
+        out += [(s, idx[k], idx[k + 1]) for k in range(len(idx) - 1)]
+    return out
+
+
+def _python_path(w, theta, precision):
+    """The same chunks registered through the Python binding."""
+    from lyspeq_amd.qml import QMLEngine, QMLError
+    eng = QMLEngine(w.binning, precision=precision)
+    for i, (R, dv) in enumerate(w.pairs):
+        eng.add_table(R, dv, w.deriv[i], w.signal[i], w.v_length)
+    ids = []
+    for s, i1, i2 in _pieces(w):
+        c = w.chunks[s]
+        try:
+            cid, _ = eng.add_chunk(c["wave"][i1:i2], c["delta"][i1:i2], c["ivar"][i1:i2], c["R"])
+            ids.append(cid)
+        except QMLError as e:
+            if e.code not in (-3, -4, -5):
+                raise
+    eng.commit()
+    T = eng.T
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(theta, F, d)
+    res = [eng.chunk_result(cid) for cid in ids]
+    eng.close()
+    return F, d, res
+
+
+def _oracle(w, theta):
+    b = P.oracle_bins(w.binning)
+    tabs = P.oracle_tables(w)
+    T = b.total_kz_bins
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    n = 0
+    for s, i1, i2 in _pieces(w):
+        c = w.chunks[s]
+        try:
+            ch = OracleChunk(b, tabs, c["wave"][i1:i2], c["delta"][i1:i2], c["ivar"][i1:i2], c["R"])
+        except ChunkError:
+            continue
+        ch.one_iteration(theta, F, d)
+        n += 1
+    return F, d, n
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_cpp_host_layer_matches_python_binding_and_oracle(tmp_path, precision):
+    print(f"[case] C++ host layer, desi sample, {precision}", flush=True)
+    w = synth.desi_config(nchunks=16)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 3)
+    F, dbt, chunks = _run_driver(tmp_path, w, th, precision, iterations=2)
+    Fp, dp, res = _python_path(w, th, precision)
+    assert len(chunks) == len(res) >= 12
+    assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
+    for (s, fis, nq, st, fc, d), (fr, dr, sr) in zip(chunks, res):
+        assert st == sr == 0
+        assert np.array_equal(fc, fr) and np.array_equal(d, dr)
+    if precision == "fp64":
+        Fo, do, n = _oracle(w, th)
+        assert n == len(chunks)
+        assert P.rel_err(F, Fo) < FP64_TOL
+        for k in range(3):
+            assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+
+
+def test_cpp_one_qso_estimate_splits_spectra_like_decide_indices(tmp_path):
+    print("[case] C++ OneQSOEstimate with DynamicChunkNumber 2", flush=True)
+    w = synth.cpu_reference_config(nchunks=6)
+    # 256 pixels x 60 km/s = 15300 km/s per spectrum; two chunks of ~128 pixels
+    w.binning.number_of_chunks = 2
+    w.binning.max_forest_length_v = 16000.0
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 9)
+    F, dbt, chunks = _run_driver(tmp_path, w, th, "fp64")
+    pieces = _pieces(w)
+    assert len(pieces) == 2 * len(w.chunks)
+    Fp, dp, res = _python_path(w, th, "fp64")
+    assert len(chunks) == len(res) == len(pieces)
+    assert [c[0] for c in chunks] == [p[0] for p in pieces]
+    assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
+    Fo, do, n = _oracle(w, th)
+    assert n == len(chunks)
+    assert P.rel_err(F, Fo) < FP64_TOL
+    for k in range(3):
+        assert P.rel_err(dbt[k], do[k]) < FP64_TOL

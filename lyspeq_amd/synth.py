@@ -258,18 +258,20 @@ def desi_config(seed: int = 1, nchunks: int = 4096, with_tables=True) -> Workloa
     return _finish_tables(w, with_tables)
 
 
-def hires_config(seed: int = 2, nchunks: int = 512, with_tables=True) -> Workload:
+def hires_config(seed: int = 2, nchunks: int = 512, with_tables=True, n_range=(800, 2000)) -> Workload:
     """High-resolution batch: N ~ U[800,2000], dv ~ U[2.5,6] km/s (uniform in
     ln lambda), R ~ U[40000,70000] (integer thousands), 13 z bins, Nk = 50 to
-    k = 0.3 s/km, noise median 0.1, 10% masked in runs of 5-40 pixels."""
+    k = 0.3 s/km, noise median 0.1, 10% masked in runs of 5-40 pixels.
+    n_range narrows the pixel-count range (tests)."""
     b = Binning(k0=0.0, nklin=30, dklin=0.002, nklog=20, dklog=math.log10(0.3 / 0.06) / 20,
                 z0=2.0, z_bin_width=0.2, nz=13, method=1, cont_logl_order=1, cont_lam_order=-1,
-                max_forest_length_v=2.4e4, max_pixel_length_v=150.0)
+                # chunks span 2000-12000 km/s; the short-chunk cut is 0.45 x this
+                max_forest_length_v=4.0e3, max_pixel_length_v=150.0)
     w = Workload("hires-512", b, nv=24201, v_length=12100.0, nz_table=29, pd13=PD13_DEFAULT,
                  description="N~U[800,2000], dv~U[2.5,6], R~U[40000,70000], 13 z bins, Nk=50")
     rng = np.random.default_rng(seed)
     for _ in range(nchunks):
-        N = int(rng.integers(800, 2001))
+        N = int(rng.integers(n_range[0], n_range[1] + 1))
         dvp = round(rng.uniform(2.5, 6.0), 1)
         R_fwhm = int(round(rng.uniform(40000, 70000) / 1000.0)) * 1000
         zc = rng.uniform(2.2, 4.2)

@@ -150,3 +150,39 @@ def test_poisoned_scratch_gives_identical_results(precision, monkeypatch):
     assert np.array_equal(F0, F2) and np.array_equal(d0, d2)
     for i in out0:
         assert np.array_equal(out0[i][0], out1[i][0]) and np.array_equal(out0[i][1], out1[i][1])
+
+
+def test_hires_chunks_fp64_match_oracle():
+    """High-resolution workload (Nk = 50, 0.5 km/s tables, masked runs) at
+    N ~ 850, where the oracle still finishes in seconds."""
+    _print_case("hires N~850 fp64 vs oracle")
+    w = synth.hires_config(nchunks=2, n_range=(820, 880))
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 21)
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, th, idx)
+    eng.close()
+    oout, oF, od = P.run_oracle(w, th, idx)
+    assert set(out) == set(oout) and len(out) == 2
+    for i in out:
+        assert P.rel_err(out[i][0], oout[i][0]) < FP64_TOL
+        for k in range(3):
+            assert P.rel_err(out[i][1][k], oout[i][1][k]) < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+
+
+def test_hires_full_size_mixed_precision_against_fp64():
+    """N ~ 2000 (Np = 2048, n up to ~250 stored Q): the mixed paths against
+    the FP64 path at the largest chunk size of the high-resolution workload."""
+    _print_case("hires N~2000 mixed vs fp64")
+    w = synth.hires_config(nchunks=2, n_range=(1950, 2000))
+    T = w.binning.total_kz_bins
+    idx = list(range(len(w.chunks)))
+    _, _, F64, d64 = P.run_engine(w, np.zeros(T), idx, precision="fp64", scratch_gb=64)
+    for precision, tol in (("fp32", 2e-4), ("bf16x3", 2e-3)):
+        _, _, Fm, dm = P.run_engine(w, np.zeros(T), idx, precision=precision, scratch_gb=64)
+        ef = P.fisher_norm_err(Fm, F64)
+        eb = P.rel_err(dm[1], d64[1])
+        et = P.rel_err(dm[2], d64[2])
+        print(f"hires {precision}: F {ef:.2e} b {eb:.2e} t {et:.2e}", flush=True)
+        assert np.isfinite(Fm).all() and P.rel_err(dm[0], d64[0]) < 1e-12
+        assert ef < tol and eb < tol and et < tol

@@ -104,6 +104,35 @@ def run_engine(w, precision, steps, warmup, scratch_gb):
     return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms)
 
 
+def copy_calibration(mb: int):
+    """A plain device-to-device copy of mb MiB (the blit kernel of the HIP
+    runtime), run twice: the reference for scaling the FETCH_SIZE and
+    WRITE_SIZE counters of the engine's kernels to bytes."""
+    import torch
+    n = mb * 1024 * 1024
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 2}), flush=True)
+
+
+TRAFFIC_DIR = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel: str, precision: str, workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary (profiles/traffic_<kernel>_<precision>_<workload>.json), if one
+    was recorded for this configuration."""
+    path = os.path.join(TRAFFIC_DIR, f"traffic_{kernel}_{precision}_{workload}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return float(t["hbm_bytes_per_launch"]), os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(w, budget_s=20.0, max_chunks=48):
     """The CPU oracle (numpy + threaded LAPACK/BLAS) on the first chunks of
     the same batch, until budget_s of CPU work is spent."""
@@ -147,7 +176,12 @@ def main():
     ap.add_argument("--scratch-gb", type=float, default=96.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain MIB-MiB device copy (PMC counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU sharding of chunks is described in "
               "DESIGN.md but not built, so --gpus must be 1", file=sys.stderr)
@@ -204,9 +238,16 @@ def main():
     work = kernel_work(resm["shapes"], args.precision)
     if dom in ("whiten_left", "whiten_right", "gram"):
         ach = work[dom] / (km[dom] / 1e3)
+        traffic, tsrc = measured_traffic(dom, args.precision, w.name) if not args.nchunks else (None, None)
         roof = {"kernel": dom, "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
-                "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": traffic,
                 "launches": nb, "avg_launch_ms": km[dom] / nb}
+        if tsrc:
+            roof["traffic_source"] = tsrc
+            # the same launches against the HBM roof: at split-BF16 operand
+            # planes (4 bytes per element) the whitening products are close to it
+            roof["hbm_achieved_GBps"] = traffic / (roof["avg_launch_ms"] / 1e3) / 1e9
+            roof["hbm_frac"] = roof["hbm_achieved_GBps"] * 1e9 / HBM_PEAK
     else:
         byt = work["xgen_bytes"] if dom == "xgen" else None
         ach = (byt / (km[dom] / 1e3)) if byt else None

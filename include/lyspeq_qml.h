@@ -87,6 +87,27 @@ typedef struct {
     int R_fwhm;           /* spectrograph resolution (integer R) */
 } qml_chunk_in;
 
+/* A chunk's resolution matrix, in the reference's storage
+ * (mathtools/matrix_helper.hpp:122-216), for the pixel range of the
+ * qml_chunk_in it comes with (the engine cuts it with the masked boundary,
+ * io/qso_file.cpp:82-85).
+ *   QML_RMAT_DIA: mxhelp::DiaMatrix, values nelem x size; diagonal d holds
+ *     R[c - (nelem/2 - d)][c] at column c.  Q and S are built on the pixel
+ *     grid and sandwiched (Resolution::sandwich, DiaMatrix case).
+ *   QML_RMAT_OVERSAMPLED: mxhelp::OversampledMatrix, values size x nelem;
+ *     row i holds R[i][i*oversampling + a], on a fine grid of
+ *     size*oversampling + nelem - 1 wavelengths spaced by the chunk's median
+ *     dlambda / oversampling and starting nelem/2 fine steps before its first
+ *     pixel (core/chunk_estimate.cpp:936-944). */
+#define QML_RMAT_DIA 1
+#define QML_RMAT_OVERSAMPLED 2
+typedef struct {
+    int kind;             /* QML_RMAT_* */
+    int nelem;            /* diagonals or elements per row; odd */
+    int oversampling;     /* QML_RMAT_OVERSAMPLED: factor >= 1 */
+    const double *values; /* read during the call only */
+} qml_rmat;
+
 /* What the Chunk constructor derived (core/chunk_estimate.cpp:141-283). */
 typedef struct {
     int size;               /* pixels after the masked-boundary cut */
@@ -158,6 +179,14 @@ int qml_add_chunk(qml_ctx *ctx, const qml_chunk_in *in, qml_chunk_info *info, in
 /* Uploads all registered chunks, plans the size-bucketed batches and
  * allocates device scratch.  Inputs stay resident in device memory. */
 int qml_commit(qml_ctx *ctx);
+
+/* qml_add_chunk with a resolution matrix (specifics::USE_RESOLUTION_MATRIX):
+ * the chunk uses table 0 for S_fid (RES_INDEX = 0), Q_k is the analytic band
+ * window (dk/pi) cos(k_c v) sinc(dk v/2) on the matrix's grid, and both are
+ * sandwiched by R (core/chunk_estimate.cpp:189-193,400-480).  Limit:
+ * 15*oversampling + nelem <= 128. */
+int qml_add_chunk_rmat(qml_ctx *ctx, const qml_chunk_in *chunk, const qml_rmat *rmat, qml_chunk_info *info,
+                       int *chunk_id);
 
 /* One Newton-Raphson iteration's chunk loop.  theta is TOTAL_KZ_BINS values
  * (the ps_estimate argument of Chunk::oneQSOiteration).  Adds every chunk's

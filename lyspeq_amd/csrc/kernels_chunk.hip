@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_theta_tables(const ChunkDev *cs, const 
 // mirrored (core/chunk_estimate.cpp:384-536,668-698).
 __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const double *in,
                                                   const TableDev *tabs, BinsDev bins,
-                                                  const double *aux, double *mat) {
+                                                  const double *aux, const double *theta, double *mat) {
     const ChunkDev c = cs[blockIdx.y];
     const int n32 = c.Np / 32;
     const int I = blockIdx.x / n32, J = blockIdx.x % n32;
@@ -57,6 +57,23 @@ __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const doub
         if (i >= c.N || j >= c.N) {
             a = (i == j) ? 1.0 : 0.0;
             s = 0.0;
+        } else if (c.rm) {
+            // resolution-matrix mode: S and the band matrices are materialised
+            double v, z;
+            vz_pair(wave[i], wave[j], i, j, v, z);
+            const size_t np2 = (size_t)c.Np * c.Np, ij = (size_t)i * c.Np + j;
+            const double *Qb = mat + c.qb_off;
+            s = bins.turn_off_sfid ? 0.0 : Qb[(size_t)c.kbu * np2 + ij];
+            double cij = s;
+            for (int iz = 0; iz < c.nzq; ++iz) {
+                const double w = zweight(bins, z, c.zm0 + iz);
+                if (w == 0.0) continue;
+                const double *th = theta + (size_t)(c.zm0 + iz) * bins.nk;
+                double q = 0.0;
+                for (int kn = 0; kn < c.kbu; ++kn) q += th[kn] * Qb[(size_t)kn * np2 + ij];
+                cij += w * q;
+            }
+            a = D[i] * cij * D[j] + ((i == j) ? 1.0 : 0.0);
         } else {
             double v, z;
             vz_pair(wave[i], wave[j], i, j, v, z);
@@ -352,7 +369,8 @@ __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double 
                 double val = 0.0;
                 if (rb < RB && bok) {
                     if (kn < c.kbu)
-                        val = t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
+                        val = c.rm ? mat[c.qb_off + ((size_t)kn * Np + i) * Np + b0 + bl]
+                                   : t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
                     else if (kn == c.kbu)
                         val = sS[wv][bl];
                 }
@@ -633,6 +651,12 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
         for (int e = 0; e < EPT; ++e)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
+                if (c.rm) {
+                    // resolution-matrix mode: the band matrix, materialised in FP64
+                    const bool okeu = (okm >> (4 * e + u)) & 1u;
+                    q[e][u] = okeu ? (CT)mat[c.qb_off + ((size_t)kn * Np + i0 + rl + 16 * e) * Np + bb + u] : (CT)0;
+                    continue;
+                }
                 CT y0, y1;
                 if (staged) {
                     const int o = iv[e][u] - lo;   // in [0, len - 2] for valid elements, 0 otherwise
@@ -792,10 +816,12 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
 
 // ------------------------------------------------------------------ launch --
 void launch_assemble(const LaunchCtx &L, const double *theta) {
-    dim3 gt((L.nvu_max + 255) / 256, L.nchunks);
-    k_theta_tables<<<gt, 256, 0, L.stream>>>(L.chunks, L.tables, L.bins, theta, L.pj);
+    if (L.nvu_max > 0) {   // chunks with a resolution matrix have no table rows
+        dim3 gt((L.nvu_max + 255) / 256, L.nchunks);
+        k_theta_tables<<<gt, 256, 0, L.stream>>>(L.chunks, L.tables, L.bins, theta, L.pj);
+    }
     dim3 grid((L.np_max / 32) * (L.np_max / 32), L.nchunks);
-    k_assemble<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.pj, L.mat);
+    k_assemble<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.pj, theta, L.mat);
 }
 void launch_solve(const LaunchCtx &L) {
     const dim3 gp((L.np_max + 255) / 256, L.nchunks), gr(L.np_max / 16, L.nchunks), gc(L.np_max / 64, L.nchunks);

@@ -48,6 +48,10 @@ struct ChunkHost {
     int n = 0, m = 0, kbu = 0, nzq = 0, zm0 = 0, nq = 0, fis = 0;
     int nvu = 0;
     long long in_off = 0;
+    // resolution matrix: rows of R (N x rnb) and the fine grid (rM values)
+    int rm = 0, rnb = 0, ros = 1, rcol0 = 0, rM = 0;
+    std::vector<double> rband, fine;
+    long long rin_off = 0, rt_off = 0;
 };
 
 struct Batch {
@@ -56,6 +60,7 @@ struct Batch {
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
     int np_max = 0, nvu_max = 0, nu_max = 0;
+    int rm_kbu = -1, rm_M = 0;   // resolution-matrix chunks: largest KBIN_UPP and fine grid
     hipEvent_t evA[7]{};   // stage A (assembly .. solves) on stream st
     hipEvent_t evB[6]{};   // stage B (stack generation .. Gram) on stream st2
 };
@@ -144,6 +149,7 @@ struct qml_ctx {
     double *d_gram = nullptr;
     void *d_X = nullptr, *d_Y = nullptr;
     double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
+    double *d_rin = nullptr, *d_rt = nullptr;   // resolution-matrix rows and R Qhat scratch
     std::vector<Batch> batches;
     hipEvent_t ev_begin{}, ev_end{}, ev_acc{};
     qml_timings tm{};
@@ -198,7 +204,7 @@ int qml_destroy(qml_ctx *c) {
         for (auto &e : b.evB) if (e) hipEventDestroy(e);
     }
     void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip, c->d_gram,
-                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum};
+                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum, c->d_rin, c->d_rt};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < 2; ++k) {
         void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
@@ -229,6 +235,10 @@ int qml_set_bins(qml_ctx *c, const qml_bins *b) {
     c->bins.zbin_centers = c->zcenters.data();
     std::memset(&c->binsdev, 0, sizeof(c->binsdev));
     for (int z = 0; z < b->nz; ++z) c->binsdev.zc[z] = c->zcenters[z];
+    for (int k = 0; k < b->nk; ++k) {
+        c->binsdev.kc[k] = c->kcenters[k];
+        c->binsdev.dk[k] = c->kedges[k + 1] - c->kedges[k];
+    }
     c->binsdev.dz = b->z_bin_width;
     c->binsdev.nz = b->nz;
     c->binsdev.nk = b->nk;
@@ -274,7 +284,8 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
     return QML_OK;
 }
 
-int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int *chunk_id) {
+static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm, qml_chunk_info *info,
+                          int *chunk_id) {
     if (!c || !in || !c->bins_set) return fail(QML_ERR_ARG, "qml_add_chunk: set bins first");
     if (c->committed) return fail(QML_ERR_ARG, "qml_add_chunk: context already committed");
     if (in->size <= 0 || !in->wave || !in->delta || !in->ivar)
@@ -305,11 +316,19 @@ int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int 
     const int minnopix = (int)(lenv / b.max_pixel_length_v);
     const double min_len = 0.90 / 2 * b.max_forest_length_v;
     if (real < minnopix || lenv < min_len) return fail(QML_ERR_SHORT_CHUNK, "Short chunk");
-    // findSpecResIndex (core/sq_table.cpp:388-397)
+    // findSpecResIndex (core/sq_table.cpp:388-397); with a resolution matrix
+    // the index is 0 (core/chunk_estimate.cpp:189-193)
     const double rounded = std::round(dv * 10) / 10;
     int rindex = -1;
-    for (size_t r = 0; r < c->tables.size(); ++r)
-        if (c->tables[r].R == in->R_fwhm && std::fabs(rounded - c->tables[r].dv) < 0.01) { rindex = (int)r; break; }
+    if (rm) {
+        if (!c->tables.empty()) rindex = 0;
+    } else {
+        for (size_t r = 0; r < c->tables.size(); ++r)
+            if (c->tables[r].R == in->R_fwhm && std::fabs(rounded - c->tables[r].dv) < 0.01) {
+                rindex = (int)r;
+                break;
+            }
+    }
     if (rindex < 0) {
         char msg[128];
         std::snprintf(msg, sizeof msg, "SpecRes R: %d, dv: %g not found in tables!", in->R_fwhm, dv);
@@ -337,10 +356,49 @@ int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int 
     nq *= b.nk;
     if (zl < 0 || zu >= b.nz || fis < 0 || fis + nq > c->T)
         return fail(QML_ERR_LIMIT, "qml_add_chunk: chunk lies outside the redshift bins");
-    {
+    if (rm) {
+        // R rows for the cut pixels [i1, i2) (Resolution(rmaster, i1, i2),
+        // mathtools/matrix_helper.cpp:936-970) and the matrix's grid
+        const int nel = rm->nelem;
+        ch.rm = 1;
+        ch.rnb = nel;
+        ch.rband.assign((size_t)N * nel, 0.0);
+        if (rm->kind == QML_RMAT_DIA) {
+            // row i, slot t <-> column i - nel/2 + t on diagonal nel-1-t
+            ch.ros = 1;
+            ch.rcol0 = -(nel / 2);
+            ch.rM = N;
+            for (int i = 0; i < N; ++i)
+                for (int t = 0; t < nel; ++t) {
+                    const int col = i - nel / 2 + t;
+                    if (col >= 0 && col < N) ch.rband[(size_t)i * nel + t] = rm->values[(size_t)(nel - 1 - t) * in->size + i1 + col];
+                }
+        } else {
+            ch.ros = rm->oversampling;
+            ch.rcol0 = 0;
+            ch.rM = N * ch.ros + nel - 1;
+            for (int i = 0; i < N; ++i)
+                for (int t = 0; t < nel; ++t) ch.rband[(size_t)i * nel + t] = rm->values[(size_t)(i1 + i) * nel + t];
+        }
+        ch.nvu = 0;   // the derivative table is not used
+    } else {
         const TableHost &tb = c->tables[rindex];
         const double tdv = tb.v_length / (tb.nv - 1.0);
         ch.nvu = (int)std::min<double>(tb.nv, std::floor(lenv / tdv) + 4.0);
+    }
+    if (rm) {
+        // the grid of the matrix (core/chunk_estimate.cpp:936-944), units of lambda_Lya
+        ch.fine.resize(ch.rM);
+        if (rm->kind == QML_RMAT_DIA) {
+            for (int i = 0; i < N; ++i) ch.fine[i] = ch.wave[i] / LYA_REST;
+        } else {
+            std::vector<double> dl(N - 1);
+            for (int i = 0; i < N - 1; ++i) dl[i] = ch.wave[i + 1] - ch.wave[i];
+            std::sort(dl.begin(), dl.end());
+            const double fine_dl = median_sorted(dl) / LYA_REST / ch.ros;
+            const int disp = ch.rnb / 2;
+            for (int a = 0; a < ch.rM; ++a) ch.fine[a] = ch.wave[0] / LYA_REST + (a - disp) * fine_dl;
+        }
     }
     for (double &w : ch.wave) w /= LYA_REST;
     ch.N = N;
@@ -412,6 +470,25 @@ int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int 
     return QML_OK;
 }
 
+int qml_add_chunk(qml_ctx *c, const qml_chunk_in *in, qml_chunk_info *info, int *chunk_id) {
+    return add_chunk_impl(c, in, nullptr, info, chunk_id);
+}
+
+int qml_add_chunk_rmat(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm, qml_chunk_info *info, int *chunk_id) {
+    if (!rm || !rm->values || rm->nelem < 1 || rm->nelem % 2 == 0)
+        return fail(QML_ERR_ARG, "qml_add_chunk_rmat: resolution matrix needs an odd number of elements per row");
+    if (rm->kind == QML_RMAT_DIA) {
+        if (16 + rm->nelem > RM_ROWS) return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: too many diagonals");
+    } else if (rm->kind == QML_RMAT_OVERSAMPLED) {
+        if (rm->oversampling < 1 || 15 * rm->oversampling + rm->nelem > RM_ROWS)
+            return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: 15 x oversampling + nelem exceeds 128");
+    } else {
+        return fail(QML_ERR_ARG, "qml_add_chunk_rmat: unknown matrix kind");
+    }
+    if (in && in->size < 2) return fail(QML_ERR_ARG, "qml_add_chunk_rmat: chunk too small");
+    return add_chunk_impl(c, in, rm, info, chunk_id);
+}
+
 int qml_commit(qml_ctx *c) {
     if (!c || !c->bins_set) return fail(QML_ERR_ARG, "qml_commit: set bins first");
     if (c->committed) return fail(QML_ERR_ARG, "qml_commit: already committed");
@@ -470,6 +547,24 @@ int qml_commit(qml_ctx *c) {
     }
     HIPCHK(hipMalloc(&c->d_in, hin.size() * sizeof(double)));
     HIPCHK(hipMemcpy(c->d_in, hin.data(), hin.size() * sizeof(double), hipMemcpyHostToDevice));
+    {   // resolution-matrix rows and grids
+        size_t rin_total = 0;
+        for (auto &ch : c->chunks)
+            if (ch.rm) {
+                ch.rin_off = (long long)rin_total;
+                rin_total += ch.rband.size() + ch.fine.size();
+            }
+        if (rin_total) {
+            std::vector<double> hr(rin_total);
+            for (const auto &ch : c->chunks)
+                if (ch.rm) {
+                    std::copy(ch.rband.begin(), ch.rband.end(), hr.begin() + ch.rin_off);
+                    std::copy(ch.fine.begin(), ch.fine.end(), hr.begin() + ch.rin_off + ch.rband.size());
+                }
+            HIPCHK(hipMalloc(&c->d_rin, rin_total * sizeof(double)));
+            HIPCHK(hipMemcpy(c->d_rin, hr.data(), rin_total * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
     HIPCHK(hipMalloc(&c->d_outs, std::max(nch, 1) * sizeof(ChunkOut)));
     if (nch) HIPCHK(hipMemcpy(c->d_outs, c->outs.data(), nch * sizeof(ChunkOut), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->d_fc, std::max<size_t>(fc_total, 1) * sizeof(double)));
@@ -483,12 +578,14 @@ int qml_commit(qml_ctx *c) {
     HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
 
     // per-chunk scratch sizes (elements of each arena)
-    struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
+    struct Need { size_t mat, vec, pj, sj, stk, lic, gram, rt; };
     auto need = [&](const ChunkHost &ch) {
         Need n{};
         const size_t Np = ch.Np, m = ch.m, nt = Np / TILE, ntu = nt * (nt + 1) / 2;
         const size_t mt = (m + TILE - 1) / TILE;
-        n.mat = 4 * Np * Np;
+        // A, L, Li, S, then for a resolution-matrix chunk its kbu Q_k and S
+        n.mat = (4 + (ch.rm ? (size_t)ch.kbu + 1 : 0)) * Np * Np;
+        n.rt = ch.rm ? (size_t)ch.N * ch.rM : 0;
         n.vec = (size_t)VEC_SLOTS * Np;
         n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
         n.sj = m * MAX_NU * MAX_NU + ch.n;
@@ -499,7 +596,7 @@ int qml_commit(qml_ctx *c) {
     };
     auto bytes = [&](const Need &n) {
         // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
-        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
+        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram + n.rt) + es * (2 * n.stk + 2 * n.lic);
     };
     std::vector<int> order(nch);
     std::iota(order.begin(), order.end(), 0);
@@ -565,14 +662,23 @@ int qml_commit(qml_ctx *c) {
             d.stk_off = (long long)acc.stk * su;
             d.gram_off = (long long)acc.gram;
             d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic * su;
+            d.rm = ch.rm;
+            d.rnb = ch.rnb;
+            d.ros = ch.ros;
+            d.rcol0 = ch.rcol0;
+            d.rM = ch.rM;
+            d.rin_off = ch.rin_off;
+            d.rt_off = (long long)acc.rt;
+            d.qb_off = (long long)(acc.mat + 4 * (size_t)ch.Np * ch.Np);
             acc.mat += n.mat; acc.vec += n.vec; acc.pj += n.pj; acc.sj += n.sj;
-            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram;
+            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram; acc.rt += n.rt;
             cd.push_back(d);
         }
         mx.mat = std::max(mx.mat, acc.mat); mx.vec = std::max(mx.vec, acc.vec);
         mx.pj = std::max(mx.pj, acc.pj); mx.sj = std::max(mx.sj, acc.sj);
         mx.stk = std::max(mx.stk, acc.stk); mx.lic = std::max(mx.lic, acc.lic);
         mx.gram = std::max(mx.gram, acc.gram);
+        mx.rt = std::max(mx.rt, acc.rt);
         devs.push_back(cd);
     }
     for (int k = 0; k < 2; ++k) {
@@ -584,6 +690,7 @@ int qml_commit(qml_ctx *c) {
         HIPCHK(hipMemset(c->d_vec[k], 0, std::max<size_t>(mx.vec, 1) * 8));
     }
     HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
+    if (mx.rt) HIPCHK(hipMalloc(&c->d_rt, mx.rt * 8));
     HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
     HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
     // LYSPEQ_QML_POISON=1 fills every scratch arena with all-ones bytes (NaN
@@ -613,6 +720,10 @@ int qml_commit(qml_ctx *c) {
             B.np_max = std::max(B.np_max, d.Np);
             B.nvu_max = std::max(B.nvu_max, d.nvu);
             B.nu_max = std::max(B.nu_max, d.nu);
+            if (d.rm) {
+                B.rm_kbu = std::max(B.rm_kbu, d.kbu);
+                B.rm_M = std::max(B.rm_M, d.rM);
+            }
             // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16.
             const int g = (c->prec == 2) ? 2 : 1;
             const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
@@ -667,6 +778,10 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.np_max = B.np_max;
         L.nvu_max = B.nvu_max;
         L.nu_max = B.nu_max;
+        L.rin = c->d_rin;
+        L.rt = c->d_rt;
+        L.rm_kbu = B.rm_kbu;
+        L.rm_M = B.rm_M;
         L.in = c->d_in;
         L.tables = c->d_tables;
         L.bins = c->binsdev;
@@ -678,6 +793,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
         if (c->poison) launch_poison_lds(c->st);
         HIPCHK(hipEventRecord(B.evA[0], c->st));
+        if (B.rm_kbu >= 0) launch_rmat(L);
         launch_assemble(L, c->d_theta);
         HIPCHK(hipEventRecord(B.evA[1], c->st));
         launch_cholesky(L, c->d_status);

@@ -27,6 +27,7 @@ static_assert(V_G < VEC_SLOTS, "vector slots");
 
 struct BinsDev {
     double zc[MAX_ZB];
+    double kc[MAX_NK], dk[MAX_NK];   // k-band centres and widths (resolution-matrix mode)
     double dz;
     int nz, nk, method;
     int turn_off_sfid;
@@ -60,7 +61,18 @@ struct ChunkDev {
     long long sj_off;    // into the s arena (doubles): m x MAX_NU^2, then d (n)
     long long gram_off;  // into the Gram partials (doubles)
     long long lic_off;   // into the compute-type copy of Li
+    // resolution-matrix mode (rm = 1): Q_k = R Qhat_k R^T and S = R Shat R^T
+    // are materialised at qb_off (kbu + 1 matrices of Np x Np in the FP64
+    // matrix arena, S last).  R is stored by rows at rin_off: N x rnb values,
+    // row i covering fine columns rcol0 + i*ros .. + rnb - 1, then the rM
+    // fine-grid wavelengths (units of lambda_Lya).
+    int rm, rnb, ros, rcol0, rM;
+    long long rin_off;   // into the resolution-matrix input arena (doubles)
+    long long rt_off;    // into the R Qhat scratch (N x rM doubles)
+    long long qb_off;    // into the FP64 matrix arena
 };
+// Fine rows a 16-row tile of R touches: 15 ros + rnb <= RM_ROWS.
+constexpr int RM_ROWS = 128;
 
 struct ChunkOut {        // per chunk id, persistent
     long long fc_off;    // n_q x n_q

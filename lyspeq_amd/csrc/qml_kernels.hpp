@@ -19,10 +19,18 @@ struct LaunchCtx {
     double *vec;              // FP64 vector arena
     double *pj;               // aux arena: P_j, At_j, theta tables
     double *sj;               // s_j and d arena
+    // resolution-matrix mode
+    const double *rin;        // R rows and fine grids
+    double *rt;               // R Qhat scratch
+    int rm_kbu;               // largest KBIN_UPP of the batch's resolution-matrix chunks; -1 if none
+    int rm_M;                 // largest fine-grid size of those chunks
 };
 
 // kernels_chunk.hip
 void launch_assemble(const LaunchCtx &L, const double *theta);
+// kernels_rmat.hip: Q_k = R Qhat_k R^T (k < KBIN_UPP) and S = R Shat R^T for
+// the batch's resolution-matrix chunks, into their qb_off matrices.
+void launch_rmat(const LaunchCtx &L);
 // kernels_factor.hip: L = chol(A) and Li = L^-1 (both formed by
 // launch_cholesky; launch_trinv is kept for the timing slot and is empty)
 void launch_cholesky(const LaunchCtx &L, int *status);

@@ -69,6 +69,14 @@ class _ChunkIn(C.Structure):
                 ("ivar", C.POINTER(C.c_double)), ("size", C.c_int), ("R_fwhm", C.c_int)]
 
 
+class _RMat(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nelem", C.c_int), ("oversampling", C.c_int),
+                ("values", C.POINTER(C.c_double))]
+
+
+RMAT_KINDS = {"dia": 1, "oversampled": 2}
+
+
 class ChunkInfo(C.Structure):
     _fields_ = [("size", C.c_int), ("real_size", C.c_int), ("res_index", C.c_int),
                 ("kbin_upp", C.c_int), ("zbin", C.c_int), ("zbin_low", C.c_int),
@@ -97,7 +105,7 @@ KERNEL_NAMES = ["assemble", "cholesky", "trinv", "solve", "dpass", "small", "xge
 _lib = None
 
 EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
-           "qml_add_chunk", "qml_commit", "qml_iterate", "qml_get_chunk_result",
+           "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins"]
 
@@ -119,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_add_table.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp, dp,
                                   C.POINTER(C.c_int)]
     lib.qml_add_chunk.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(ChunkInfo), C.POINTER(C.c_int)]
+    lib.qml_add_chunk_rmat.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(_RMat), C.POINTER(ChunkInfo),
+                                       C.POINTER(C.c_int)]
     lib.qml_commit.argtypes = [vp]
     lib.qml_iterate.argtypes = [vp, dp, dp, dp]
     lib.qml_get_chunk_result.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_int)]
@@ -258,6 +268,23 @@ class QMLEngine:
         info = ChunkInfo()
         cid = C.c_int(-1)
         _check(self._lib.qml_add_chunk(self._ctx, C.byref(cin), C.byref(info), C.byref(cid)), self._lib)
+        self.chunk_infos.append(info)
+        return cid.value, info
+
+    def add_chunk_rmat(self, wave, delta, ivar, R_fwhm: int, kind: str, values, nelem: int,
+                       oversampling: int = 1):
+        """add_chunk with a resolution matrix in the reference's storage:
+        kind "dia" (values nelem x size) or "oversampled" (size x nelem)."""
+        w = np.ascontiguousarray(wave, dtype=np.float64)
+        d = np.ascontiguousarray(delta, dtype=np.float64)
+        iv = np.ascontiguousarray(ivar, dtype=np.float64)
+        vals = np.ascontiguousarray(values, dtype=np.float64)
+        cin = _ChunkIn(_dptr(w), _dptr(d), _dptr(iv), w.size, int(R_fwhm))
+        rm = _RMat(RMAT_KINDS[kind], int(nelem), int(oversampling), _dptr(vals))
+        info = ChunkInfo()
+        cid = C.c_int(-1)
+        _check(self._lib.qml_add_chunk_rmat(self._ctx, C.byref(cin), C.byref(rm), C.byref(info), C.byref(cid)),
+               self._lib)
         self.chunk_infos.append(info)
         return cid.value, info
 

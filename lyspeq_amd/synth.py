@@ -286,7 +286,78 @@ def hires_config(seed: int = 2, nchunks: int = 512, with_tables=True, n_range=(8
     return _finish_tables(w, with_tables)
 
 
-CONFIGS = {"cpu": cpu_reference_config, "desi": desi_config, "hires": hires_config}
+def _gaussian_rows(wave, sigma, nelem, oversampling):
+    """Row-normalised Gaussian resolution rows: oversampling == 1 gives the
+    nelem-diagonal matrix in mxhelp::DiaMatrix storage (nelem x N: diagonal d
+    holds R[c - (nelem//2 - d)][c] at column c), otherwise the
+    mxhelp::OversampledMatrix rows (N x nelem) on a grid of spacing
+    dlambda / oversampling."""
+    N = wave.size
+    dlam = np.median(np.diff(wave))
+    if oversampling == 1:
+        h = nelem // 2
+        dense = np.zeros((N, N))
+        for i in range(N):
+            lo, hi = max(0, i - h), min(N, i + h + 1)
+            g = np.exp(-0.5 * ((wave[lo:hi] - wave[i]) / sigma[i]) ** 2)
+            dense[i, lo:hi] = g / g.sum()
+        vals = np.zeros((nelem, N))
+        for d in range(nelem):
+            off = h - d
+            for c in range(N):
+                r = c - off
+                if 0 <= r < N:
+                    vals[d, c] = dense[r, c]
+        return vals
+    noff = (nelem - 1) // (2 * oversampling)
+    x = (np.arange(nelem) / oversampling - noff) * dlam
+    g = np.exp(-0.5 * (x[None, :] / sigma[:, None]) ** 2)
+    return g / g.sum(axis=1, keepdims=True)
+
+
+def rmat_config(seed: int = 3, nchunks: int = 8192, with_tables=True, n_range=(100, 1200),
+                kind: str = "oversampled") -> Workload:
+    """Resolution-matrix batch: N ~ log-U[100,1200] on 0.8 A pixels, an
+    11-diagonal Gaussian resolution matrix per chunk with the per-pixel width
+    jittered by +-15% (R ~ U[2000,5000]), oversampled x3 (31 elements per
+    row) or kept diagonal (kind "dia"), z ~ U[2.0,4.4], 13 z bins, Nk = 35,
+    degree-2 polynomial in ln lambda for the continuum.  Q_k is the analytic
+    band window on the matrix's grid; S_fid comes from table 0."""
+    b = Binning(k0=0.0, nklin=20, dklin=0.001, nklog=15, dklog=math.log10(5.0) / 15,
+                z0=2.0, z_bin_width=0.2, nz=13, method=1, cont_logl_order=2, cont_lam_order=-1,
+                # N = 100 pixels at z = 4.4 span 3600 km/s; the short-chunk cut is 0.45 x this
+                max_forest_length_v=7.0e3)
+    w = Workload(f"rmat-{kind}-{nchunks}", b, nv=8001, v_length=80000.0, nz_table=29, pd13=PD13_DEFAULT,
+                 description=f"N~logU[{n_range[0]},{n_range[1]}] on 0.8 A pixels, 11-diagonal Gaussian R "
+                             f"({kind}), 13 z bins, Nk=35, 3 templates")
+    rng = np.random.default_rng(seed)
+    os_ = 3 if kind == "oversampled" else 1
+    nelem = 2 * 5 * os_ + 1
+    for _ in range(nchunks):
+        N = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        R_fwhm = int(round(rng.uniform(2000, 5000) / 100.0)) * 100
+        zc = rng.uniform(2.0, 4.4)
+        lc = LYA_REST * (1 + zc)
+        wave = lc + 0.8 * (np.arange(N) - (N - 1) / 2.0)
+        v = SPEED_OF_LIGHT * np.log(wave / lc)
+        dvp = float(np.median(np.diff(v)))
+        R_kms = SPEED_OF_LIGHT / R_fwhm / ONE_SIGMA_2_FWHM
+        delta = _realise_delta(rng, v, zc, R_kms, dvp, w.pd13)
+        sigma, ivar = _noise_and_mask(rng, N, 0.3, 0.4, 0.05)
+        delta = delta + sigma * rng.standard_normal(N)
+        sig_l = wave / (R_fwhm * ONE_SIGMA_2_FWHM) * (1 + rng.uniform(-0.15, 0.15, N))
+        w.chunks.append(dict(wave=wave, delta=delta, ivar=ivar, R=R_fwhm, dv_table=50.0,
+                             rmat=dict(kind=kind, nelem=nelem, oversampling=os_,
+                                       values=_gaussian_rows(wave, sig_l, nelem, os_))))
+    # table 0 supplies S_fid on (z, v); its derivative rows are not used
+    w.pairs = [(3000, 50.0)]
+    if with_tables:
+        w.deriv = derivative_tables(b.kband_edges(), w.pairs, w.nv, w.v_length)
+        w.signal = signal_tables(b, w.pairs, w.nv, w.v_length, w.nz_table, w.pd13)
+    return w
+
+
+CONFIGS = {"cpu": cpu_reference_config, "desi": desi_config, "hires": hires_config, "rmat": rmat_config}
 
 
 def load_engine(engine, w: Workload, chunk_ids: Optional[List[int]] = None):
@@ -300,7 +371,12 @@ def load_engine(engine, w: Workload, chunk_ids: Optional[List[int]] = None):
     for i in ids:
         c = w.chunks[i]
         try:
-            cid, _ = engine.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
+            if "rmat" in c:
+                rm = c["rmat"]
+                cid, _ = engine.add_chunk_rmat(c["wave"], c["delta"], c["ivar"], c["R"], rm["kind"],
+                                               rm["values"], rm["nelem"], rm["oversampling"])
+            else:
+                cid, _ = engine.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
             ok.append((i, cid))
         except QMLError as e:
             if e.code in (-3, -4, -5):

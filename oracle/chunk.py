@@ -73,6 +73,46 @@ class TableSet:
         return v1, dv, z1, dz
 
 
+@dataclass
+class ResoMatrix:
+    """A chunk's resolution matrix in the reference's storage
+    (mathtools/matrix_helper.hpp:122-216).
+
+    kind "dia": mxhelp::DiaMatrix, ``values`` (nelem, size): diagonal d holds
+    R[c - (nelem//2 - d)][c] at column c (mathtools/matrix_helper.cpp:504-519,
+    770-785).  kind "oversampled": mxhelp::OversampledMatrix, ``values``
+    (size, nelem): row i holds R[i][i*oversampling + a], with ncols =
+    size*oversampling + nelem - 1 (mathtools/matrix_helper.cpp:833-842)."""
+    kind: str
+    values: np.ndarray
+    nelem: int
+    oversampling: int = 1
+
+    def subset(self, i1: int, i2: int) -> "ResoMatrix":
+        # Resolution(rmaster, i1, i2), mathtools/matrix_helper.cpp:936-970
+        if self.kind == "dia":
+            return ResoMatrix("dia", self.values[:, i1:i2].copy(), self.nelem)
+        return ResoMatrix("oversampled", self.values[i1:i2].copy(), self.nelem, self.oversampling)
+
+    def dense(self, n: int) -> np.ndarray:
+        """R as an n x ncols array."""
+        if self.kind == "dia":
+            r = np.zeros((n, n))
+            h = self.nelem // 2
+            for d in range(self.nelem):
+                off = h - d
+                for c in range(n):
+                    i = c - off
+                    if 0 <= i < n:
+                        r[i, c] = self.values[d, c]
+            return r
+        ncols = n * self.oversampling + self.nelem - 1
+        r = np.zeros((n, ncols))
+        for i in range(n):
+            r[i, i * self.oversampling:i * self.oversampling + self.nelem] = self.values[i]
+        return r
+
+
 def median_sorted(a: np.ndarray) -> float:
     # stats::medianOfSortedArray, mathtools/stats.cpp:8-16
     n = a.size
@@ -95,7 +135,8 @@ class Chunk:
     """Mirror of class Chunk (core/chunk_estimate.hpp:46-119)."""
 
     def __init__(self, bins: Bins, tables: List[TableSet], wave, delta, ivar,
-                 R_fwhm: int, i1: int = 0, i2: Optional[int] = None):
+                 R_fwhm: int, i1: int = 0, i2: Optional[int] = None,
+                 rmat: Optional[ResoMatrix] = None):
         wave = np.asarray(wave, dtype=np.float64)
         i2 = wave.size if i2 is None else i2
         w = wave[i1:i2].copy()
@@ -107,6 +148,8 @@ class Chunk:
             raise ChunkError("Empty spectrum when masked pixels at boundary are removed!")
         a, b = nz_idx[0], nz_idx[-1] + 1
         w, d, iv = w[a:b], d[a:b], iv[a:b]
+        # the resolution matrix follows the cut (io/qso_file.cpp:82-85)
+        self.rmat = rmat.subset(i1 + a, i1 + b) if rmat is not None else None
         self.bins = bins
         self.N = w.size
         self.real_size = int(np.count_nonzero(iv))
@@ -121,8 +164,9 @@ class Chunk:
         minnopix = int(lenv / bins.max_pixel_length_v)
         if self.real_size < minnopix or lenv < bins.min_forest_length_v:
             raise ChunkError("Short chunk")
-        self.res_index = find_spec_res_index(tables, self.R_fwhm, self.dv_kms)
-        if self.res_index == -1:
+        # with a resolution matrix the table index is 0 (core/chunk_estimate.cpp:189-193)
+        self.res_index = 0 if self.rmat is not None else find_spec_res_index(tables, self.R_fwhm, self.dv_kms)
+        if self.res_index == -1 or not tables:
             raise ChunkError(f"SpecRes R: {self.R_fwhm}, dv: {self.dv_kms} not found in tables!")
         self.table = tables[self.res_index]
         # readMinMaxMedRedshift, io/qso_file.cpp:280-300
@@ -137,6 +181,19 @@ class Chunk:
         self.wave = w / LYA_REST
         self.delta = d
         self.ivar = iv
+        if self.rmat is not None:
+            # fine grid (core/chunk_estimate.cpp:936-944): oversampled R spans
+            # ncols points of spacing dlambda / LYA_REST / oversampling, the
+            # chunk's median dlambda (io/qso_file.cpp:21-29,325-329)
+            if self.rmat.kind == "oversampled":
+                dlam = median_sorted(np.sort(w[1:] - w[:-1]))
+                fine = dlam / LYA_REST / self.rmat.oversampling
+                ncols = self.N * self.rmat.oversampling + self.rmat.nelem - 1
+                disp = self.rmat.nelem // 2
+                self.fine_wave = self.wave[0] + (np.arange(ncols) - disp) * fine
+            else:
+                self.fine_wave = self.wave.copy()
+            self.R = self.rmat.dense(self.N)
         self._set_nq_and_fisher_index()
         nk = bins.nk
         self.stored = [i for i in range(self.n_q) if (i + self.fisher_index_start) % nk < self.kbin_upp]
@@ -174,9 +231,37 @@ class Chunk:
         m.T[iu] = m[iu]
         return m
 
+    def _fine_vz(self):
+        """v/z on the fine grid of the resolution matrix (upper, mirrored)."""
+        li = self.fine_wave[:, None]
+        lj = self.fine_wave[None, :]
+        n = self.fine_wave.size
+        iu = np.triu_indices(n)
+        v = np.zeros((n, n))
+        z = np.zeros((n, n))
+        v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
+        z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
+        return v, z, iu
+
+    def _sandwich(self, fine):
+        """Resolution::sandwich, R fine R^T, upper triangle mirrored
+        (mathtools/matrix_helper.cpp:877-902)."""
+        out = self.R @ fine @ self.R.T
+        iu = np.triu_indices(self.N, 1)
+        out.T[iu] = out[iu]
+        return out
+
     def signal_matrix(self, v, z):
         """_setFiducialSignalMatrix, core/chunk_estimate.cpp:400-426."""
         v1, dv, z1, dz = self.table.grids(self.bins)
+        if self.rmat is not None:
+            fv, fz, iu = self._fine_vz()
+            n = fv.shape[0]
+            s = np.zeros((n, n))
+            s[iu] = evaluate_2d(fz[iu], fv[iu], z1, dz, v1, dv, self.table.signal.reshape(-1),
+                                self.table.nz, self.table.nv)
+            s.T[np.triu_indices(n, 1)] = s[np.triu_indices(n, 1)]
+            return self._sandwich(s)
         iu = np.triu_indices(self.N)
         s = np.zeros((self.N, self.N))
         s[iu] = evaluate_2d(z[iu], v[iu], z1, dz, v1, dv, self.table.signal.reshape(-1),
@@ -185,6 +270,20 @@ class Chunk:
 
     def q_base(self, v, kn):
         """_setQiMatrix for a first-z-bin k band, core/chunk_estimate.cpp:429-480."""
+        if self.rmat is not None:
+            # analytic band window on the fine grid, then R . Q . R^T (:444-454,471-472)
+            kc = self.bins.kband_centers[kn]
+            dk = self.bins.kband_edges[kn + 1] - self.bins.kband_edges[kn]
+            fv, _, iu = self._fine_vz()
+            n = fv.shape[0]
+            vv = fv[iu]
+            with np.errstate(invalid="ignore", divide="ignore"):
+                x = dk * vv / 2
+                val = np.where(vv == 0, dk / MY_PI, (dk / MY_PI) * np.cos(kc * vv) * (np.sin(x) / x))
+            q = np.zeros((n, n))
+            q[iu] = val
+            q.T[np.triu_indices(n, 1)] = q[np.triu_indices(n, 1)]
+            return self._sandwich(q)
         v1, dv, _, _ = self.table.grids(self.bins)
         iu = np.triu_indices(self.N)
         q = np.zeros((self.N, self.N))

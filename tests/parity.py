@@ -2,7 +2,7 @@
 import numpy as np
 
 from oracle.bins import Bins
-from oracle.chunk import Chunk as OracleChunk, ChunkError, TableSet
+from oracle.chunk import Chunk as OracleChunk, ChunkError, ResoMatrix, TableSet
 from lyspeq_amd.qml import Binning, QMLEngine
 from lyspeq_amd import synth
 
@@ -18,6 +18,14 @@ def oracle_bins(b: Binning) -> Bins:
 def oracle_tables(w: synth.Workload):
     return [TableSet(R, dv, w.nv, w.v_length, w.nz_table, w.deriv[i], w.signal[i].reshape(-1))
             for i, (R, dv) in enumerate(w.pairs)]
+
+
+def oracle_rmat(c):
+    """The workload chunk's resolution matrix for the oracle, or None."""
+    if "rmat" not in c:
+        return None
+    rm = c["rmat"]
+    return ResoMatrix(rm["kind"], np.asarray(rm["values"]), rm["nelem"], rm["oversampling"])
 
 
 def theta_perturbed(b: Binning, pd13, scale, seed):
@@ -39,7 +47,7 @@ def run_oracle(w: synth.Workload, theta, idx):
     for i in idx:
         c = w.chunks[i]
         try:
-            ch = OracleChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+            ch = OracleChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"], rmat=oracle_rmat(c))
         except ChunkError:
             continue
         ok = ch.one_iteration(theta, Fs, ds)

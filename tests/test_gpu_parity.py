@@ -186,3 +186,41 @@ def test_hires_full_size_mixed_precision_against_fp64():
         print(f"hires {precision}: F {ef:.2e} b {eb:.2e} t {et:.2e}", flush=True)
         assert np.isfinite(Fm).all() and P.rel_err(dm[0], d64[0]) < 1e-12
         assert ef < tol and eb < tol and et < tol
+
+
+@pytest.mark.parametrize("kind", ["dia", "oversampled"])
+def test_resolution_matrix_chunks_fp64_match_oracle(kind):
+    """Chunks with their own resolution matrix (11 diagonals, Gaussian rows
+    with jittered widths; kept diagonal or oversampled x3): Q_k = R Qhat_k R^T
+    and S = R Shat R^T on the matrix's grid, 3 continuum templates."""
+    _print_case(f"resolution matrix {kind} fp64 vs oracle")
+    w = synth.rmat_config(nchunks=6, n_range=(100, 170), kind=kind)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 31)
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, th, idx)
+    eng.close()
+    oout, oF, od = P.run_oracle(w, th, idx)
+    assert set(out) == set(oout) and len(out) >= 5
+    worst = 0.0
+    for i in out:
+        worst = max(worst, P.rel_err(out[i][0], oout[i][0]),
+                    *(P.rel_err(out[i][1][k], oout[i][1][k]) for k in range(3)))
+    print(f"resolution matrix {kind}: worst per-chunk relative error {worst:.2e}", flush=True)
+    assert worst < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+    for k in range(3):
+        assert P.rel_err(dbt[k], od[k]) < FP64_TOL
+
+
+def test_resolution_matrix_mixed_precision_against_fp64():
+    _print_case("resolution matrix mixed vs fp64")
+    w = synth.rmat_config(nchunks=12, n_range=(100, 400))
+    T = w.binning.total_kz_bins
+    idx = list(range(len(w.chunks)))
+    _, _, F64, d64 = P.run_engine(w, np.zeros(T), idx, precision="fp64")
+    for precision, tol in (("fp32", 2e-4), ("bf16x3", 2e-3)):
+        _, _, Fm, dm = P.run_engine(w, np.zeros(T), idx, precision=precision)
+        ef, eb, et = P.fisher_norm_err(Fm, F64), P.rel_err(dm[1], d64[1]), P.rel_err(dm[2], d64[2])
+        print(f"resolution matrix {precision}: F {ef:.2e} b {eb:.2e} t {et:.2e}", flush=True)
+        assert P.rel_err(dm[0], d64[0]) < 1e-12
+        assert ef < tol and eb < tol and et < tol

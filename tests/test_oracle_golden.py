@@ -203,3 +203,67 @@ def test_brute_force_tiny_chunk_long_double():
             wc = exact @ qs[cc].astype(np.longdouble)
             ref = float(np.trace(wa @ wc))
             assert abs(F[ch.stored[a], ch.stored[cc]] - ref) <= 1e-11 * abs(F).max()
+
+
+# ---------------------------------------------------------------- resolution matrix
+def _rmat_chunk(kind="dia", nelem=1, values=None, scale=1.0):
+    from lyspeq_amd import synth
+    import parity as P
+    from oracle.chunk import Chunk as OChunk, ResoMatrix
+    w = synth.rmat_config(nchunks=1, n_range=(110, 110), kind="dia")
+    c = w.chunks[0]
+    n = len(c["wave"])
+    if values is None:
+        values = np.ones((nelem, n)) if kind == "dia" else None
+    rm = ResoMatrix(kind, scale * values, nelem, 1 if kind == "dia" else 3)
+    b = P.oracle_bins(w.binning)
+    ivar = np.ones(n)   # no masked boundary: the matrix is used as given
+    return OChunk(b, P.oracle_tables(w), c["wave"], c["delta"], ivar, c["R"], rmat=rm), b
+
+
+def test_rmat_identity_reproduces_the_analytic_band_window():
+    """R = I: Q_kn is the analytic window (dk/pi) cos(kc v) sinc(dk v / 2) on the
+    pixel grid (core/chunk_estimate.cpp:444-454)."""
+    ch, b = _rmat_chunk("dia", 1)
+    v, z = ch.vz_matrices()
+    vv = v + v.T - np.diag(np.diag(v))
+    for kn in (0, 7, 20, 34):
+        q = ch.q_base(v, kn)
+        kc, dk = b.kband_centers[kn], b.kband_edges[kn + 1] - b.kband_edges[kn]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            x = dk * vv / 2
+            want = np.where(vv == 0, dk / np.pi, dk / np.pi * np.cos(kc * vv) * np.sin(x) / x)
+        np.testing.assert_allclose(q, want, rtol=1e-13, atol=1e-16)
+
+
+def test_rmat_band_windows_telescope():
+    """sum_kn Qhat_kn(v) = (sin(k_end v) - sin(k_0 v)) / (pi v): pins k_c and dk
+    (centres at the arithmetic mid-points, core/global_numbers.cpp:240-242)."""
+    ch, b = _rmat_chunk("dia", 1)
+    v, _ = ch.vz_matrices()
+    vv = v + v.T - np.diag(np.diag(v))
+    tot = sum(ch.q_base(v, kn) for kn in range(b.nk))
+    k0, k1 = b.kband_edges[0], b.kband_edges[-1]
+    off = vv != 0
+    np.testing.assert_allclose(tot[off], (np.sin(k1 * vv[off]) - np.sin(k0 * vv[off])) / (np.pi * vv[off]),
+                               rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(np.diag(tot), (k1 - k0) / np.pi, rtol=1e-13)
+
+
+def test_rmat_sandwich_is_quadratic_in_r_and_samples_with_a_delta_row():
+    """Q(2R) = 4 Q(R) for a Gaussian R; an oversampled R whose rows are a unit
+    spike at the row's centre samples Qhat at the pixels themselves."""
+    from lyspeq_amd import synth
+    w = synth.rmat_config(nchunks=1, n_range=(110, 110), kind="dia")
+    vals = w.chunks[0]["rmat"]["values"]
+    ch1, _ = _rmat_chunk("dia", 11, vals)
+    ch2, _ = _rmat_chunk("dia", 11, vals, scale=2.0)
+    v, _ = ch1.vz_matrices()
+    np.testing.assert_allclose(ch2.q_base(v, 5), 4.0 * ch1.q_base(v, 5), rtol=1e-12, atol=1e-18)
+    n = len(w.chunks[0]["wave"])
+    spike = np.zeros((n, 31))
+    spike[:, 15] = 1.0
+    chs, b = _rmat_chunk("oversampled", 31, spike)
+    chi, _ = _rmat_chunk("dia", 1)
+    for kn in (3, 30):
+        np.testing.assert_allclose(chs.q_base(v, kn), chi.q_base(v, kn), rtol=1e-7, atol=1e-12)

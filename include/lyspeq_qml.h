@@ -134,7 +134,7 @@ typedef struct {
     double total_ms;      /* first to last device event of the iteration */
     long long n_batches;
     /* device time per kernel, summed over batches, indexed by QML_K_* */
-    double kernel_ms[12];
+    double kernel_ms[13];
 } qml_timings;
 
 /* Indices into qml_timings.kernel_ms. */
@@ -150,6 +150,7 @@ typedef struct {
 #define QML_K_GRAM 9
 #define QML_K_FINALIZE 10
 #define QML_K_ACCUMULATE 11
+#define QML_K_RMAT 12       /* resolution-matrix sandwiches R Qhat R^T (part of assemble_ms) */
 
 /* Context lifetime.  qml_create selects the device and creates a stream. */
 int qml_create(qml_ctx **ctx, const qml_opts *opts);
@@ -184,7 +185,9 @@ int qml_commit(qml_ctx *ctx);
  * the chunk uses table 0 for S_fid (RES_INDEX = 0), Q_k is the analytic band
  * window (dk/pi) cos(k_c v) sinc(dk v/2) on the matrix's grid, and both are
  * sandwiched by R (core/chunk_estimate.cpp:189-193,400-480).  Limit:
- * 15*oversampling + nelem <= 128. */
+ * 15*oversampling + nelem <= 96 (diagonal storage: oversampling 1), e.g. 11
+ * diagonals oversampled x3 (31 elements per row) give 76.  Beyond it the
+ * call returns QML_ERR_LIMIT. */
 int qml_add_chunk_rmat(qml_ctx *ctx, const qml_chunk_in *chunk, const qml_rmat *rmat, qml_chunk_info *info,
                        int *chunk_id);
 

@@ -51,7 +51,7 @@ struct ChunkHost {
     // resolution matrix: rows of R (N x rnb) and the fine grid (rM values)
     int rm = 0, rnb = 0, ros = 1, rcol0 = 0, rM = 0;
     std::vector<double> rband, fine;
-    long long rin_off = 0, rt_off = 0;
+    long long rin_off = 0;
 };
 
 struct Batch {
@@ -60,9 +60,11 @@ struct Batch {
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
     int np_max = 0, nvu_max = 0, nu_max = 0;
-    int rm_kbu = -1, rm_M = 0;   // resolution-matrix chunks: largest KBIN_UPP and fine grid
+    int rm_kbu = -1;   // resolution-matrix chunks: largest KBIN_UPP
+    RmLayout rm_lay;   // and the LDS layout of their sandwich launch
     hipEvent_t evA[7]{};   // stage A (assembly .. solves) on stream st
     hipEvent_t evB[6]{};   // stage B (stack generation .. Gram) on stream st2
+    hipEvent_t evR{};      // after the resolution-matrix sandwiches (stage A)
 };
 
 double median_sorted(const std::vector<double> &a) {
@@ -149,7 +151,7 @@ struct qml_ctx {
     double *d_gram = nullptr;
     void *d_X = nullptr, *d_Y = nullptr;
     double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
-    double *d_rin = nullptr, *d_rt = nullptr;   // resolution-matrix rows and R Qhat scratch
+    double *d_rin = nullptr;   // resolution-matrix rows and fine grids
     std::vector<Batch> batches;
     hipEvent_t ev_begin{}, ev_end{}, ev_acc{};
     qml_timings tm{};
@@ -202,9 +204,10 @@ int qml_destroy(qml_ctx *c) {
         hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
         for (auto &e : b.evA) if (e) hipEventDestroy(e);
         for (auto &e : b.evB) if (e) hipEventDestroy(e);
+        if (b.evR) hipEventDestroy(b.evR);
     }
     void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip, c->d_gram,
-                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum, c->d_rin, c->d_rt};
+                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum, c->d_rin};
     for (void *p : ptrs) if (p) hipFree(p);
     for (int k = 0; k < 2; ++k) {
         void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
@@ -235,10 +238,7 @@ int qml_set_bins(qml_ctx *c, const qml_bins *b) {
     c->bins.zbin_centers = c->zcenters.data();
     std::memset(&c->binsdev, 0, sizeof(c->binsdev));
     for (int z = 0; z < b->nz; ++z) c->binsdev.zc[z] = c->zcenters[z];
-    for (int k = 0; k < b->nk; ++k) {
-        c->binsdev.kc[k] = c->kcenters[k];
-        c->binsdev.dk[k] = c->kedges[k + 1] - c->kedges[k];
-    }
+    for (int k = 0; k <= b->nk; ++k) c->binsdev.ke[k] = c->kedges[k];
     c->binsdev.dz = b->z_bin_width;
     c->binsdev.nz = b->nz;
     c->binsdev.nk = b->nk;
@@ -364,10 +364,12 @@ static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm
         ch.rnb = nel;
         ch.rband.assign((size_t)N * nel, 0.0);
         if (rm->kind == QML_RMAT_DIA) {
-            // row i, slot t <-> column i - nel/2 + t on diagonal nel-1-t
+            // row i, slot t <-> column i - nel/2 + t on diagonal nel-1-t; the
+            // grid gets nel/2 points beyond each end so that every slot lies
+            // on it (slots off the matrix keep weight 0)
             ch.ros = 1;
-            ch.rcol0 = -(nel / 2);
-            ch.rM = N;
+            ch.rcol0 = 0;
+            ch.rM = N + nel - 1;
             for (int i = 0; i < N; ++i)
                 for (int t = 0; t < nel; ++t) {
                     const int col = i - nel / 2 + t;
@@ -390,7 +392,14 @@ static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm
         // the grid of the matrix (core/chunk_estimate.cpp:936-944), units of lambda_Lya
         ch.fine.resize(ch.rM);
         if (rm->kind == QML_RMAT_DIA) {
-            for (int i = 0; i < N; ++i) ch.fine[i] = ch.wave[i] / LYA_REST;
+            const int h = ch.rnb / 2;
+            const double w0 = ch.wave[0], w1 = ch.wave[N - 1];
+            const double d0 = (N > 1) ? ch.wave[1] - w0 : 1.0, d1 = (N > 1) ? w1 - ch.wave[N - 2] : 1.0;
+            for (int a = 0; a < ch.rM; ++a) {
+                const int p = a - h;
+                const double w = (p < 0) ? w0 + p * d0 : (p >= N) ? w1 + (p - N + 1) * d1 : ch.wave[p];
+                ch.fine[a] = w / LYA_REST;
+            }
         } else {
             std::vector<double> dl(N - 1);
             for (int i = 0; i < N - 1; ++i) dl[i] = ch.wave[i + 1] - ch.wave[i];
@@ -478,10 +487,10 @@ int qml_add_chunk_rmat(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm, q
     if (!rm || !rm->values || rm->nelem < 1 || rm->nelem % 2 == 0)
         return fail(QML_ERR_ARG, "qml_add_chunk_rmat: resolution matrix needs an odd number of elements per row");
     if (rm->kind == QML_RMAT_DIA) {
-        if (16 + rm->nelem > RM_ROWS) return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: too many diagonals");
+        if (15 + rm->nelem > RM_ROWS) return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: more than 81 diagonals");
     } else if (rm->kind == QML_RMAT_OVERSAMPLED) {
         if (rm->oversampling < 1 || 15 * rm->oversampling + rm->nelem > RM_ROWS)
-            return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: 15 x oversampling + nelem exceeds 128");
+            return fail(QML_ERR_LIMIT, "qml_add_chunk_rmat: 15 x oversampling + nelem exceeds 96");
     } else {
         return fail(QML_ERR_ARG, "qml_add_chunk_rmat: unknown matrix kind");
     }
@@ -578,14 +587,13 @@ int qml_commit(qml_ctx *c) {
     HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
 
     // per-chunk scratch sizes (elements of each arena)
-    struct Need { size_t mat, vec, pj, sj, stk, lic, gram, rt; };
+    struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
     auto need = [&](const ChunkHost &ch) {
         Need n{};
         const size_t Np = ch.Np, m = ch.m, nt = Np / TILE, ntu = nt * (nt + 1) / 2;
         const size_t mt = (m + TILE - 1) / TILE;
         // A, L, Li, S, then for a resolution-matrix chunk its kbu Q_k and S
         n.mat = (4 + (ch.rm ? (size_t)ch.kbu + 1 : 0)) * Np * Np;
-        n.rt = ch.rm ? (size_t)ch.N * ch.rM : 0;
         n.vec = (size_t)VEC_SLOTS * Np;
         n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
         n.sj = m * MAX_NU * MAX_NU + ch.n;
@@ -596,7 +604,7 @@ int qml_commit(qml_ctx *c) {
     };
     auto bytes = [&](const Need &n) {
         // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
-        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram + n.rt) + es * (2 * n.stk + 2 * n.lic);
+        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
     };
     std::vector<int> order(nch);
     std::iota(order.begin(), order.end(), 0);
@@ -668,17 +676,15 @@ int qml_commit(qml_ctx *c) {
             d.rcol0 = ch.rcol0;
             d.rM = ch.rM;
             d.rin_off = ch.rin_off;
-            d.rt_off = (long long)acc.rt;
             d.qb_off = (long long)(acc.mat + 4 * (size_t)ch.Np * ch.Np);
             acc.mat += n.mat; acc.vec += n.vec; acc.pj += n.pj; acc.sj += n.sj;
-            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram; acc.rt += n.rt;
+            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram;
             cd.push_back(d);
         }
         mx.mat = std::max(mx.mat, acc.mat); mx.vec = std::max(mx.vec, acc.vec);
         mx.pj = std::max(mx.pj, acc.pj); mx.sj = std::max(mx.sj, acc.sj);
         mx.stk = std::max(mx.stk, acc.stk); mx.lic = std::max(mx.lic, acc.lic);
         mx.gram = std::max(mx.gram, acc.gram);
-        mx.rt = std::max(mx.rt, acc.rt);
         devs.push_back(cd);
     }
     for (int k = 0; k < 2; ++k) {
@@ -690,7 +696,6 @@ int qml_commit(qml_ctx *c) {
         HIPCHK(hipMemset(c->d_vec[k], 0, std::max<size_t>(mx.vec, 1) * 8));
     }
     HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
-    if (mx.rt) HIPCHK(hipMalloc(&c->d_rt, mx.rt * 8));
     HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
     HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
     // LYSPEQ_QML_POISON=1 fills every scratch arena with all-ones bytes (NaN
@@ -715,6 +720,7 @@ int qml_commit(qml_ctx *c) {
         B.ids = groups[bi];
         const auto &cd = devs[bi];
         std::vector<Work> wl, wr, wg;
+        int ra_max = 0, rb_max = 0, nb_max = 0;
         for (int li = 0; li < (int)cd.size(); ++li) {
             const ChunkDev &d = cd[li];
             B.np_max = std::max(B.np_max, d.Np);
@@ -722,7 +728,9 @@ int qml_commit(qml_ctx *c) {
             B.nu_max = std::max(B.nu_max, d.nu);
             if (d.rm) {
                 B.rm_kbu = std::max(B.rm_kbu, d.kbu);
-                B.rm_M = std::max(B.rm_M, d.rM);
+                ra_max = std::max(ra_max, 15 * d.ros + d.rnb);
+                rb_max = std::max(rb_max, 31 * d.ros + d.rnb);
+                nb_max = std::max(nb_max, d.rnb);
             }
             // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16.
             const int g = (c->prec == 2) ? 2 : 1;
@@ -737,6 +745,15 @@ int qml_commit(qml_ctx *c) {
                 for (int I = 0; I < mtg; ++I)
                     for (int J = I; J < mtg; ++J) wg.push_back({li, s, I, J});
         }
+        if (B.rm_kbu >= 0) {
+            // 16 x 32 output tiles where their LDS fits in 64 KB, their a range
+            // in 80 rows and their b range in 4 passes; else 16 x 16 (which
+            // always fits under the registration limit on 15 ros + rnb)
+            B.rm_lay = rmat_layout(32, ra_max, rb_max, nb_max);
+            if (B.rm_lay.bytes > 65536 || B.rm_lay.em != 10 || B.rm_lay.npass > 4)
+                B.rm_lay = rmat_layout(16, ra_max, ra_max, nb_max);
+            if (!rmat_layout_ok(B.rm_lay)) return fail(QML_ERR_LIMIT, "qml_commit: no sandwich kernel for the resolution matrices");
+        }
         B.nwl = (int)wl.size();
         B.nwr = (int)wr.size();
         B.nwg = (int)wg.size();
@@ -750,6 +767,7 @@ int qml_commit(qml_ctx *c) {
         if (!wg.empty()) HIPCHK(hipMemcpy(B.d_wg, wg.data(), wg.size() * sizeof(Work), hipMemcpyHostToDevice));
         for (auto &e : B.evA) HIPCHK(hipEventCreate(&e));
         for (auto &e : B.evB) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreate(&B.evR));
         c->batches.push_back(B);
     }
     c->hF.assign((size_t)c->T * c->T, 0.0);
@@ -779,9 +797,8 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.nvu_max = B.nvu_max;
         L.nu_max = B.nu_max;
         L.rin = c->d_rin;
-        L.rt = c->d_rt;
         L.rm_kbu = B.rm_kbu;
-        L.rm_M = B.rm_M;
+        L.rm_lay = B.rm_lay;
         L.in = c->d_in;
         L.tables = c->d_tables;
         L.bins = c->binsdev;
@@ -793,7 +810,8 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
         if (c->poison) launch_poison_lds(c->st);
         HIPCHK(hipEventRecord(B.evA[0], c->st));
-        if (B.rm_kbu >= 0) launch_rmat(L);
+        if (!launch_rmat(L)) return fail(QML_ERR_LIMIT, "qml_iterate: no sandwich kernel for the resolution matrices");
+        HIPCHK(hipEventRecord(B.evR, c->st));
         launch_assemble(L, c->d_theta);
         HIPCHK(hipEventRecord(B.evA[1], c->st));
         launch_cholesky(L, c->d_status);
@@ -843,8 +861,13 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
     for (auto &B : c->batches) {
         for (int k = 0; k < 6; ++k) {
             float ms = 0;
-            hipEventElapsedTime(&ms, B.evA[k], B.evA[k + 1]);
+            hipEventElapsedTime(&ms, k == 0 ? B.evR : B.evA[k], B.evA[k + 1]);
             tm.kernel_ms[k] += ms;
+        }
+        {
+            float ms = 0;
+            hipEventElapsedTime(&ms, B.evA[0], B.evR);
+            tm.kernel_ms[QML_K_RMAT] += ms;
         }
         for (int k = 0; k < 5; ++k) {
             float ms = 0;
@@ -858,7 +881,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         tm.kernel_ms[QML_K_ACCUMULATE] = ms;
         tm.accumulate_ms = ms;
     }
-    tm.assemble_ms = tm.kernel_ms[QML_K_ASSEMBLE];
+    tm.assemble_ms = tm.kernel_ms[QML_K_ASSEMBLE] + tm.kernel_ms[QML_K_RMAT];
     tm.invert_ms = tm.kernel_ms[QML_K_CHOLESKY] + tm.kernel_ms[QML_K_TRINV] + tm.kernel_ms[QML_K_SOLVE];
     tm.weight_ms = tm.kernel_ms[QML_K_DPASS] + tm.kernel_ms[QML_K_SMALL] + tm.kernel_ms[QML_K_XGEN] +
                    tm.kernel_ms[QML_K_WHITEN_LEFT] + tm.kernel_ms[QML_K_WHITEN_RIGHT];

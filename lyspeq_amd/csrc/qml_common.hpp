@@ -27,7 +27,7 @@ static_assert(V_G < VEC_SLOTS, "vector slots");
 
 struct BinsDev {
     double zc[MAX_ZB];
-    double kc[MAX_NK], dk[MAX_NK];   // k-band centres and widths (resolution-matrix mode)
+    double ke[MAX_NK + 1];           // k-band edges (resolution-matrix mode)
     double dz;
     int nz, nk, method;
     int turn_off_sfid;
@@ -68,11 +68,10 @@ struct ChunkDev {
     // fine-grid wavelengths (units of lambda_Lya).
     int rm, rnb, ros, rcol0, rM;
     long long rin_off;   // into the resolution-matrix input arena (doubles)
-    long long rt_off;    // into the R Qhat scratch (N x rM doubles)
     long long qb_off;    // into the FP64 matrix arena
 };
 // Fine rows a 16-row tile of R touches: 15 ros + rnb <= RM_ROWS.
-constexpr int RM_ROWS = 128;
+constexpr int RM_ROWS = 96;
 
 struct ChunkOut {        // per chunk id, persistent
     long long fc_off;    // n_q x n_q

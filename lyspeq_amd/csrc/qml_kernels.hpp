@@ -5,6 +5,19 @@
 
 namespace qml {
 
+// LDS carve-up of k_rm_sandwich (doubles): Qh qr x 33, T 16 x tw, R rows
+// 16 x nbm and tj x nbm, grid-point u then (sin, cos) for npts points; em
+// Qh entries per thread (qr = 8 em), npass passes of 32 b columns.
+struct RmLayout {
+    int qr = 0, tw = 0, nbm = 0, npts = 0, tj = 16, em = 12, npass = 0;
+    size_t bytes = 0;
+};
+// Layout for output tiles of 16 x tj and the batch's largest a range
+// (15 ros + rnb), b range ((tj - 1) ros + rnb) and rnb.
+RmLayout rmat_layout(int tj, int ra_max, int rb_max, int nb_max);
+// Whether launch_rmat has a kernel for this layout.
+bool rmat_layout_ok(const RmLayout &lay);
+
 struct LaunchCtx {
     hipStream_t stream;
     const ChunkDev *chunks;   // device, nchunks entries
@@ -21,16 +34,16 @@ struct LaunchCtx {
     double *sj;               // s_j and d arena
     // resolution-matrix mode
     const double *rin;        // R rows and fine grids
-    double *rt;               // R Qhat scratch
     int rm_kbu;               // largest KBIN_UPP of the batch's resolution-matrix chunks; -1 if none
-    int rm_M;                 // largest fine-grid size of those chunks
+    RmLayout rm_lay;          // LDS layout of the batch's sandwich launch
 };
 
 // kernels_chunk.hip
 void launch_assemble(const LaunchCtx &L, const double *theta);
 // kernels_rmat.hip: Q_k = R Qhat_k R^T (k < KBIN_UPP) and S = R Shat R^T for
-// the batch's resolution-matrix chunks, into their qb_off matrices.
-void launch_rmat(const LaunchCtx &L);
+// the batch's resolution-matrix chunks, into their qb_off matrices; false if
+// no kernel matches the batch's layout (nothing is launched).
+bool launch_rmat(const LaunchCtx &L);
 // kernels_factor.hip: L = chol(A) and Li = L^-1 (both formed by
 // launch_cholesky; launch_trinv is kept for the timing slot and is empty)
 void launch_cholesky(const LaunchCtx &L, int *status);

@@ -90,7 +90,7 @@ class Timings(C.Structure):
     _fields_ = [("assemble_ms", C.c_double), ("invert_ms", C.c_double),
                 ("weight_ms", C.c_double), ("fisher_ms", C.c_double),
                 ("accumulate_ms", C.c_double), ("total_ms", C.c_double),
-                ("n_batches", C.c_longlong), ("kernel_ms", C.c_double * 12)]
+                ("n_batches", C.c_longlong), ("kernel_ms", C.c_double * 13)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "kernel_ms"}
@@ -99,7 +99,7 @@ class Timings(C.Structure):
 
 
 KERNEL_NAMES = ["assemble", "cholesky", "trinv", "solve", "dpass", "small", "xgen",
-                "whiten_left", "whiten_right", "gram", "finalize", "accumulate"]
+                "whiten_left", "whiten_right", "gram", "finalize", "accumulate", "rmat"]
 
 
 _lib = None

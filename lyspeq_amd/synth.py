@@ -346,6 +346,11 @@ def rmat_config(seed: int = 3, nchunks: int = 8192, with_tables=True, n_range=(1
         sigma, ivar = _noise_and_mask(rng, N, 0.3, 0.4, 0.05)
         delta = delta + sigma * rng.standard_normal(N)
         sig_l = wave / (R_fwhm * ONE_SIGMA_2_FWHM) * (1 + rng.uniform(-0.15, 0.15, N))
+        # the reader's cutBoundary to the z-bin range, before R is formed
+        keep = (wave >= LYA_REST * (1 + b.z_lower_edge)) & (wave <= LYA_REST * (1 + b.z_upper_edge))
+        wave, delta, ivar, sig_l = wave[keep], delta[keep], ivar[keep], sig_l[keep]
+        if wave.size < n_range[0]:
+            continue
         w.chunks.append(dict(wave=wave, delta=delta, ivar=ivar, R=R_fwhm, dv_table=50.0,
                              rmat=dict(kind=kind, nelem=nelem, oversampling=os_,
                                        values=_gaussian_rows(wave, sig_l, nelem, os_))))

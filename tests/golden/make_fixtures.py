@@ -5,7 +5,7 @@ Run once in a container where the reference is mounted:
 
 Only inputs and expected outputs are taken (the reference's tests/input and
 tests/truth data files, plus the numeric truth arrays that
-tests/cblas_tests.cpp:24-62 holds inline).  Large text matrices are parsed and
+tests/cblas_tests.cpp:24-62,303-530 holds inline).  Large text matrices are parsed and
 stored as compressed float64 arrays with the same values.
 """
 import os
@@ -54,6 +54,14 @@ def main(ref):
              truth_lu_inversion_A=inline_array(src, "truth_lu_inversion_A"),
              matrix_for_SVD_A=inline_array(src, "matrix_for_SVD_A"),
              truth_SVD_A=inline_array(src, "truth_SVD_A"))
+    # resolution-matrix storage (tests/cblas_tests.cpp:303-530)
+    np.savez(os.path.join(HERE, "resolution_inline.npz"),
+             **{name: inline_array(src, name) for name in (
+                 "oversamp_values", "oversample_multiplier_A",
+                 "truth_oversample_left_multiplication", "truth_oversample_right_multiplication",
+                 "diamatrix_diagonals", "diamatrix_rows", "diamatrix_multiplier_B",
+                 "truth_diamatrix_LN_multiplication", "truth_diamatrix_LT_multiplication",
+                 "truth_diamatrix_RN_multiplication", "truth_diamatrix_RT_multiplication")})
 
 
 if __name__ == "__main__":

@@ -267,3 +267,30 @@ def test_rmat_sandwich_is_quadratic_in_r_and_samples_with_a_delta_row():
     chi, _ = _rmat_chunk("dia", 1)
     for kn in (3, 30):
         np.testing.assert_allclose(chs.q_base(v, kn), chi.q_base(v, kn), rtol=1e-7, atol=1e-12)
+
+
+def test_rmat_storage_against_reference_matrix_products(golden_dir):
+    """The oracle's reading of both resolution-matrix storages against the
+    products the reference's own matrix tests hold (tests/cblas_tests.cpp:
+    303-427 OversampledMatrix multiplyLeft / sandwich, 429-530 DiaMatrix
+    multiply in the four side/transpose combinations)."""
+    from oracle.chunk import ResoMatrix
+    d = np.load(f"{golden_dir}/resolution_inline.npz")
+    R = ResoMatrix("dia", d["diamatrix_diagonals"].reshape(5, 7), 5).dense(7)
+    B = d["diamatrix_multiplier_B"].reshape(7, 7)
+    for name, got in (("LN", R @ B), ("LT", R.T @ B), ("RN", B @ R), ("RT", B @ R.T)):
+        np.testing.assert_allclose(got, d[f"truth_diamatrix_{name}_multiplication"].reshape(7, 7),
+                                   rtol=0, atol=1e-12)
+    # the row view: row i holds R[i][i - 2 .. i + 2]; entries outside the matrix are not used
+    rows = d["diamatrix_rows"].reshape(7, 5)
+    for i in range(7):
+        for t in range(5):
+            if 0 <= i - 2 + t < 7:
+                assert R[i, i - 2 + t] == rows[i, t]
+    Ro = ResoMatrix("oversampled", d["oversamp_values"].reshape(7, 5), 5, 2).dense(7)
+    A = d["oversample_multiplier_A"].reshape(18, 18)
+    assert Ro.shape == (7, 18)
+    # the truths are printed to 6 significant digits
+    np.testing.assert_allclose(Ro @ A, d["truth_oversample_left_multiplication"].reshape(7, 18), rtol=1e-5)
+    np.testing.assert_allclose(Ro @ A @ Ro.T, d["truth_oversample_right_multiplication"].reshape(7, 7),
+                               rtol=1e-5)

@@ -191,6 +191,28 @@ int qml_commit(qml_ctx *ctx);
 int qml_add_chunk_rmat(qml_ctx *ctx, const qml_chunk_in *chunk, const qml_rmat *rmat, qml_chunk_info *info,
                        int *chunk_id);
 
+/* Read-time preprocessing of a whole spectrum's resolution matrix, before it
+ * is cut into chunks (core/one_qso_estimate.cpp:74-89).  Host code; no
+ * context or device is needed.  values is the diagonal storage (ndiags x
+ * size, as QML_RMAT_DIA); a row running off the matrix is completed with its
+ * own values mirrored about the diagonal (DiaMatrix::getRow,
+ * mathtools/matrix_helper.cpp:608-638).
+ *
+ * qml_rmat_deconvolve: DiaMatrix::deconvolve(m) (mathtools/matrix_helper.cpp:
+ * 653-682): each row, zero-padded by 5 on both sides, has its Fourier modes
+ * j >= 1 divided by sinc(pi m j / (ndiags + 10)) (RealField::deconvolveSinc,
+ * mathtools/real_field.cpp:95-111); values is updated in place.  m > 0.
+ *
+ * qml_rmat_oversample: Resolution::oversample(oversampling) (mathtools/
+ * matrix_helper.cpp:1029-1073): each row, shifted by (its minimum - its
+ * smallest nonzero magnitude), is interpolated in log by a natural cubic
+ * spline at oversampling points per pixel, shifted back and normalised to
+ * unit sum.  rows receives size x (2 (ndiags/2) oversampling + 1) values, the
+ * QML_RMAT_OVERSAMPLED storage of the whole spectrum; a chunk's rows are the
+ * slice for its pixels. */
+int qml_rmat_deconvolve(int size, int ndiags, double m, double *values);
+int qml_rmat_oversample(int size, int ndiags, const double *values, int oversampling, double *rows);
+
 /* One Newton-Raphson iteration's chunk loop.  theta is TOTAL_KZ_BINS values
  * (the ps_estimate argument of Chunk::oneQSOiteration).  Adds every chunk's
  * unhalved Fisher matrix into the upper triangle of fisher_sum (T x T,

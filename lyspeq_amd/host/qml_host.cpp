@@ -55,12 +55,31 @@ int Engine::addTable(int R_fwhm, double dv_kms, int nv, double v_length, int nz,
     return r;
 }
 
-int Engine::addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info) {
+int Engine::addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info, const ResolutionRows *rr) {
     if (committed_) throw QMLError(QML_ERR_ARG, "Chunk: the engine has already run; construct chunks first");
     if (i1 < 0 || i2 > q.size() || i2 <= i1) throw std::out_of_range("Chunk: pixel range outside the spectrum");
     qml_chunk_in in{q.wave.data() + i1, q.delta.data() + i1, q.ivar.data() + i1, i2 - i1, q.R_fwhm};
     int id = -1;
-    const int rc = qml_add_chunk(ctx_, &in, info, &id);
+    int rc;
+    if (rr && rr->kind != 0) {
+        // Resolution(rmaster, i1, i2) (mathtools/matrix_helper.cpp:936-970):
+        // columns [i1, i2) of each diagonal, or rows [i1, i2)
+        const int n = i2 - i1, size = q.size();
+        std::vector<double> part;
+        const double *vals;
+        if (rr->kind == QML_RMAT_DIA) {
+            part.resize((size_t)rr->nelem * n);
+            for (int d = 0; d < rr->nelem; ++d)
+                std::copy_n(rr->values.data() + (size_t)d * size + i1, n, part.data() + (size_t)d * n);
+            vals = part.data();
+        } else {
+            vals = rr->values.data() + (size_t)i1 * rr->nelem;
+        }
+        const qml_rmat rm{rr->kind, rr->nelem, rr->oversampling, vals};
+        rc = qml_add_chunk_rmat(ctx_, &in, &rm, info, &id);
+    } else {
+        rc = qml_add_chunk(ctx_, &in, info, &id);
+    }
     if (rc != QML_OK) {
         // the reference constructor throws; OneQSOEstimate skips the chunk
         throw std::runtime_error(std::string("Chunk::Chunk::") + qml_last_error());
@@ -132,8 +151,13 @@ void Engine::flush() {
 void Engine::timings(qml_timings *t) { check(qml_get_timings(ctx_, t)); }
 
 // ------------------------------------------------------------------- Chunk --
-Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2) : eng_(&eng) {
-    id_ = eng.addChunk(q, i1, i2, &info_);
+Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2, const ResolutionRows *rr) : eng_(&eng) {
+    ResolutionRows own;
+    if (!rr && q.rmat_ndiags > 0) {
+        own = OneQSOEstimate::prepareResolution(q, eng.binning());
+        rr = &own;
+    }
+    id_ = eng.addChunk(q, i1, i2, &info_, rr);
     if ((int)eng.chunks_.size() <= id_) eng.chunks_.resize(id_ + 1, nullptr);
     eng.chunks_[id_] = this;
     fisher_index_start = info_.fisher_index_start;
@@ -185,13 +209,46 @@ std::vector<int> OneQSOEstimate::decideIndices(int size, const double *wave, con
     return indices;
 }
 
+ResolutionRows OneQSOEstimate::prepareResolution(const QSOSpectrum &q, const Binning &bins) {
+    ResolutionRows rr;
+    if (q.rmat_ndiags <= 0) return rr;
+    const int nd = q.rmat_ndiags, size = q.size();
+    if ((int)q.rmat.size() != nd * size) throw std::runtime_error("resolution matrix size does not match the spectrum");
+    rr.kind = QML_RMAT_DIA;
+    rr.nelem = nd;
+    rr.values = q.rmat;
+    if (bins.resomat_deconvolution_m > 0 &&
+        qml_rmat_deconvolve(size, nd, bins.resomat_deconvolution_m, rr.values.data()) != QML_OK)
+        throw std::runtime_error("resolution matrix cannot be deconvolved");
+    if (bins.oversampling_factor > 0) {
+        const int os = bins.oversampling_factor, nel = 2 * (nd / 2) * os + 1;
+        std::vector<double> rows((size_t)size * nel);
+        if (qml_rmat_oversample(size, nd, rr.values.data(), os, rows.data()) != QML_OK)
+            throw std::runtime_error("resolution matrix cannot be oversampled");
+        rr.kind = QML_RMAT_OVERSAMPLED;
+        rr.nelem = nel;
+        rr.oversampling = os;
+        rr.values.swap(rows);
+    }
+    return rr;
+}
+
 OneQSOEstimate::OneQSOEstimate(Engine &eng, const QSOSpectrum &q) {
+    // _readQsoFile's preprocessing of the matrix; a failure skips the spectrum
+    // (core/one_qso_estimate.cpp:74-89,127-130)
+    ResolutionRows rr;
+    try {
+        rr = prepareResolution(q, eng.binning());
+    } catch (std::exception &e) {
+        std::fprintf(stderr, "%s in %s.\n", e.what(), q.fname.c_str());
+        return;
+    }
     const std::vector<int> idx = decideIndices(q.size(), q.wave.data(), eng.binning());
     const int n = (int)idx.size() - 1;
     chunks.reserve(n);
     for (int nc = 0; nc < n; ++nc) {
         try {
-            chunks.push_back(std::make_unique<Chunk>(eng, q, idx[nc], idx[nc + 1]));
+            chunks.push_back(std::make_unique<Chunk>(eng, q, idx[nc], idx[nc + 1], rr.kind ? &rr : nullptr));
         } catch (QMLError &) {
             throw;   // engine failures are not chunk rejections
         } catch (std::exception &e) {

@@ -49,7 +49,21 @@ struct QSOSpectrum {
     std::vector<double> wave, delta, ivar;
     int R_fwhm = 0;
     std::string fname;
+    // resolution matrix as read from the file (qio::QSOFile::Rmat before
+    // preprocessing), diagonal storage rmat_ndiags x size; empty without one
+    std::vector<double> rmat;
+    int rmat_ndiags = 0;
     int size() const { return (int)wave.size(); }
+};
+
+// A whole spectrum's resolution matrix after the reader's preprocessing
+// (core/one_qso_estimate.cpp:74-89): QML_RMAT_DIA (nelem x size) or
+// QML_RMAT_OVERSAMPLED (size x nelem rows).
+struct ResolutionRows {
+    int kind = 0;
+    int nelem = 0;
+    int oversampling = 1;
+    std::vector<double> values;
 };
 
 // Binning and the specifics the chunk step depends on (bins:: and
@@ -64,6 +78,8 @@ struct Binning {
     double max_forest_length_v = 3.5e4;  // MaximumForestLength (before division)
     int number_of_chunks = 1;            // DynamicChunkNumber
     double max_pixel_length_v = 120.0;   // MaximumMeanPixelSize
+    int oversampling_factor = -1;        // OversampleRmat: > 0 oversamples diagonal matrices
+    double resomat_deconvolution_m = -1; // ResoMatrixDeconvolutionM: > 0 deconvolves them first
     int nk() const { return (int)kband_edges.size() - 1; }
     int nz() const { return (int)zbin_centers.size(); }
     int totalKzBins() const { return nk() * nz(); }
@@ -99,7 +115,7 @@ public:
 
 private:
     friend class Chunk;
-    int addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info);
+    int addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info, const ResolutionRows *rr);
     void enqueue(Chunk *c, const double *ps, std::vector<std::unique_ptr<double[]>> *dbt, double *fisher);
     void forget(Chunk *c);
     void check(int code) const;
@@ -117,8 +133,10 @@ private:
 
 class Chunk {
 public:
-    // Chunk(const qio::QSOFile&, int i1, int i2): pixels [i1, i2) of q.
-    Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2);
+    // Chunk(const qio::QSOFile&, int i1, int i2): pixels [i1, i2) of q.  A
+    // spectrum with a resolution matrix uses rr, its preprocessed matrix
+    // (prepareResolution); without rr the chunk preprocesses q's itself.
+    Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2, const ResolutionRows *rr = nullptr);
     ~Chunk();
     Chunk(const Chunk &) = delete;
     Chunk(Chunk &&) = delete;
@@ -157,6 +175,12 @@ public:
     OneQSOEstimate(Engine &eng, const QSOSpectrum &q);
 
     static std::vector<int> decideIndices(int size, const double *wave, const Binning &bins);
+
+    // The reader's preprocessing of q's resolution matrix under the bins'
+    // specifics: deconvolution if resomat_deconvolution_m > 0, then
+    // oversampling if oversampling_factor > 0 (qml_rmat_deconvolve,
+    // qml_rmat_oversample).  kind 0 if q has no matrix.
+    static ResolutionRows prepareResolution(const QSOSpectrum &q, const Binning &bins);
 
     std::vector<std::unique_ptr<Chunk>> chunks;
 

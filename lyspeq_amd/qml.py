@@ -107,7 +107,7 @@ _lib = None
 EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
            "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
-           "qml_total_kz_bins"]
+           "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample"]
 
 
 def load_library(path: str = LIB_PATH):
@@ -127,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_add_table.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp, dp,
                                   C.POINTER(C.c_int)]
     lib.qml_add_chunk.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(ChunkInfo), C.POINTER(C.c_int)]
+    lib.qml_rmat_deconvolve.argtypes = [C.c_int, C.c_int, C.c_double, dp]
+    lib.qml_rmat_oversample.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp]
     lib.qml_add_chunk_rmat.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(_RMat), C.POINTER(ChunkInfo),
                                        C.POINTER(C.c_int)]
     lib.qml_commit.argtypes = [vp]
@@ -149,6 +151,31 @@ def _check(code: int, lib):
     if code != QML_OK:
         msg = lib.qml_last_error().decode()
         raise _ERRORS.get(code, QMLError)(code, msg)
+
+
+def rmat_deconvolve(values, m: float) -> np.ndarray:
+    """qml_rmat_deconvolve (DiaMatrix::deconvolve) on a copy of a whole
+    spectrum's diagonal storage, ndiags x size."""
+    lib = load_library()
+    v = np.array(values, dtype=np.float64, order="C")
+    nd, size = v.shape
+    code = lib.qml_rmat_deconvolve(size, nd, float(m), _dptr(v))
+    if code != QML_OK:
+        raise QMLError(code, "qml_rmat_deconvolve: invalid arguments")
+    return v
+
+
+def rmat_oversample(values, oversampling: int) -> np.ndarray:
+    """qml_rmat_oversample (Resolution::oversample): a whole spectrum's
+    diagonal storage (ndiags x size) to oversampled rows, size x nelem."""
+    lib = load_library()
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    nd, size = v.shape
+    out = np.empty((size, 2 * (nd // 2) * int(oversampling) + 1))
+    code = lib.qml_rmat_oversample(size, nd, _dptr(v), int(oversampling), _dptr(out))
+    if code != QML_OK:
+        raise QMLError(code, "qml_rmat_oversample: invalid arguments")
+    return out
 
 
 @dataclass

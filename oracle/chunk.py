@@ -94,6 +94,60 @@ class ResoMatrix:
             return ResoMatrix("dia", self.values[:, i1:i2].copy(), self.nelem)
         return ResoMatrix("oversampled", self.values[i1:i2].copy(), self.nelem, self.oversampling)
 
+    def dia_row(self, i: int) -> np.ndarray:
+        """DiaMatrix::getRow (mathtools/matrix_helper.cpp:608-638): the nelem
+        values of row i centred on the diagonal; slots off the matrix take
+        the row's own value mirrored about the centre."""
+        n, h = self.values.shape[1], self.nelem // 2
+        row = np.empty(self.nelem)
+        for p in range(self.nelem):
+            q = p if 0 <= i - h + p < n else self.nelem - 1 - p
+            row[p] = self.values[self.nelem - 1 - q, i - h + q]
+        return row
+
+    def deconvolve(self, m: float) -> "ResoMatrix":
+        """DiaMatrix::deconvolve (mathtools/matrix_helper.cpp:653-682) with
+        RealField::deconvolveSinc (mathtools/real_field.cpp:80-111): each row,
+        zero-padded by 5 on both sides, divided by sinc(pi m j / n) in Fourier
+        mode j >= 1 (n = nelem + 10; the FFT pair is normalised to identity)."""
+        assert self.kind == "dia"
+        vals = self.values.copy()
+        n, h, pad = vals.shape[1], self.nelem // 2, 5
+        nf = self.nelem + 2 * pad
+        j = np.arange(1, nf // 2 + 1)
+        a = MY_PI * m * j / nf
+        for i in range(n):
+            x = np.zeros(nf)
+            x[pad:pad + self.nelem] = self.dia_row(i)
+            X = np.fft.rfft(x)
+            X[1:] /= np.sin(a) / a
+            y = np.fft.irfft(X, nf)
+            for p in range(self.nelem):
+                if 0 <= i - h + p < n:
+                    vals[self.nelem - 1 - p, i - h + p] = y[pad + p]
+        return ResoMatrix("dia", vals, self.nelem)
+
+    def oversample(self, osamp: int, dlambda: float = 1.0) -> "ResoMatrix":
+        """Resolution::oversample (mathtools/matrix_helper.cpp:1029-1073):
+        per row, shift = min(row) - (smallest magnitude above 1e-15), natural
+        cubic spline of log(row - shift) on (i - noff) dlambda, evaluated at
+        (jj / osamp - noff) dlambda, shift + exp, normalised to unit sum."""
+        assert self.kind == "dia"
+        n, noff = self.values.shape[1], self.nelem // 2
+        nel = 2 * noff * osamp + 1
+        xin = (np.arange(self.nelem) - noff) * dlambda
+        xout = (np.arange(nel) / osamp - noff) * dlambda
+        rows = np.empty((n, nel))
+        for i in range(n):
+            row = self.dia_row(i)
+            mags = np.abs(row)
+            nz = mags[mags > 1e-15]
+            shift = row.min() - (nz.min() if nz.size else 1e-15)
+            y = np.log(row - shift)
+            out = shift + np.exp(natural_cubic_spline(xin, y, xout))
+            rows[i] = out * (1.0 / out.sum())
+        return ResoMatrix("oversampled", rows, nel, osamp)
+
     def dense(self, n: int) -> np.ndarray:
         """R as an n x ncols array."""
         if self.kind == "dia":
@@ -111,6 +165,23 @@ class ResoMatrix:
         for i in range(n):
             r[i, i * self.oversampling:i * self.oversampling + self.nelem] = self.values[i]
         return r
+
+
+def natural_cubic_spline(x: np.ndarray, y: np.ndarray, xe: np.ndarray) -> np.ndarray:
+    """Natural cubic spline through (x, y) on a uniform grid, evaluated at xe
+    (gsl_interp_cspline, used by Resolution::oversample): second derivatives
+    from the tridiagonal system with zero end values."""
+    n = x.size
+    h = x[1] - x[0]
+    m2 = np.zeros(n)
+    if n > 2:
+        A = np.diag(np.full(n - 2, 4.0)) + np.diag(np.ones(n - 3), 1) + np.diag(np.ones(n - 3), -1)
+        r = 6.0 * (y[2:] - 2.0 * y[1:-1] + y[:-2]) / (h * h)
+        m2[1:-1] = np.linalg.solve(A, r)
+    k = np.clip(np.floor((xe - x[0]) / h).astype(int), 0, n - 2)
+    t = (xe - x[k]) / h
+    u = 1.0 - t
+    return u * y[k] + t * y[k + 1] + h * h / 6.0 * ((u ** 3 - u) * m2[k] + (t ** 3 - t) * m2[k + 1])
 
 
 def median_sorted(a: np.ndarray) -> float:

@@ -8,10 +8,12 @@
 // int32 nk, nz; f64 kband_edges[nk+1], zbin_centers[nz], z_bin_width;
 // int32 method, logl_order, lam_order, turn_off_sfid; f64
 // max_forest_length_v; int32 number_of_chunks; f64 max_pixel_length_v;
+// int32 oversampling_factor; f64 resomat_deconvolution_m;
 // int32 precision, iterations; int32 ntables, then per table int32 R, f64
 // dv, int32 nv, f64 v_length, int32 nz, f64 deriv[nk*nv], f64 signal[nv*nz];
 // int32 nspec, then per spectrum int32 R, int32 n, f64 wave[n], delta[n],
-// ivar[n]; f64 theta[nk*nz].
+// ivar[n], int32 ndiags, f64 rmat[ndiags*n] (diagonal storage; none if
+// ndiags is 0); f64 theta[nk*nz].
 //
 // Output file: int32 T, f64 fisher_sum[T*T], f64 dbt_sum[3*T] (the sums of
 // the last iteration), int32 nchunks, then per chunk (in construction order)
@@ -65,6 +67,8 @@ int main(int argc, char **argv) {
         b.max_forest_length_v = r.get<double>();
         b.number_of_chunks = r.get<int32_t>();
         b.max_pixel_length_v = r.get<double>();
+        b.oversampling_factor = r.get<int32_t>();
+        b.resomat_deconvolution_m = r.get<double>();
         const int prec = r.get<int32_t>(), iters = r.get<int32_t>();
         Engine eng(b, prec);
         const int ntab = r.get<int32_t>();
@@ -86,6 +90,8 @@ int main(int argc, char **argv) {
             q.wave = r.vec(n);
             q.delta = r.vec(n);
             q.ivar = r.vec(n);
+            q.rmat_ndiags = r.get<int32_t>();
+            q.rmat = r.vec((size_t)q.rmat_ndiags * n);
             q.fname = "spectrum-" + std::to_string(s);
             qsos.push_back(std::make_unique<OneQSOEstimate>(eng, q));
         }

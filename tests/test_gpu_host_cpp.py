@@ -25,7 +25,8 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-10
 
 
-def _write_workload(path, w, theta, precision, iterations=1):
+def _write_workload(path, w, theta, precision, iterations=1, prep=None):
+    prep = prep or {}
     b = w.binning
     edges, zc = b.kband_edges(), b.zbin_centers()
     nk, nz = len(edges) - 1, len(zc)
@@ -36,6 +37,7 @@ def _write_workload(path, w, theta, precision, iterations=1):
         f.write(struct.pack("<d", b.z_bin_width))
         f.write(struct.pack("<iiii", b.method, b.cont_logl_order, b.cont_lam_order, int(bool(b.turn_off_sfid))))
         f.write(struct.pack("<did", b.max_forest_length_v, b.number_of_chunks, b.max_pixel_length_v))
+        f.write(struct.pack("<id", int(prep.get("oversampling", -1)), float(prep.get("deconvolution_m", -1.0))))
         f.write(struct.pack("<ii", PRECISIONS[precision], iterations))
         f.write(struct.pack("<i", len(w.pairs)))
         for i, (R, dv) in enumerate(w.pairs):
@@ -51,6 +53,12 @@ def _write_workload(path, w, theta, precision, iterations=1):
             f.write(struct.pack("<ii", int(c["R"]), n))
             for key in ("wave", "delta", "ivar"):
                 f.write(np.ascontiguousarray(c[key], "<f8").tobytes())
+            if "rmat" in c:
+                assert c["rmat"]["kind"] == "dia"
+                f.write(struct.pack("<i", int(c["rmat"]["nelem"])))
+                f.write(np.ascontiguousarray(c["rmat"]["values"], "<f8").tobytes())
+            else:
+                f.write(struct.pack("<i", 0))
         f.write(np.ascontiguousarray(theta, "<f8").tobytes())
 
 
@@ -77,11 +85,11 @@ def _read_output(path):
     return F, dbt, chunks
 
 
-def _run_driver(tmp_path, w, theta, precision, iterations=1):
+def _run_driver(tmp_path, w, theta, precision, iterations=1, prep=None):
     if not os.path.exists(HOST_DRIVER):
         pytest.fail(f"{HOST_DRIVER} is missing: build it with `python -m lyspeq_amd.build`")
     wl, out = tmp_path / "workload.bin", tmp_path / "out.bin"
-    _write_workload(wl, w, theta, precision, iterations)
+    _write_workload(wl, w, theta, precision, iterations, prep)
     r = subprocess.run([HOST_DRIVER, str(wl), str(out)], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
@@ -175,6 +183,64 @@ def test_cpp_one_qso_estimate_splits_spectra_like_decide_indices(tmp_path):
     assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
     Fo, do, n = _oracle(w, th)
     assert n == len(chunks)
+    assert P.rel_err(F, Fo) < FP64_TOL
+    for k in range(3):
+        assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+
+
+def test_cpp_resolution_matrix_spectra_preprocessed_like_the_reader(tmp_path):
+    """Spectra carrying an 11-diagonal matrix in diagonal storage, with
+    ResoMatrixDeconvolutionM = 1 and OversampleRmat = 3: the C++ host layer
+    preprocesses each spectrum (OneQSOEstimate::prepareResolution) and
+    registers the chunks' row slices; the Python binding does the same through
+    qml_rmat_deconvolve/qml_rmat_oversample; the oracle preprocesses its own
+    copy (core/one_qso_estimate.cpp:74-89)."""
+    from lyspeq_amd.qml import QMLEngine, QMLError, rmat_deconvolve, rmat_oversample
+    from oracle.chunk import ResoMatrix
+    print("[case] C++ host layer, resolution matrices deconvolved and oversampled", flush=True)
+    w = synth.rmat_config(nchunks=8, n_range=(110, 180), kind="dia")
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 41)
+    F, dbt, chunks = _run_driver(tmp_path, w, th, "fp64", prep={"oversampling": 3, "deconvolution_m": 1.0})
+    # the same through the Python binding
+    eng = QMLEngine(w.binning, precision="fp64")
+    for i, (R, dv) in enumerate(w.pairs):
+        eng.add_table(R, dv, w.deriv[i], w.signal[i], w.v_length)
+    b = P.oracle_bins(w.binning)
+    tabs = P.oracle_tables(w)
+    T = b.total_kz_bins
+    Fo, do = np.zeros((T, T)), np.zeros((3, T))
+    ids, n_oracle = [], 0
+    for s, c in enumerate(w.chunks):
+        dia = np.asarray(c["rmat"]["values"])
+        rows = rmat_oversample(rmat_deconvolve(dia, 1.0), 3)
+        orows = ResoMatrix("dia", dia, 11).deconvolve(1.0).oversample(3).values
+        idx = decide_indices(np.asarray(c["wave"]), b)
+        for k in range(len(idx) - 1):
+            i1, i2 = idx[k], idx[k + 1]
+            try:
+                cid, _ = eng.add_chunk_rmat(c["wave"][i1:i2], c["delta"][i1:i2], c["ivar"][i1:i2], c["R"],
+                                            "oversampled", rows[i1:i2], 31, 3)
+                ids.append(cid)
+            except QMLError as e:
+                if e.code not in (-3, -4, -5):
+                    raise
+            try:
+                ch = OracleChunk(b, tabs, c["wave"][i1:i2], c["delta"][i1:i2], c["ivar"][i1:i2], c["R"],
+                                 rmat=ResoMatrix("oversampled", orows[i1:i2], 31, 3))
+            except ChunkError:
+                continue
+            ch.one_iteration(th, Fo, do)
+            n_oracle += 1
+    eng.commit()
+    Fp, dp = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(th, Fp, dp)
+    res = [eng.chunk_result(cid) for cid in ids]
+    eng.close()
+    assert len(chunks) == len(res) == n_oracle >= 6
+    assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
+    for (s, fis, nq, st, fc, d), (fr, dr, sr) in zip(chunks, res):
+        assert st == sr == 0 and np.array_equal(fc, fr) and np.array_equal(d, dr)
+    print(f"preprocessed matrices, C++ layer vs oracle: F {P.rel_err(F, Fo):.2e}", flush=True)
     assert P.rel_err(F, Fo) < FP64_TOL
     for k in range(3):
         assert P.rel_err(dbt[k], do[k]) < FP64_TOL

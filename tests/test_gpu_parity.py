@@ -212,6 +212,36 @@ def test_resolution_matrix_chunks_fp64_match_oracle(kind):
         assert P.rel_err(dbt[k], od[k]) < FP64_TOL
 
 
+def test_resolution_matrix_read_time_preprocessing_fp64_matches_oracle():
+    """Diagonal-storage matrices deconvolved (m = 1) and oversampled x3 at
+    read time, then registered as oversampled rows (core/one_qso_estimate.cpp:
+    74-89): the engine takes rows from qml_rmat_deconvolve/qml_rmat_oversample,
+    the oracle preprocesses its own copy."""
+    import copy
+    from lyspeq_amd.qml import rmat_deconvolve, rmat_oversample
+    _print_case("resolution matrix deconvolved + oversampled at read time, fp64 vs oracle")
+    w = synth.rmat_config(nchunks=6, n_range=(100, 170), kind="dia")
+    wo = copy.copy(w)
+    wo.chunks = [dict(c) for c in w.chunks]
+    for c, co in zip(w.chunks, wo.chunks):
+        dia = np.asarray(c["rmat"]["values"])
+        c["rmat"] = dict(kind="oversampled", nelem=31, oversampling=3,
+                         values=rmat_oversample(rmat_deconvolve(dia, 1.0), 3))
+        ro = P.oracle_rmat(co).deconvolve(1.0).oversample(3)
+        co["rmat"] = dict(kind="oversampled", nelem=31, oversampling=3, values=ro.values)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 37)
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, th, idx)
+    eng.close()
+    oout, oF, od = P.run_oracle(wo, th, idx)
+    assert set(out) == set(oout) and len(out) >= 5
+    worst = max(max(P.rel_err(out[i][0], oout[i][0]),
+                    *(P.rel_err(out[i][1][k], oout[i][1][k]) for k in range(3))) for i in out)
+    print(f"preprocessed resolution matrix: worst per-chunk relative error {worst:.2e}", flush=True)
+    assert worst < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+
+
 def test_resolution_matrix_mixed_precision_against_fp64():
     _print_case("resolution matrix mixed vs fp64")
     w = synth.rmat_config(nchunks=12, n_range=(100, 400))

@@ -294,3 +294,98 @@ def test_rmat_storage_against_reference_matrix_products(golden_dir):
     np.testing.assert_allclose(Ro @ A, d["truth_oversample_left_multiplication"].reshape(7, 18), rtol=1e-5)
     np.testing.assert_allclose(Ro @ A @ Ro.T, d["truth_oversample_right_multiplication"].reshape(7, 7),
                                rtol=1e-5)
+
+
+def test_rmat_rows_with_mirrored_edges_against_reference_rows(golden_dir):
+    """DiaMatrix::getRow, edge slots included: rows 0-1 and 5-6 of the
+    reference's 7x7, 5-diagonal matrix run off it and are completed by
+    mirroring (tests/cblas_tests.cpp:431-447)."""
+    from oracle.chunk import ResoMatrix
+    d = np.load(f"{golden_dir}/resolution_inline.npz")
+    R = ResoMatrix("dia", d["diamatrix_diagonals"].reshape(5, 7), 5)
+    rows = np.array([R.dia_row(i) for i in range(7)])
+    assert np.array_equal(rows, d["diamatrix_rows"].reshape(7, 5))
+
+
+def test_rmat_oversampling_spline_and_invariants():
+    """Resolution::oversample: the spline is scipy's natural cubic spline;
+    at the original points (every osamp-th output) the shifted log
+    interpolation returns the row itself, so ratios there are the row's; rows
+    sum to one; osamp = 1 only normalises."""
+    from scipy.interpolate import CubicSpline
+    from oracle.chunk import ResoMatrix, natural_cubic_spline
+    rng = np.random.default_rng(5)
+    x = np.arange(11) - 5.0
+    for _ in range(3):
+        y = rng.normal(size=11)
+        xe = np.linspace(-5, 5, 61)
+        np.testing.assert_allclose(natural_cubic_spline(x, y, xe), CubicSpline(x, y, bc_type="natural")(xe),
+                                   rtol=0, atol=1e-14)
+    n = 40
+    sig = rng.uniform(0.8, 1.6, n)
+    offs = np.arange(11) - 5
+    dense_rows = np.exp(-0.5 * (offs[None, :] / sig[:, None]) ** 2)
+    dense_rows[:, 0] -= 2e-3                     # a negative tail value exercises the shift
+    vals = np.zeros((11, n))
+    for i in range(n):
+        for p in range(11):
+            c = i - 5 + p
+            if 0 <= c < n:
+                vals[10 - p, c] = dense_rows[i, p]
+    R = ResoMatrix("dia", vals, 11)
+    O = R.oversample(3, dlambda=0.8)
+    assert O.values.shape == (n, 31) and O.nelem == 31 and O.oversampling == 3
+    np.testing.assert_allclose(O.values.sum(axis=1), 1.0, rtol=1e-14)
+    for i in (0, 7, n - 1):
+        row = R.dia_row(i)
+        knots = O.values[i, ::3]
+        np.testing.assert_allclose(knots / knots[5], row / row[5], rtol=1e-12)
+    O1 = R.oversample(1)
+    for i in (3, 20):
+        np.testing.assert_allclose(O1.values[i], R.dia_row(i) / R.dia_row(i).sum(), rtol=1e-13)
+    # the grid spacing does not change the interpolant
+    np.testing.assert_allclose(R.oversample(3, dlambda=0.3).values, O.values, rtol=1e-12)
+
+
+def test_rmat_deconvolution_against_a_direct_dft():
+    """DiaMatrix::deconvolve on a delta row: the interior row becomes the
+    inverse DFT of 1 / sinc(pi m j / n) over the 21-point padded row, summed
+    here mode by mode over j = -10..10 with complex exponentials; m -> 0
+    leaves the matrix unchanged."""
+    from oracle.chunk import ResoMatrix
+    n, nel, pad = 30, 11, 5
+    vals = np.zeros((nel, n))
+    vals[5, :] = 1.0                            # the identity: delta rows
+    for m in (1.0, 1.7):
+        D = ResoMatrix("dia", vals, nel).deconvolve(m)
+        nf = nel + 2 * pad
+        t = np.arange(nf)
+        x = np.zeros(nf)
+        x[pad + 5] = 1.0
+        want = np.zeros(nf, complex)
+        for j in range(-(nf // 2), nf // 2 + 1):
+            Xj = np.sum(x * np.exp(-2j * np.pi * j * t / nf))
+            s = 1.0 if j == 0 else np.sin(np.pi * m * abs(j) / nf) / (np.pi * m * abs(j) / nf)
+            want += Xj / s * np.exp(2j * np.pi * j * t / nf)
+        want = (want / nf).real
+        got = D.dia_row(15)
+        np.testing.assert_allclose(got, want[pad:pad + nel], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(ResoMatrix("dia", vals, nel).deconvolve(1e-6).values, vals, atol=1e-11)
+
+
+def test_rmat_preprocessing_through_the_c_abi_matches_the_oracle():
+    """qml_rmat_deconvolve and qml_rmat_oversample (host code behind the C
+    ABI; no device is used) against the oracle on an 11-diagonal Gaussian
+    matrix with jittered widths, edges included."""
+    from lyspeq_amd.qml import rmat_deconvolve, rmat_oversample
+    from oracle.chunk import ResoMatrix
+    from lyspeq_amd import synth
+    w = synth.rmat_config(nchunks=1, n_range=(180, 180), kind="dia", with_tables=False)
+    vals = np.asarray(w.chunks[0]["rmat"]["values"])
+    ro = ResoMatrix("dia", vals, 11)
+    dec = rmat_deconvolve(vals, 1.0)
+    np.testing.assert_allclose(dec, ro.deconvolve(1.0).values, rtol=0, atol=1e-14)
+    for osamp in (1, 3, 4):
+        got = rmat_oversample(dec, osamp)
+        want = ResoMatrix("dia", dec, 11).oversample(osamp).values
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-14)

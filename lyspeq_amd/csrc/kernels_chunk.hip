@@ -470,6 +470,10 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
 }
 
 // ------------------------------------------------------- stack generation --
+// A double in the global address space (loads through it are global_load,
+// counted by vmcnt only).
+typedef const double __attribute__((address_space(1))) GlobalD;
+
 // Stores of four consecutive elements of one row in the compute type.
 // __bf16 means the split form: hi = bf16(v) in plane 2j and lo = bf16(v - hi)
 // in plane 2j+1.
@@ -511,9 +515,8 @@ template <> struct XStore4<__bf16> {
 // the chunk with nonzero weight) is computed once; one member then costs a
 // table interpolation (shared by the band's z bins) and 2 nu multiply-adds.
 // For each k band the table slice reached by the tile and the band's At
-// rows/columns are staged in LDS, double-buffered: the next band's values
-// are loaded into registers before the current band is computed and written
-// to LDS after it, so their latency overlaps the work (values beyond the
+// rows/columns are staged in LDS, double-buffered, through registers loaded
+// two bands ahead, so their latency overlaps the work (values beyond the
 // prefetch registers are copied synchronously).  A slice longer than the LDS
 // buffer is read from global memory instead.
 template <class OutT, class CT, int NUM>
@@ -523,7 +526,7 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     constexpr int EPT = (sizeof(CT) == 4) ? 2 : 1;               // row passes of 16
     constexpr int TR = 16 * EPT, TC = 64;
     constexpr int SL = (sizeof(CT) == 4) ? 2048 : 1024;        // slice buffer
-    constexpr int NSL = 4, NAT = 2;   // prefetch registers: slice values, At values
+    constexpr int NSL = 4, NAT = 4;   // prefetch registers: slice values, At values
     const ChunkDev c = cs[blockIdx.z];
     const int i0 = blockIdx.y * TR, b0 = blockIdx.x * TC;
     if (i0 >= c.Np || b0 >= c.Np) return;
@@ -600,87 +603,114 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     if (hi < lo) lo = 0;
     const int nat = nzq * nu * (TC + TR);
 
-    // register prefetch of band kn, then its store into LDS buffer bf
-    CT rs[NSL], ra[NAT];
-    auto at_value = [&](int kn, int f) {
+    // Register prefetch of band kn, then its store into LDS buffer bf.  The
+    // loads are unconditional, at clamped (always valid) addresses, and keep
+    // their raw values until the store, so they stay in flight while the
+    // current band is computed and written.  At entry f (z bin m, template x,
+    // column or row p) sits at a band-independent offset plus kn MAX_NU Np.
+    double rs[NSL], ra[NAT];
+    long long aoff[NAT];
+    const long long astride = (long long)MAX_NU * Np;
+    auto at_offset = [&](int f) {
         const int m = f / (nu * (TC + TR)), x = (f / (TC + TR)) % nu, p = f % (TC + TR);
-        const double *At = aux + c.at_off + ((size_t)(m * kbu + kn) * MAX_NU + x) * Np;
-        return (CT)At[p < TC ? b0 + p : i0 + p - TC];
+        return c.at_off + ((long long)(m * kbu) * MAX_NU + x) * Np + (p < TC ? b0 + p : i0 + p - TC);
     };
+#pragma unroll
+    for (int r = 0; r < NAT; ++r) aoff[r] = (nat > 0) ? at_offset(min(tid + 256 * r, nat - 1)) : 0;
+    const bool pre = staged && len > 0;
     auto fetch = [&](int kn) {
-        const double *row = t.qt + (size_t)kn * t.nv + lo;
+        if (pre) {
+            // a global-memory pointer: flat loads would also hold up the LDS waits
+            const GlobalD *row = (const GlobalD *)(t.qt + (size_t)kn * t.nv + lo);
 #pragma unroll
-        for (int r = 0; r < NSL; ++r) {
-            const int x = tid + 256 * r;
-            rs[r] = (staged && x < len) ? (CT)row[x] : (CT)0;
+            for (int r = 0; r < NSL; ++r) rs[r] = row[min(tid + 256 * r, len - 1)];
         }
+        if (nat > 0) {
 #pragma unroll
-        for (int r = 0; r < NAT; ++r) {
-            const int f = tid + 256 * r;
-            ra[r] = (f < nat) ? at_value(kn, f) : (CT)0;
+            for (int r = 0; r < NAT; ++r) ra[r] = aux[aoff[r] + kn * astride];
         }
     };
     auto put = [&](int kn, int bf) {
+        if (pre) {
 #pragma unroll
-        for (int r = 0; r < NSL; ++r) {
-            const int x = tid + 256 * r;
-            if (staged && x < len) slice[bf][x] = rs[r];
+            for (int r = 0; r < NSL; ++r) {
+                const int x = tid + 256 * r;
+                if (x < len) slice[bf][x] = (CT)rs[r];
+            }
         }
 #pragma unroll
         for (int r = 0; r < NAT; ++r) {
             const int f = tid + 256 * r;
-            if (f < nat) atv[bf][f] = ra[r];
+            if (f < nat) atv[bf][f] = (CT)ra[r];
         }
-        if (staged && len > 256 * NSL) {
+        if (pre && len > 256 * NSL) {
             const double *row = t.qt + (size_t)kn * t.nv + lo;
             for (int x = tid + 256 * NSL; x < len; x += 256) slice[bf][x] = (CT)row[x];
         }
-        for (int f = tid + 256 * NAT; f < nat; f += 256) atv[bf][f] = at_value(kn, f);
+        for (int f = tid + 256 * NAT; f < nat; f += 256) atv[bf][f] = (CT)aux[at_offset(f) + kn * astride];
     };
+    // Band kn+1 goes into LDS at the start of iteration kn, from registers
+    // loaded during iteration kn-1; then band kn+2 is fetched.  The wait for
+    // those loads thus also covers only stores issued a whole band earlier.
+    // At entries past nat stay zero, so the member loop reads MAX-template
+    // rows without guards (their factors U are zero for x >= nu)
+    for (int f = tid; f < 2 * DP_ZB * MAX_NU * (TC + TR); f += 256) (&atv[0][0])[f] = (CT)0;
+    __syncthreads();
     if (kbu > 0) {
         fetch(0);
         put(0, 0);
+        if (kbu > 1) fetch(1);
     }
     __syncthreads();
     for (int kn = 0; kn < kbu; ++kn) {
         const int cur = kn & 1;
-        if (kn + 1 < kbu) fetch(kn + 1);
+        if (kn + 1 < kbu) put(kn + 1, cur ^ 1);
+        if (kn + 2 < kbu) fetch(kn + 2);
+        // the band's values at the tile's elements; exact zeros outside the
+        // chunk (a tile with no element inside stages no slice), so that
+        // every member term below is finite and zero there
         CT q[EPT][4];
-        const double *grow = t.qt + (size_t)kn * t.nv;
+        if (c.rm) {
+            // resolution-matrix mode: the band matrix, materialised in FP64
 #pragma unroll
-        for (int e = 0; e < EPT; ++e)
+            for (int e = 0; e < EPT; ++e)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (c.rm) {
-                    // resolution-matrix mode: the band matrix, materialised in FP64
+                for (int u = 0; u < 4; ++u) {
                     const bool okeu = (okm >> (4 * e + u)) & 1u;
                     q[e][u] = okeu ? (CT)mat[c.qb_off + ((size_t)kn * Np + i0 + rl + 16 * e) * Np + bb + u] : (CT)0;
-                    continue;
                 }
-                CT y0, y1;
-                if (staged) {
-                    const int o = iv[e][u] - lo;   // in [0, len - 2] for valid elements, 0 otherwise
-                    y0 = slice[cur][o > 0 ? o : 0];
-                    y1 = slice[cur][o > 0 ? o + 1 : 1];
-                } else {
-                    y0 = (CT)grow[iv[e][u]];
-                    y1 = (CT)grow[iv[e][u] + 1];
+        } else if (staged) {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int o = max(iv[e][u] - lo, 0);   // in [0, len - 2] for elements inside
+                    const CT y = slice[cur][o] * ((CT)1 - fr[e][u]) + slice[cur][o + 1] * fr[e][u];
+                    q[e][u] = ((okm >> (4 * e + u)) & 1u) ? y : (CT)0;
                 }
-                q[e][u] = y0 * ((CT)1 - fr[e][u]) + y1 * fr[e][u];
-            }
+        } else {
+            const double *grow = t.qt + (size_t)kn * t.nv;
+#pragma unroll
+            for (int e = 0; e < EPT; ++e)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const CT y = (CT)grow[iv[e][u]] * ((CT)1 - fr[e][u]) + (CT)grow[iv[e][u] + 1] * fr[e][u];
+                    q[e][u] = ((okm >> (4 * e + u)) & 1u) ? y : (CT)0;
+                }
+        }
         for (int m = 0; m < nzq; ++m) {
             const CT *ab = &atv[cur][m * nu * (TC + TR)];
             CT ac[4][NUM > 0 ? NUM : 1];
 #pragma unroll
             for (int x = 0; x < NUM; ++x)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) ac[u][x] = (x < nu) ? ab[x * (TC + TR) + 4 * cg + u] : (CT)0;
+                for (int u = 0; u < 4; ++u) ac[u][x] = ab[x * (TC + TR) + 4 * cg + u];
             const int j = m * kbu + kn;
 #pragma unroll
             for (int e = 0; e < EPT; ++e) {
                 CT ar[NUM > 0 ? NUM : 1];
 #pragma unroll
-                for (int x = 0; x < NUM; ++x) ar[x] = (x < nu) ? ab[x * (TC + TR) + TC + rl + 16 * e] : (CT)0;
+                for (int x = 0; x < NUM; ++x) ar[x] = ab[x * (TC + TR) + TC + rl + 16 * e];
                 CT val[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -688,14 +718,12 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
                     CT acc = q[e][u] * w;
 #pragma unroll
                     for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
-                    // padding elements are exact zeros: a tile with no valid
-                    // element stages no table slice, so q is undefined there
-                    val[u] = ((okm >> (4 * e + u)) & 1u) ? sc[e][u] * acc : (CT)0;
+                    // outside the chunk q, U and D are zero: an exact zero
+                    val[u] = sc[e][u] * acc;
                 }
                 XStore4<OutT>::put(Xc, msz, j, (size_t)(i0 + rl + 16 * e) * Np + bb, val);
             }
         }
-        if (kn + 1 < kbu) put(kn + 1, cur ^ 1);
         __syncthreads();
     }
     // the noise (H) and signal (T) members, in FP64 arithmetic

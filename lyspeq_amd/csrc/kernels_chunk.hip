@@ -293,6 +293,12 @@ __global__ __launch_bounds__(256) void k_solve_finish(const ChunkDev *cs, const 
 constexpr int DP_ZB = 8, DQ_RB = 4, DQ_CB = 3;
 typedef double dq4 __attribute__((ext_vector_type(4)));
 
+// RBT x CBT: the batch's largest counts of 16-row (band) and 16-column
+// (z bin x component) operand blocks, fixed at compile time so the matrix-core
+// work has no branches; blocks past a chunk's own counts are zero.  The table
+// rows of a K step (4 columns) are gathered two steps ahead into two named
+// register sets, so their latency overlaps the two intervening steps.
+template <int RBT, int CBT>
 __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double *in,
                                                const TableDev *tabs, BinsDev bins,
                                                const double *mat, const double *vec, double *P) {
@@ -311,22 +317,31 @@ __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double 
     __shared__ double sw[4][DP_ZB][64];
     __shared__ double sR[64][1 + MAX_NU];
     const int nx = 1 + nu;
-    const int nrow = c.kbu + 1, RB = (nrow + 15) / 16;
-    const int ncq = c.nzq * nx, ncol = ncq + nx, CB = (ncol + 15) / 16;
+    const int ncq = c.nzq * nx, ncol = ncq + nx;
+    const bool tab = !c.rm;
     // this lane's B columns: (z bin m, component x); m = -1 for the S columns
-    int bm[DQ_CB], bx[DQ_CB];
+    int bm[CBT], bx[CBT];
 #pragma unroll
-    for (int cb = 0; cb < DQ_CB; ++cb) {
+    for (int cb = 0; cb < CBT; ++cb) {
         const int col = cb * 16 + l % 16;
         if (col < ncq) { bm[cb] = col / nx; bx[cb] = col % nx; }
         else if (col < ncol) { bm[cb] = -1; bx[cb] = col - ncq; }
         else { bm[cb] = -2; bx[cb] = 0; }
     }
-    dq4 acc[DQ_RB][DQ_CB];
+    dq4 acc[RBT][CBT];
 #pragma unroll
-    for (int rb = 0; rb < DQ_RB; ++rb)
+    for (int rb = 0; rb < RBT; ++rb)
 #pragma unroll
-        for (int cb = 0; cb < DQ_CB; ++cb) acc[rb][cb] = dq4{0.0, 0.0, 0.0, 0.0};
+        for (int cb = 0; cb < CBT; ++cb) acc[rb][cb] = dq4{0.0, 0.0, 0.0, 0.0};
+    dvec2 gA[RBT], gB[RBT];
+    // both interpolation nodes of this lane's band at K slot kk + l/16, at a
+    // clamped (always valid) slot and band, so every gather is unconditional
+    auto gather = [&](int kk, dvec2 *g) {
+        const int iv = siv[wv][min(kk, 60) + l / 16];   // 0 for slots outside the chunk
+#pragma unroll
+        for (int rb = 0; rb < RBT; ++rb)
+            g[rb] = *(const GlobalD2 *)(t.q2 + 2 * ((size_t)iv * nk + min(rb * 16 + l % 16, nk - 1)));
+    };
     for (int b0 = 0; b0 < c.N; b0 += 64) {
         __syncthreads();
         {
@@ -357,49 +372,58 @@ __global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double 
         }
         __syncthreads();
         const int nb = min(64, c.N - b0);
-        for (int kk = 0; kk < nb; kk += 4) {
+        // one K step: operands of columns kk.. from the band values (the
+        // gathered table nodes, or the materialised matrices), then the MFMAs
+        auto step = [&](int kk, const dvec2 *g, bool tabs_mode) {
             const int bl = kk + l / 16;            // this lane's K slot
             const bool bok = rowok && bl < nb;
-            const int iv = siv[wv][bl];
             const double fr = sfr[wv][bl];
-            double a[DQ_RB], bv[DQ_CB];
+            const double sv = sS[wv][bl];
+            double a[RBT], bv[CBT];
 #pragma unroll
-            for (int rb = 0; rb < DQ_RB; ++rb) {
+            for (int rb = 0; rb < RBT; ++rb) {
                 const int kn = rb * 16 + l % 16;
-                double val = 0.0;
-                if (rb < RB && bok) {
-                    if (kn < c.kbu)
-                        val = c.rm ? mat[c.qb_off + ((size_t)kn * Np + i) * Np + b0 + bl]
-                                   : t.q[(size_t)iv * nk + kn] * (1 - fr) + t.q[(size_t)(iv + 1) * nk + kn] * fr;
-                    else if (kn == c.kbu)
-                        val = sS[wv][bl];
-                }
-                a[rb] = val;
+                double q;
+                if (tabs_mode) q = g[rb].x * (1 - fr) + g[rb].y * fr;
+                else q = mat[c.qb_off + ((size_t)min(kn, c.kbu - 1) * Np + i) * Np + b0 + bl];
+                a[rb] = (!bok) ? 0.0 : (kn < c.kbu) ? q : (kn == c.kbu) ? sv : 0.0;
             }
 #pragma unroll
-            for (int cb = 0; cb < DQ_CB; ++cb) {
-                double val = 0.0;
-                if (cb < CB && bok && bm[cb] >= -1) {
-                    const double r = sR[bl][bx[cb]];
-                    val = (bm[cb] >= 0) ? sw[wv][bm[cb]][bl] * r : r;
-                }
-                bv[cb] = val;
+            for (int cb = 0; cb < CBT; ++cb) {
+                const double r = sR[bl][bx[cb]];
+                const double wz = sw[wv][max(bm[cb], 0)][bl];
+                bv[cb] = (!bok || bm[cb] < -1) ? 0.0 : (bm[cb] >= 0) ? wz * r : r;
             }
 #pragma unroll
-            for (int rb = 0; rb < DQ_RB; ++rb)
+            for (int rb = 0; rb < RBT; ++rb)
 #pragma unroll
-                for (int cb = 0; cb < DQ_CB; ++cb)
-                    if (rb < RB && cb < CB)
-                        acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bv[cb], acc[rb][cb], 0, 0, 0);
+                for (int cb = 0; cb < CBT; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bv[cb], acc[rb][cb], 0, 0, 0);
+        };
+        if (tab) {
+            gather(0, gA);
+            gather(4, gB);
+            for (int kk = 0; kk < nb; kk += 8) {
+                step(kk, gA, true);
+                gather(kk + 8, gA);
+                if (kk + 4 < nb) {
+                    step(kk + 4, gB, true);
+                    gather(kk + 12, gB);
+                }
+            }
+        } else {
+            // resolution-matrix mode (kbu >= 1 there; rows past the chunk read
+            // inside the padded matrix and are zeroed by the selects)
+            for (int kk = 0; kk < nb; kk += 4) step(kk, gA, false);
         }
     }
     if (!rowok) return;
     // FP64 MFMA layout: element u of a lane holds row 4u + lane/16, column lane%16
 #pragma unroll
-    for (int rb = 0; rb < DQ_RB; ++rb)
+    for (int rb = 0; rb < RBT; ++rb)
 #pragma unroll
-        for (int cb = 0; cb < DQ_CB; ++cb) {
-            if (rb >= RB || cb >= CB || bm[cb] < -1) continue;
+        for (int cb = 0; cb < CBT; ++cb) {
+            if (bm[cb] < -1) continue;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int kn = rb * 16 + 4 * u + l / 16;
@@ -470,10 +494,6 @@ __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double 
 }
 
 // ------------------------------------------------------- stack generation --
-// A double in the global address space (loads through it are global_load,
-// counted by vmcnt only).
-typedef const double __attribute__((address_space(1))) GlobalD;
-
 // Stores of four consecutive elements of one row in the compute type.
 // __bf16 means the split form: hi = bf16(v) in plane 2j and lo = bf16(v - hi)
 // in plane 2j+1.
@@ -861,9 +881,24 @@ void launch_solve(const LaunchCtx &L) {
     k_mv_cols<<<gc, 256, 0, L.stream>>>(L.chunks, L.mat, L.vec, V_F, V_G, 1);
     k_solve_finish<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec);
 }
+template <int RBT>
+static void launch_dpass_rb(const LaunchCtx &L, dim3 grid) {
+    if (L.dp_cb <= 1)
+        k_dpass<RBT, 1><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
+    else if (L.dp_cb == 2)
+        k_dpass<RBT, 2><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
+    else
+        k_dpass<RBT, DQ_CB><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
+}
 void launch_dpass(const LaunchCtx &L) {
-    dim3 grid((L.np_max + 3) / 4, L.nchunks);
-    k_dpass<<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.vec, L.pj);
+    const dim3 grid((L.np_max + 3) / 4, L.nchunks);
+    switch (L.dp_rb) {
+        case 0:
+        case 1: launch_dpass_rb<1>(L, grid); break;
+        case 2: launch_dpass_rb<2>(L, grid); break;
+        case 3: launch_dpass_rb<3>(L, grid); break;
+        default: launch_dpass_rb<DQ_RB>(L, grid); break;
+    }
 }
 void launch_small(const LaunchCtx &L) { k_small<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj); }
 template <class OutT, class CT>

@@ -38,7 +38,7 @@ struct TableHost {
     double dv;
     int nv, nz;
     double v_length;
-    double *d_q = nullptr, *d_qt = nullptr, *d_s = nullptr;
+    double *d_q = nullptr, *d_qt = nullptr, *d_q2 = nullptr, *d_s = nullptr;
 };
 
 struct ChunkHost {
@@ -60,6 +60,7 @@ struct Batch {
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
     int np_max = 0, nvu_max = 0, nu_max = 0;
+    int dp_rb = 1, dp_cb = 1;   // d-pass operand blocks of the batch
     int rm_kbu = -1;   // resolution-matrix chunks: largest KBIN_UPP
     RmLayout rm_lay;   // and the LDS layout of their sandwich launch
     hipEvent_t evA[7]{};   // stage A (assembly .. solves) on stream st
@@ -199,7 +200,7 @@ int qml_destroy(qml_ctx *c) {
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
     if (c->st2) hipStreamSynchronize(c->st2);
-    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_s); }
+    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_q2); hipFree(t.d_s); }
     for (auto &b : c->batches) {
         hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
         for (auto &e : b.evA) if (e) hipEventDestroy(e);
@@ -278,6 +279,15 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
     HIPCHK(hipMemcpy(t.d_qt, deriv, q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&t.d_s, s.size() * sizeof(double)));
     HIPCHK(hipMemcpy(t.d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    // interpolation pairs (q[j][kn], q[j+1][kn]); the last row repeats its node
+    std::vector<double> q2(2 * q.size());
+    for (int j = 0; j < nv; ++j)
+        for (int kn = 0; kn < nk; ++kn) {
+            q2[2 * ((size_t)j * nk + kn)] = q[(size_t)j * nk + kn];
+            q2[2 * ((size_t)j * nk + kn) + 1] = q[(size_t)std::min(j + 1, nv - 1) * nk + kn];
+        }
+    HIPCHK(hipMalloc(&t.d_q2, q2.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(t.d_q2, q2.data(), q2.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(t.d_s, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
     c->tables.push_back(t);
     if (r_index) *r_index = (int)c->tables.size() - 1;
@@ -510,6 +520,7 @@ int qml_commit(qml_ctx *c) {
         const auto &t = c->tables[r];
         td[r].q = t.d_q;
         td[r].qt = t.d_qt;
+        td[r].q2 = t.d_q2;
         td[r].s = t.d_s;
         td[r].nv = t.nv;
         td[r].nz = t.nz;
@@ -726,6 +737,8 @@ int qml_commit(qml_ctx *c) {
             B.np_max = std::max(B.np_max, d.Np);
             B.nvu_max = std::max(B.nvu_max, d.nvu);
             B.nu_max = std::max(B.nu_max, d.nu);
+            B.dp_rb = std::max(B.dp_rb, (d.kbu + 1 + 15) / 16);
+            B.dp_cb = std::max(B.dp_cb, ((d.nzq + 1) * (1 + d.nu) + 15) / 16);
             if (d.rm) {
                 B.rm_kbu = std::max(B.rm_kbu, d.kbu);
                 ra_max = std::max(ra_max, 15 * d.ros + d.rnb);
@@ -796,6 +809,8 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.np_max = B.np_max;
         L.nvu_max = B.nvu_max;
         L.nu_max = B.nu_max;
+        L.dp_rb = B.dp_rb;
+        L.dp_cb = B.dp_cb;
         L.rin = c->d_rin;
         L.rm_kbu = B.rm_kbu;
         L.rm_lay = B.rm_lay;

@@ -36,6 +36,7 @@ struct BinsDev {
 struct TableDev {
     const double *q;   // nv x nk, k contiguous (interleaved for row gathers)
     const double *qt;  // nk x nv, v contiguous (for per-band slices)
+    const double *q2;  // nv x nk pairs (q[v][k], q[v+1][k]): both nodes of a gather in one load
     const double *s;   // nv x nz, z contiguous
     int nv, nz;
     double v1, dv, z1, dz;
@@ -70,6 +71,12 @@ struct ChunkDev {
     long long rin_off;   // into the resolution-matrix input arena (doubles)
     long long qb_off;    // into the FP64 matrix arena
 };
+// A double in the global address space: loads through it are global_load,
+// counted by vmcnt only (a flat load would also hold up the LDS waits).
+typedef const double __attribute__((address_space(1))) GlobalD;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef const dvec2 __attribute__((address_space(1))) GlobalD2;
+
 // Fine rows a 16-row tile of R touches: 15 ros + rnb <= RM_ROWS.
 constexpr int RM_ROWS = 96;
 

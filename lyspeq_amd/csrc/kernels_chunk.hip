@@ -793,42 +793,56 @@ __global__ void k_convert_li(const ChunkDev *cs, const double *mat, OutT *lic) {
 
 // ------------------------------------------------------------ Gram reduce --
 // E = sum over splits; per-chunk F, d, b, t in the reference's index space.
+// One thread per output value (the F upper triangle, then d, b, t), many
+// workgroups per chunk; values with no stored band, the lower triangle of F
+// and every value of a failed chunk are zero.
 __global__ __launch_bounds__(256) void k_gram_finalize(const ChunkDev *cs, const ChunkOut *outs,
                                                        const double *gram, const double *sj,
                                                        const int *status, int nk,
                                                        double *fc, double *dbt) {
-    const ChunkDev c = cs[blockIdx.x];
+    const ChunkDev c = cs[blockIdx.y];
     const ChunkOut o = outs[c.gidx];
-    const int tid = threadIdx.x;
-    const int ntile = c.mt * (c.mt + 1) / 2;
-    const double *d = sj + c.sj_off + (size_t)c.m * MAX_NU * MAX_NU;
-    double *F = fc + o.fc_off;
-    double *B = dbt + o.dbt_off;
+    const int nq = o.nq;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nq * nq + 3 * nq) return;
     const int fail = status[c.gidx];
-    for (int e = tid; e < o.nq * o.nq; e += 256) F[e] = 0.0;
-    for (int e = tid; e < 3 * o.nq; e += 256) B[e] = 0.0;
-    __syncthreads();
-    if (fail) return;
+    const int ntile = c.mt * (c.mt + 1) / 2;
+    // stored index of reference index r (z bin r / nk, band r % nk), or -1
+    auto sid = [&](int r) {
+        const int kn = r % nk;
+        return kn < c.kbu ? (r / nk) * c.kbu + kn : -1;
+    };
     auto E = [&](int a, int b) {
         if (a > b) { int tmp = a; a = b; b = tmp; }
         const int I = a / TILE, J = b / TILE;
         const int tidx = I * c.mt - I * (I - 1) / 2 + (J - I);
-        const int r = a % TILE, q = b % TILE;
-        double sum = 0.0;
-        for (int sp = 0; sp < c.ntu; ++sp)
-            sum += gram[c.gram_off + ((size_t)sp * ntile + tidx) * TILE * TILE + r * TILE + q];
-        return sum;
+        const double *g = gram + c.gram_off + (size_t)tidx * TILE * TILE + (a % TILE) * TILE + b % TILE;
+        const size_t step = (size_t)ntile * TILE * TILE;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int sp = 0;
+        for (; sp + 3 < c.ntu; sp += 4) {
+            s0 += g[sp * step];
+            s1 += g[(sp + 1) * step];
+            s2 += g[(sp + 2) * step];
+            s3 += g[(sp + 3) * step];
+        }
+        for (; sp < c.ntu; ++sp) s0 += g[sp * step];
+        return (s0 + s1) + (s2 + s3);
     };
-    auto kz = [&](int sidx) { return (sidx / c.kbu) * nk + (sidx % c.kbu); };
-    for (int e = tid; e < c.n * c.n; e += 256) {
-        const int a = e / c.n, b = e % c.n;
-        if (b < a) continue;
-        F[(size_t)kz(a) * o.nq + kz(b)] = E(a, b);
-    }
-    for (int a = tid; a < c.n; a += 256) {
-        B[kz(a)] = d[a];
-        B[o.nq + kz(a)] = E(a, c.n);
-        B[2 * o.nq + kz(a)] = E(a, c.n + 1);
+    double val = 0.0;
+    if (e < nq * nq) {
+        const int r = e / nq, q = e % nq;
+        const int a = sid(r), b = sid(q);
+        if (!fail && q >= r && a >= 0 && b >= 0) val = E(a, b);
+        fc[o.fc_off + e] = val;
+    } else {
+        const int f = e - nq * nq, k = f / nq, r = f % nq;
+        const int a = sid(r);
+        if (!fail && a >= 0) {
+            const double *d = sj + c.sj_off + (size_t)c.m * MAX_NU * MAX_NU;
+            val = (k == 0) ? d[a] : E(a, c.n + k - 1);
+        }
+        dbt[o.dbt_off + f] = val;
     }
 }
 
@@ -936,7 +950,9 @@ __global__ __launch_bounds__(256) void k_poison_lds() {
 void launch_poison_lds(hipStream_t st) { k_poison_lds<<<2048, 256, 65536, st>>>(); }
 void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double *gram, const int *status,
                           double *fc, double *dbt) {
-    k_gram_finalize<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, outs, gram, L.sj, status, L.bins.nk, fc, dbt);
+    const int tot = L.nq_max * L.nq_max + 3 * L.nq_max;
+    k_gram_finalize<<<dim3((tot + 255) / 256, L.nchunks), 256, 0, L.stream>>>(L.chunks, outs, gram, L.sj, status,
+                                                                             L.bins.nk, fc, dbt);
 }
 void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, const int *skip, int nchunks,
                        int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum) {

@@ -94,8 +94,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(const ChunkDev *cs, double *m
     __shared__ int fail;
     if (tid == 0) fail = 0;
     load_block(a, L, Np, p0, p0, false);
+    for (int e = tid; e < FB * FB; e += 256) v[(e / FB) * FLD + e % FB] = (e / FB == e % FB) ? 1.0 : 0.0;
     __syncthreads();
-    // unblocked right-looking factorisation of the 64x64 block
+    // unblocked right-looking factorisation of the 64x64 block; alongside it,
+    // V = L^-1 by forward elimination of the identity: at step k row k of V
+    // (complete by then) is divided by L_kk, and every row i > k takes off
+    // L_ik times it
     for (int k = 0; k < FB; ++k) {
         if (tid == 0) {
             const double d = a[k * FLD + k];
@@ -103,25 +107,19 @@ __global__ __launch_bounds__(256) void k_chol_diag(const ChunkDev *cs, double *m
             a[k * FLD + k] = (d > 0.0) ? sqrt(d) : 1.0;
         }
         __syncthreads();
-        if (tid > k && tid < FB) a[tid * FLD + k] /= a[k * FLD + k];
+        const double dkk = a[k * FLD + k];
+        if (tid > k && tid < FB) a[tid * FLD + k] /= dkk;
+        if (tid >= FB && tid - FB <= k) v[k * FLD + tid - FB] /= dkk;
         __syncthreads();
         for (int e = tid; e < FB * FB; e += 256) {
             const int i = e / FB, j = e % FB;
-            if (j > k && j <= i) a[i * FLD + j] -= a[i * FLD + k] * a[j * FLD + k];
+            if (i > k) {
+                if (j > k && j <= i) a[i * FLD + j] -= a[i * FLD + k] * a[j * FLD + k];
+                else if (j <= k) v[i * FLD + j] -= a[i * FLD + k] * v[k * FLD + j];
+            }
         }
         __syncthreads();
     }
-    // inverse by column: thread j solves L x = e_j for rows j..63
-    if (tid < FB) {
-        const int j = tid;
-        for (int i = 0; i < j; ++i) v[i * FLD + j] = 0.0;
-        for (int i = j; i < FB; ++i) {
-            double s = (i == j) ? 1.0 : 0.0;
-            for (int k = j; k < i; ++k) s -= a[i * FLD + k] * v[k * FLD + j];
-            v[i * FLD + j] = s / a[i * FLD + i];
-        }
-    }
-    __syncthreads();
     for (int e = tid; e < FB * FB; e += 256) {
         const int i = e / FB, j = e % FB;
         const size_t o = (size_t)(p0 + i) * Np + p0 + j;

@@ -61,6 +61,7 @@ struct Batch {
     int nwl = 0, nwr = 0, nwg = 0;
     int np_max = 0, nvu_max = 0, nu_max = 0;
     int dp_rb = 1, dp_cb = 1;   // d-pass operand blocks of the batch
+    int nq_max = 0;             // most N_Q_MATRICES of a chunk in the batch
     int rm_kbu = -1;   // resolution-matrix chunks: largest KBIN_UPP
     RmLayout rm_lay;   // and the LDS layout of their sandwich launch
     hipEvent_t evA[7]{};   // stage A (assembly .. solves) on stream st
@@ -737,6 +738,7 @@ int qml_commit(qml_ctx *c) {
             B.np_max = std::max(B.np_max, d.Np);
             B.nvu_max = std::max(B.nvu_max, d.nvu);
             B.nu_max = std::max(B.nu_max, d.nu);
+            B.nq_max = std::max(B.nq_max, c->outs[d.gidx].nq);
             B.dp_rb = std::max(B.dp_rb, (d.kbu + 1 + 15) / 16);
             B.dp_cb = std::max(B.dp_cb, ((d.nzq + 1) * (1 + d.nu) + 15) / 16);
             if (d.rm) {
@@ -810,6 +812,7 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         L.nvu_max = B.nvu_max;
         L.nu_max = B.nu_max;
         L.dp_rb = B.dp_rb;
+        L.nq_max = B.nq_max;
         L.dp_cb = B.dp_cb;
         L.rin = c->d_rin;
         L.rm_kbu = B.rm_kbu;

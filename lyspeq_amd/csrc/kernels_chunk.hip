@@ -539,8 +539,8 @@ template <> struct XStore4<__bf16> {
 // two bands ahead, so their latency overlaps the work (values beyond the
 // prefetch registers are copied synchronously).  A slice longer than the LDS
 // buffer is read from global memory instead.
-template <class OutT, class CT, int NUM>
-__global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *in,
+template <class OutT, class CT, int NUM, int OCC = 1>
+__global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const double *in,
                                               const TableDev *tabs, BinsDev bins,
                                               const double *mat, const double *aux, OutT *X) {
     constexpr int EPT = (sizeof(CT) == 4) ? 2 : 1;               // row passes of 16
@@ -566,8 +566,9 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     __shared__ int red_lo[4], red_hi[4];
 
     // pair geometry
-    int iv[EPT][4], mlo[EPT][4];
+    int iv[EPT][4];
     unsigned okm = 0;   // bit 4e+u: element (e, u) lies inside the chunk
+    unsigned mpk = 0;   // bits 4(4e+u)..+3: the element's first z bin (at most 8)
     CT fr[EPT][4], sc[EPT][4], wlo[EPT][4], whi[EPT][4];
     CT ur[EPT][NUM > 0 ? NUM : 1], ucl[4][NUM > 0 ? NUM : 1];
     int lo = 1 << 30, hi = -1;
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
                     }
                 }
             }
-            mlo[e][u] = m0;
+            mpk |= (unsigned)m0 << (4 * (4 * e + u));
             wlo[e][u] = w0;
             whi[e][u] = w1;
         }
@@ -629,11 +630,12 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
     // current band is computed and written.  At entry f (z bin m, template x,
     // column or row p) sits at a band-independent offset plus kn MAX_NU Np.
     double rs[NSL], ra[NAT];
-    long long aoff[NAT];
-    const long long astride = (long long)MAX_NU * Np;
+    int aoff[NAT];   // from the chunk's At block, which holds under 2^31 values
+    const double *auxc = aux + c.at_off;
+    const int astride = MAX_NU * Np;
     auto at_offset = [&](int f) {
         const int m = f / (nu * (TC + TR)), x = (f / (TC + TR)) % nu, p = f % (TC + TR);
-        return c.at_off + ((long long)(m * kbu) * MAX_NU + x) * Np + (p < TC ? b0 + p : i0 + p - TC);
+        return ((m * kbu) * MAX_NU + x) * Np + (p < TC ? b0 + p : i0 + p - TC);
     };
 #pragma unroll
     for (int r = 0; r < NAT; ++r) aoff[r] = (nat > 0) ? at_offset(min(tid + 256 * r, nat - 1)) : 0;
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
         }
         if (nat > 0) {
 #pragma unroll
-            for (int r = 0; r < NAT; ++r) ra[r] = aux[aoff[r] + kn * astride];
+            for (int r = 0; r < NAT; ++r) ra[r] = auxc[aoff[r] + kn * astride];
         }
     };
     auto put = [&](int kn, int bf) {
@@ -667,7 +669,7 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
             const double *row = t.qt + (size_t)kn * t.nv + lo;
             for (int x = tid + 256 * NSL; x < len; x += 256) slice[bf][x] = (CT)row[x];
         }
-        for (int f = tid + 256 * NAT; f < nat; f += 256) atv[bf][f] = (CT)aux[at_offset(f) + kn * astride];
+        for (int f = tid + 256 * NAT; f < nat; f += 256) atv[bf][f] = (CT)auxc[at_offset(f) + kn * astride];
     };
     // Band kn+1 goes into LDS at the start of iteration kn, from registers
     // loaded during iteration kn-1; then band kn+2 is fetched.  The wait for
@@ -734,7 +736,8 @@ __global__ __launch_bounds__(256) void k_xgen(const ChunkDev *cs, const double *
                 CT val[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const CT w = (m == mlo[e][u]) ? wlo[e][u] : ((m == mlo[e][u] + 1) ? whi[e][u] : (CT)0);
+                    const int ml = (mpk >> (4 * (4 * e + u))) & 15u;
+                    const CT w = (m == ml) ? wlo[e][u] : ((m == ml + 1) ? whi[e][u] : (CT)0);
                     CT acc = q[e][u] * w;
 #pragma unroll
                     for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
@@ -919,10 +922,13 @@ template <class OutT, class CT>
 static void launch_xgen_t(const LaunchCtx &L, OutT *X) {
     const int tr = (sizeof(CT) == 4) ? 32 : 16;
     dim3 grid(L.np_max / 64, L.np_max / tr, L.nchunks);
+    // FP32 arithmetic with up to two templates runs three waves per SIMD
+    // (a few registers spill; measured faster than two waves without)
+    constexpr int OCC = (sizeof(CT) == 4) ? 3 : 1;
     if (L.nu_max <= 0)
-        k_xgen<OutT, CT, 0><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+        k_xgen<OutT, CT, 0, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
     else if (L.nu_max <= 2)
-        k_xgen<OutT, CT, 2><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+        k_xgen<OutT, CT, 2, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
     else
         k_xgen<OutT, CT, MAX_NU><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
 }

@@ -378,36 +378,39 @@ namespace qml {
 // and issues MFMAs only.  Every product is A B^T with both operands K-major
 // (X_j is symmetric; Li^T and the Gram are NT by construction).
 //
-// A workgroup computes a BM x BN tile with one wave per 64x64 quarter
+// A workgroup computes a 128x128 tile with one wave per 64x64 quarter
 // (4x4 blocks of 16x16, three v_mfma_f32_16x16x32_bf16 per block and K step:
-// lo*hi, hi*lo, hi*hi).  The whitening products use 128x256 (left) and
-// 256x128 (right) tiles of eight waves: the triangular operand keeps the
-// 128 granularity of its K range, and the bytes staged per MFMA fall by a
-// quarter against 128x128.  The Gram uses 128x128.  Waves whose quarter lies
-// outside the matrix, or strictly below the diagonal for the upper-only
-// products, skip their MFMAs (a separate copy of the loop, so the
-// accumulators stay in AGPRs).
+// lo*hi, hi*lo, hi*hi).  Waves whose quarter lies outside the matrix, or
+// strictly below the diagonal for the upper-only products, skip their MFMAs
+// (a separate copy of the loop) but still stage their share of the operands.
+// NST LDS stages: the whitening products use one (168 VGPRs and 36 KB of
+// LDS, three workgroups per CU; the next stage waits in registers), the Gram
+// two (two workgroups per CU, a second register set two stages ahead).
 //
 // LDS: per stage four planes (A hi, A lo, B hi, B lo) of rows x 32 bf16
 // (64 bytes, unpadded).  The four 16-byte chunks of row r sit at slot
-// c ^ ((r >> 1) & 3): any 8 consecutive rows read at one chunk, and the
-// staging writes (two rows' four chunks per 8 lanes), then fall on 8
-// distinct 16-byte slots of a 128-byte bank line.
+// c ^ (((r >> 1) ^ ((r >> 3) & 1)) & 3): any 16 consecutive rows (from a multiple
+// of 16) read at one chunk, and the staging writes (four rows' four chunks
+// per 16 lanes), then fall on 16 distinct 16-byte slots of a 256-byte span
+// of the 64 banks, and any aligned 8 of them on 8 distinct slots of 128
+// bytes.
 //
 // The MFMA is issued as B-fragment x A-fragment, so that each lane ends with
 // four consecutive columns of one output row (8-byte bf16 / 32-byte FP64
 // stores) instead of four rows of one column.
 constexpr int BK = 32;
 
-__device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ ((r >> 1) & 3)) << 3); }
+// the swizzle repeats every 16 rows, so fragment block x sits at a fixed
+// offset x * 16 * BK from block 0
+__device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ (((r >> 1) ^ ((r >> 3) & 1)) & 3)) << 3); }
 
 template <int MODE> struct Bf3Shape;
 template <> struct Bf3Shape<MODE_LEFT> { static constexpr int BM = 128, BN = 128; };
 template <> struct Bf3Shape<MODE_RIGHT> { static constexpr int BM = 128, BN = 128; };
 template <> struct Bf3Shape<MODE_GRAM> { static constexpr int BM = 128, BN = 128; };
 
-template <int MODE>
-__global__ __launch_bounds__(64 * (Bf3Shape<MODE>::BM / 64) * (Bf3Shape<MODE>::BN / 64))
+template <int MODE, int NST>
+__global__ __launch_bounds__(64 * (Bf3Shape<MODE>::BM / 64) * (Bf3Shape<MODE>::BN / 64), NST == 1 ? 3 : 2)
 void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *Abase, const __bf16 *Bbase,
                 void *Cbase) {
     constexpr int BM = Bf3Shape<MODE>::BM, BN = Bf3Shape<MODE>::BN;
@@ -453,7 +456,9 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         kend = kbeg + TILE * TILE;
         Mv = Nv = c.m;
     }
-    __shared__ __attribute__((aligned(16))) __bf16 smem[2 * STG];
+    constexpr int EP = 72;   // row pitch of the staged epilogue block, in bf16
+    constexpr int SMEM = (NST * STG > 4 * 64 * EP) ? NST * STG : 4 * 64 * EP;
+    __shared__ __attribute__((aligned(16))) __bf16 smem[SMEM];
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int wm = w / WN, wn = w % WN;
     const int r0 = I0 + 64 * wm, c0 = J0 + 64 * wn;
@@ -480,82 +485,123 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     const __bf16 *pb1 = Bhi + (long long)brow(PB - 1) * ldb + 8 * lc;
     const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
     const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
-    uint4 rah0, ral0, rah1, ral1, rbh0, rbl0, rbh1, rbl1;
+    // two staging register sets (P, Q): stage k+2 is in flight while k+1 waits
+    // to be stored and k is multiplied
+    uint4 Prah0, Pral0, Prah1, Pral1, Prbh0, Prbl0, Prbh1, Prbl1;
+    [[maybe_unused]] uint4 Qrah0, Qral0, Qrah1, Qral1, Qrbh0, Qrbl0, Qrbh1, Qrbl1;
     // fragment reads: row 16x + lane%16 of the wave's 64-row slab, chunk lane/16
     const int rr = lane % 16, cq = lane / 16;
     const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
-#define QML_BF3_GLOAD(k0)                                                        \
-    rah0 = *(const uint4 *)(pa0 + (k0));                                         \
-    ral0 = *(const uint4 *)(pa0 + alo + (k0));                                   \
-    rbh0 = *(const uint4 *)(pb0 + (k0));                                         \
-    rbl0 = *(const uint4 *)(pb0 + blo + (k0));                                   \
+#define QML_BF3_GLOAD(R, k0)                                                     \
+    R##rah0 = *(const uint4 *)(pa0 + (k0));                                      \
+    R##ral0 = *(const uint4 *)(pa0 + alo + (k0));                                \
+    R##rbh0 = *(const uint4 *)(pb0 + (k0));                                      \
+    R##rbl0 = *(const uint4 *)(pb0 + blo + (k0));                                \
     if constexpr (PA > 1) {                                                      \
-        rah1 = *(const uint4 *)(pa1 + (k0));                                     \
-        ral1 = *(const uint4 *)(pa1 + alo + (k0));                               \
+        R##rah1 = *(const uint4 *)(pa1 + (k0));                                  \
+        R##ral1 = *(const uint4 *)(pa1 + alo + (k0));                            \
     }                                                                            \
     if constexpr (PB > 1) {                                                      \
-        rbh1 = *(const uint4 *)(pb1 + (k0));                                     \
-        rbl1 = *(const uint4 *)(pb1 + blo + (k0));                               \
+        R##rbh1 = *(const uint4 *)(pb1 + (k0));                                  \
+        R##rbl1 = *(const uint4 *)(pb1 + blo + (k0));                            \
     }
-#define QML_BF3_SSTORE(st)                                                       \
+#define QML_BF3_SSTORE(R, st)                                                    \
     {                                                                            \
         __bf16 *s0 = smem + (st) * STG;                                          \
-        *(uint4 *)(s0 + wa0) = rah0;                                             \
-        *(uint4 *)(s0 + PLA + wa0) = ral0;                                       \
-        *(uint4 *)(s0 + 2 * PLA + wb0) = rbh0;                                   \
-        *(uint4 *)(s0 + 2 * PLA + PLB + wb0) = rbl0;                             \
+        *(uint4 *)(s0 + wa0) = R##rah0;                                          \
+        *(uint4 *)(s0 + PLA + wa0) = R##ral0;                                    \
+        *(uint4 *)(s0 + 2 * PLA + wb0) = R##rbh0;                                \
+        *(uint4 *)(s0 + 2 * PLA + PLB + wb0) = R##rbl0;                          \
         if constexpr (PA > 1) {                                                  \
-            *(uint4 *)(s0 + wa1) = rah1;                                         \
-            *(uint4 *)(s0 + PLA + wa1) = ral1;                                   \
+            *(uint4 *)(s0 + wa1) = R##rah1;                                      \
+            *(uint4 *)(s0 + PLA + wa1) = R##ral1;                                \
         }                                                                        \
         if constexpr (PB > 1) {                                                  \
-            *(uint4 *)(s0 + 2 * PLA + wb1) = rbh1;                               \
-            *(uint4 *)(s0 + 2 * PLA + PLB + wb1) = rbl1;                         \
+            *(uint4 *)(s0 + 2 * PLA + wb1) = R##rbh1;                            \
+            *(uint4 *)(s0 + 2 * PLA + PLB + wb1) = R##rbl1;                      \
         }                                                                        \
     }
 #define QML_BF3_MFMA(st)                                                                              \
     {                                                                                                 \
         const __bf16 *s0 = smem + (st) * STG;                                                         \
-        bf8 ah[4], al[4], bh[4], bl[4];                                                               \
-        _Pragma("unroll") for (int x = 0; x < 4; ++x) {                                               \
-            ah[x] = *(const bf8 *)(s0 + aoff + x * 16 * BK);                                          \
-            al[x] = *(const bf8 *)(s0 + PLA + aoff + x * 16 * BK);                                    \
-            bh[x] = *(const bf8 *)(s0 + 2 * PLA + boff + x * 16 * BK);                                \
-            bl[x] = *(const bf8 *)(s0 + 2 * PLA + PLB + boff + x * 16 * BK);                          \
-        }                                                                                             \
-        _Pragma("unroll") for (int x = 0; x < 4; ++x)                                                 \
+        bf8 bh[4], bl[4];                                                                             \
         _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                               \
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al[x], acc[x][y], 0, 0, 0);    \
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah[x], acc[x][y], 0, 0, 0);    \
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah[x], acc[x][y], 0, 0, 0);    \
+            bh[y] = *(const bf8 *)(s0 + 2 * PLA + boff + y * 16 * BK);                                \
+            bl[y] = *(const bf8 *)(s0 + 2 * PLA + PLB + boff + y * 16 * BK);                          \
+        }                                                                                             \
+        _Pragma("unroll") for (int x = 0; x < 4; ++x) {                                               \
+            const bf8 ah = *(const bf8 *)(s0 + aoff + x * 16 * BK);                                   \
+            const bf8 al = *(const bf8 *)(s0 + PLA + aoff + x * 16 * BK);                             \
+            _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                           \
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al, acc[x][y], 0, 0, 0);   \
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah, acc[x][y], 0, 0, 0);   \
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah, acc[x][y], 0, 0, 0);   \
+            }                                                                                         \
         }                                                                                             \
     }
-// One K sweep: stage k0+BK is fetched into registers while stage k0 is
-// multiplied out of LDS, then stored into the other LDS buffer.
-#define QML_BF3_SWEEP(DO_MFMA)                                         \
-    {                                                                  \
-        QML_BF3_GLOAD(kbeg)                                            \
-        QML_BF3_SSTORE(0)                                              \
-        __syncthreads();                                               \
-        int st = 0;                                                    \
-        for (int k0 = kbeg; k0 < kend; k0 += BK, st ^= 1) {            \
-            const bool more = k0 + BK < kend;                          \
-            if (more) {                                                \
-                QML_BF3_GLOAD(k0 + BK)                                 \
-            }                                                          \
-            if (DO_MFMA) QML_BF3_MFMA(st)                              \
-            if (more) QML_BF3_SSTORE(st ^ 1)                           \
-            __syncthreads();                                           \
-        }                                                              \
+// One K sweep over n = (kend - kbeg)/BK stages, two steps per pass: the
+// loads of stage s+2 are issued (into the register set just stored), stage
+// s is multiplied out of LDS, and stage s+1 (loaded a whole step earlier)
+// goes into the other LDS buffer.  Loads past the sweep repeat the last
+// stage (unconditional, so the wait counts stay exact) and are not stored.
+#define QML_BF3_SWEEP(DO_MFMA)                                                \
+    {                                                                         \
+        const int ns = (kend - kbeg) / BK;                                    \
+        const int klast = kend - BK;                                          \
+        QML_BF3_GLOAD(P, kbeg)                                                \
+        QML_BF3_GLOAD(Q, min(kbeg + BK, klast))                               \
+        QML_BF3_SSTORE(P, 0)                                                  \
+        __syncthreads();                                                      \
+        int s = 0;                                                            \
+        for (; s + 1 < ns; s += 2) {                                          \
+            const int k0 = kbeg + s * BK;                                     \
+            QML_BF3_GLOAD(P, min(k0 + 2 * BK, klast))                         \
+            if (DO_MFMA) QML_BF3_MFMA(0)                                      \
+            QML_BF3_SSTORE(Q, 1)                                              \
+            __syncthreads();                                                  \
+            QML_BF3_GLOAD(Q, min(k0 + 3 * BK, klast))                         \
+            if (DO_MFMA) QML_BF3_MFMA(1)                                      \
+            if (s + 2 < ns) QML_BF3_SSTORE(P, 0)                              \
+            __syncthreads();                                                  \
+        }                                                                     \
+        if (s < ns) {                                                         \
+            if (DO_MFMA) QML_BF3_MFMA(0)                                      \
+            __syncthreads();                                                  \
+        }                                                                     \
+    }
+// Single LDS stage: stage s+1 is loaded into registers while stage s is
+// multiplied, then stored once every wave has read stage s.
+#define QML_BF3_SWEEP1(DO_MFMA)                                               \
+    {                                                                         \
+        const int ns = (kend - kbeg) / BK;                                    \
+        const int klast = kend - BK;                                          \
+        QML_BF3_GLOAD(P, kbeg)                                                \
+        QML_BF3_SSTORE(P, 0)                                                  \
+        __syncthreads();                                                      \
+        for (int s = 0; s < ns; ++s) {                                        \
+            QML_BF3_GLOAD(P, min(kbeg + (s + 1) * BK, klast))                 \
+            if (DO_MFMA) QML_BF3_MFMA(0)                                      \
+            __syncthreads();                                                  \
+            if (s + 1 < ns) {                                                 \
+                QML_BF3_SSTORE(P, 0)                                          \
+                __syncthreads();                                              \
+            }                                                                 \
+        }                                                                     \
     }
     if (kbeg < kend) {
-        if (active) QML_BF3_SWEEP(true)
-        else QML_BF3_SWEEP(false)
+        if constexpr (NST == 1) {
+            if (active) QML_BF3_SWEEP1(true)
+            else QML_BF3_SWEEP1(false)
+        } else {
+            if (active) QML_BF3_SWEEP(true)
+            else QML_BF3_SWEEP(false)
+        }
     }
 #undef QML_BF3_GLOAD
 #undef QML_BF3_SSTORE
 #undef QML_BF3_MFMA
 #undef QML_BF3_SWEEP
+#undef QML_BF3_SWEEP1
     if (!active) return;
     // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
     const int orow = lane % 16, ocol = 4 * (lane / 16);
@@ -596,8 +642,6 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         ldc = TILE;
         lo = ktot;
     }
-    constexpr int EP = 72;   // row pitch of the staged block, in bf16
-    static_assert(4 * 64 * EP <= 2 * STG, "epilogue staging fits the stage buffers");
     __bf16 *ep = smem + w * 64 * EP;
     typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
     // hi plane first; acc then keeps the residual v - hi for the lo plane
@@ -629,12 +673,14 @@ void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *wo
     if (nwork <= 0) return;
     const int grid = 8 * ((nwork + 7) / 8);
     const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
+    // whitening products: one LDS stage (three workgroups per CU); the Gram,
+    // whose K sweeps are longer: two stages
     if (mode == MODE_LEFT)
-        k_gemm_bf3<MODE_LEFT><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_LEFT, 1><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
     else if (mode == MODE_RIGHT)
-        k_gemm_bf3<MODE_RIGHT><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_RIGHT, 1><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
     else
-        k_gemm_bf3<MODE_GRAM><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_GRAM, 2><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
 }
 
 }  // namespace qml

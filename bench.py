@@ -61,8 +61,13 @@ def kernel_work(shapes, prec):
     for N, n, nt in shapes:
         m = n + 2
         N = float(N)
-        w["whiten_left"] += m * N * N * (N + 1)               # lower-triangular x dense
-        w["whiten_right"] += m * N * (N + 1) * (N + 2) / 3.0  # upper half, triangular operand
+        if prec == "bf16x3":
+            # X_j = M + M^T: P = Li M (lower x lower), G = P Li^T + Li P^T (upper half)
+            w["whiten_left"] += m * N * (N + 1) * (N + 2) / 3.0
+            w["whiten_right"] += 2.0 * m * N * (N + 1) * (N + 2) / 3.0
+        else:
+            w["whiten_left"] += m * N * N * (N + 1)               # lower-triangular x dense
+            w["whiten_right"] += m * N * (N + 1) * (N + 2) / 3.0  # upper half, triangular operand
         w["gram"] += m * (m + 1) / 2.0 * N * (N + 1)          # symmetric pairs over packed halves
         w["xgen_bytes"] += m * N * N * es                      # projected Q stack written once
     return w

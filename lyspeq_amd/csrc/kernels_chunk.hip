@@ -526,6 +526,10 @@ template <> struct XStore4<__bf16> {
 
 // X_j = D (Q_j - U At_j^T - At_j U^T) D for every member, in the compute
 // type, with arithmetic in CT (FP64 for the FP64 path, FP32 otherwise).
+// The split form stores T = M^T instead, M the lower triangle of X_j with
+// its diagonal halved (X_j = M + M^T): element (i, b) is X_j[i,b] above the
+// diagonal, half of it on the diagonal and zero below; 128-blocks strictly
+// below the diagonal are not written at all (the products never read them).
 //
 // A workgroup covers 16 EPT rows x 64 columns (EPT = 2 in FP32, 1 in
 // FP64); thread t owns the four consecutive columns 4(t%16).. and rows
@@ -547,9 +551,11 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
     constexpr int TR = 16 * EPT, TC = 64;
     constexpr int SL = (sizeof(CT) == 4) ? 2048 : 1024;        // slice buffer
     constexpr int NSL = 4, NAT = 4;   // prefetch registers: slice values, At values
+    constexpr bool UPPER = std::is_same<OutT, __bf16>::value;   // the T form
     const ChunkDev c = cs[blockIdx.z];
     const int i0 = blockIdx.y * TR, b0 = blockIdx.x * TC;
     if (i0 >= c.Np || b0 >= c.Np) return;
+    if (UPPER && (b0 >> 7) < (i0 >> 7)) return;
     const int Np = c.Np, N = c.N, nu = c.nu, tid = threadIdx.x, nzq = c.nzq, kbu = c.kbu;
     const int cg = tid % 16, rl = tid / 16;
     const int bb = b0 + 4 * cg;            // first of the thread's columns
@@ -591,7 +597,8 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
             const double xx = (v - t.v1) / t.dv;
             iv[e][u] = ok ? interp_index(xx, t.nv) : 0;
             fr[e][u] = ok ? (CT)(xx - iv[e][u]) : (CT)0;
-            sc[e][u] = ok ? (CT)(D[i] * D[b]) : (CT)0;
+            const double tf = (!UPPER || b > i) ? 1.0 : ((b == i) ? 0.5 : 0.0);
+            sc[e][u] = ok ? (CT)(D[i] * D[b] * tf) : (CT)0;
             if (ok) { lo = min(lo, iv[e][u]); hi = max(hi, iv[e][u]); }
             // the chunk's z bins [zm0, zm0 + nzq) with nonzero weight are at
             // most two consecutive ones (triangular or top-hat weights)
@@ -767,7 +774,8 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
                     double corr = 0.0;
                     for (int x = 0; x < nu; ++x)
                         corr += U[x * (size_t)Np + i] * At[(size_t)x * Np + b] + At[(size_t)x * Np + i] * U[x * (size_t)Np + b];
-                    vv = D[i] * D[b] * (base - corr);
+                    const double tf = (!UPPER || b > i) ? 1.0 : ((b == i) ? 0.5 : 0.0);
+                    vv = D[i] * D[b] * tf * (base - corr);
                 }
                 val[u] = (CT)vv;
             }

@@ -5,6 +5,11 @@
 //                                             symmetric and read row-wise as B^T)
 //   whiten_right:  G_j = Y_j Li^T            (upper tiles only, K-range
 //                                             [0, 64 (J+1)), tile-packed out)
+// The split-BF16x3 products instead write X_j = M + M^T (M the lower
+// triangle of X_j with its diagonal halved, stored as T = M^T):
+//   whiten_left :  P_j = Li M                (lower 128-tiles, K-range of tile
+//                                             (I, J) is [128 J, 128 (I+1)))
+//   whiten_right:  G_j = P_j Li^T + Li P_j^T (upper tiles, K-range [0, 128 (I+1)))
 //   gram        :  E_s = sum_{k in G tile s} G_j[k] G_j'[k]   (split-K partials,
 //                                             x2 for off-diagonal G tiles)
 //
@@ -431,20 +436,25 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     long long lda, ldb, alo, blo;
     int arows, brows, kbeg = 0, kend, Mv, Nv;
     if (MODE == MODE_LEFT) {
+        // P = Li M, lower tiles (J <= I): Li[i,k] needs k <= i and the stored
+        // T = M^T (upper triangle of X_j, diagonal halved) needs k >= n
         Ahi = Abase + c.lic_off;
         Bhi = Bbase + c.stk_off + 2 * wk.j * msz;
         lda = ldb = Np;
         alo = blo = msz;
         arows = brows = Np;
+        kbeg = J0;
         kend = min(Np, I0 + BM);
         Mv = Nv = Np;
     } else if (MODE == MODE_RIGHT) {
+        // G = P Li^T + Li P^T, upper tiles (I <= J): both terms need
+        // k <= min(i, n) <= i, so k < I0 + BM; first term here, second below
         Ahi = Abase + c.stk_off + 2 * wk.j * msz;
         Bhi = Bbase + c.lic_off;
         lda = ldb = Np;
         alo = blo = msz;
         arows = brows = Np;
-        kend = min(Np, J0 + BN);
+        kend = min(Np, I0 + BM);
         Mv = Nv = Np;
     } else {
         const long long ktot = (long long)c.ntu * TILE * TILE;
@@ -479,10 +489,14 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     const int lr = tid / 4, lc = tid % 4;
     const auto arow = [&](int q) { const int r = I0 + lr + RP * q; return r < arows ? r : 0; };
     const auto brow = [&](int q) { const int r = J0 + lr + RP * q; return r < brows ? r : 0; };
-    const __bf16 *pa0 = Ahi + (long long)arow(0) * lda + 8 * lc;
-    const __bf16 *pa1 = Ahi + (long long)arow(PA - 1) * lda + 8 * lc;
-    const __bf16 *pb0 = Bhi + (long long)brow(0) * ldb + 8 * lc;
-    const __bf16 *pb1 = Bhi + (long long)brow(PB - 1) * ldb + 8 * lc;
+    const __bf16 *pa0, *pa1, *pb0, *pb1;
+    const auto operands = [&](const __bf16 *a, const __bf16 *b) {
+        pa0 = a + (long long)arow(0) * lda + 8 * lc;
+        pa1 = a + (long long)arow(PA - 1) * lda + 8 * lc;
+        pb0 = b + (long long)brow(0) * ldb + 8 * lc;
+        pb1 = b + (long long)brow(PB - 1) * ldb + 8 * lc;
+    };
+    operands(Ahi, Bhi);
     const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
     const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
     // two staging register sets (P, Q): stage k+2 is in flight while k+1 waits
@@ -588,13 +602,21 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
             }                                                                 \
         }                                                                     \
     }
+#define QML_BF3_RUN                                                           \
+    if constexpr (NST == 1) {                                                 \
+        if (active) QML_BF3_SWEEP1(true)                                      \
+        else QML_BF3_SWEEP1(false)                                            \
+    } else {                                                                  \
+        if (active) QML_BF3_SWEEP(true)                                       \
+        else QML_BF3_SWEEP(false)                                             \
+    }
     if (kbeg < kend) {
-        if constexpr (NST == 1) {
-            if (active) QML_BF3_SWEEP1(true)
-            else QML_BF3_SWEEP1(false)
-        } else {
-            if (active) QML_BF3_SWEEP(true)
-            else QML_BF3_SWEEP(false)
+        QML_BF3_RUN
+        if constexpr (MODE == MODE_RIGHT) {
+            // second term: Li rows I against P rows J (each sweep ends on a
+            // barrier, so the stage buffers are free)
+            operands(Bhi, Ahi);
+            QML_BF3_RUN
         }
     }
 #undef QML_BF3_GLOAD
@@ -602,6 +624,7 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 #undef QML_BF3_MFMA
 #undef QML_BF3_SWEEP
 #undef QML_BF3_SWEEP1
+#undef QML_BF3_RUN
     if (!active) return;
     // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
     const int orow = lane % 16, ocol = 4 * (lane / 16);

@@ -747,12 +747,13 @@ int qml_commit(qml_ctx *c) {
                 rb_max = std::max(rb_max, 31 * d.ros + d.rnb);
                 nb_max = std::max(nb_max, d.rnb);
             }
-            // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16.
+            // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16,
+            // whose left product P = Li M is lower triangular (J <= I only)
             const int g = (c->prec == 2) ? 2 : 1;
             const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
             for (int j = 0; j < d.m; ++j)
                 for (int I = 0; I < nt; ++I)
-                    for (int J = 0; J < nt; ++J) wl.push_back({li, j, I, J});
+                    for (int J = 0; J < ((c->prec == 2) ? I + 1 : nt); ++J) wl.push_back({li, j, I, J});
             for (int j = 0; j < d.m; ++j)
                 for (int I = 0; I < nt; ++I)
                     for (int J = I; J < nt; ++J) wr.push_back({li, j, I, J});

@@ -74,6 +74,12 @@ typedef struct {
     int turn_off_sfid;         /* TurnOffBaseline > 0 */
     double max_forest_length_v;/* MaximumForestLength / DynamicChunkNumber */
     double max_pixel_length_v; /* MaximumMeanPixelSize */
+    int redshift_growth_on;    /* RedshiftGrowthOn > 0 (core/global_numbers.cpp:127): every stored
+                                  Q of z bin m is scaled by ((1+z_ij)/(1+z_m))^B
+                                  (Chunk::_scaleDerivativesWithRedshiftGrowth,
+                                  core/chunk_estimate.cpp:539-573) */
+    double fiducial_redshift_power; /* B, FiducialRedshiftPower (core/fiducial_cosmology.cpp:204);
+                                  read only when redshift_growth_on */
 } qml_bins;
 
 /* One chunk: the pixel subset [i1, i2) of a spectrum after the file-level

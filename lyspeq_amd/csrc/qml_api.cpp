@@ -231,6 +231,8 @@ int qml_set_bins(qml_ctx *c, const qml_bins *b) {
     if (b->nk <= 0 || b->nk > MAX_NK - 1 || b->nz <= 0 || b->nz > MAX_ZB || !b->kband_edges || !b->zbin_centers ||
         !(b->z_bin_width > 0))
         return fail(QML_ERR_ARG, "qml_set_bins: invalid bins");
+    if (b->redshift_growth_on > 0 && !std::isfinite(b->fiducial_redshift_power))
+        return fail(QML_ERR_ARG, "qml_set_bins: redshift growth needs a finite fiducial_redshift_power");
     c->bins = *b;
     c->kedges.assign(b->kband_edges, b->kband_edges + b->nk + 1);
     c->zcenters.assign(b->zbin_centers, b->zbin_centers + b->nz);
@@ -246,6 +248,8 @@ int qml_set_bins(qml_ctx *c, const qml_bins *b) {
     c->binsdev.nk = b->nk;
     c->binsdev.method = b->z_binning_method;
     c->binsdev.turn_off_sfid = b->turn_off_sfid;
+    c->binsdev.growth = b->redshift_growth_on > 0;
+    c->binsdev.growth_b = b->fiducial_redshift_power;
     c->T = b->nk * b->nz;
     int nvec = 0;
     if (b->cont_logl_order >= 0 || b->cont_lam_order >= 0) {

@@ -31,6 +31,8 @@ struct BinsDev {
     double dz;
     int nz, nk, method;
     int turn_off_sfid;
+    int growth;        // redshift growth of the derivatives on
+    double growth_b;   // its exponent B
 };
 
 struct TableDev {
@@ -103,7 +105,7 @@ __device__ __forceinline__ int interp_index(double xx, int n) {
 }
 
 // Redshift bin weight (core/global_numbers.cpp:393-471), elementwise.
-__device__ __forceinline__ double zweight(const BinsDev &b, double z, int zm) {
+__device__ __forceinline__ double zbin_weight(const BinsDev &b, double z, int zm) {
     const double zc = b.zc[zm], dz = b.dz;
     if (b.method == 0 || b.nz == 1) {
         return (z >= zc - dz / 2 && z <= zc + dz / 2) ? 1.0 : 0.0;
@@ -120,6 +122,15 @@ __device__ __forceinline__ double zweight(const BinsDev &b, double z, int zm) {
     }
     if (z >= zc - dz && z <= zc + dz) return 1. - fabs(z - zc) / dz;
     return 0.0;
+}
+
+// The factor every stored Q of z bin zm carries at pixel pair redshift z:
+// the bin weight, times ((1 + z) / (1 + z_m))^B with redshift growth on
+// (core/chunk_estimate.cpp:539-573, applied after the weights of :483-536).
+__device__ __forceinline__ double zweight(const BinsDev &b, double z, int zm) {
+    const double w = zbin_weight(b, z, zm);
+    if (!b.growth || w == 0.0) return w;
+    return w * pow((z + 1.0) / (b.zc[zm] + 1.0), b.growth_b);
 }
 
 // S_fid(z, v) by DiscreteInterpolation2D::evaluate with x = z (fast axis).

@@ -29,6 +29,8 @@ Engine::Engine(const Binning &bins, int precision, int device, double scratch_gb
     b.turn_off_sfid = bins_.turn_off_sfid ? 1 : 0;
     b.max_forest_length_v = bins_.maxForestLengthPerChunk();
     b.max_pixel_length_v = bins_.max_pixel_length_v;
+    b.redshift_growth_on = bins_.redshift_growth_on ? 1 : 0;
+    b.fiducial_redshift_power = bins_.fiducial_redshift_power;
     const int rc = qml_set_bins(ctx_, &b);
     if (rc != QML_OK) {
         const std::string msg = qml_last_error();

@@ -80,6 +80,8 @@ struct Binning {
     double max_pixel_length_v = 120.0;   // MaximumMeanPixelSize
     int oversampling_factor = -1;        // OversampleRmat: > 0 oversamples diagonal matrices
     double resomat_deconvolution_m = -1; // ResoMatrixDeconvolutionM: > 0 deconvolves them first
+    bool redshift_growth_on = false;     // RedshiftGrowthOn > 0
+    double fiducial_redshift_power = 3.591244;  // FiducialRedshiftPower (the growth exponent B)
     int nk() const { return (int)kband_edges.size() - 1; }
     int nz() const { return (int)zbin_centers.size(); }
     int totalKzBins() const { return nk() * nz(); }

@@ -61,7 +61,8 @@ class _Bins(C.Structure):
                 ("zbin_centers", C.POINTER(C.c_double)), ("z_bin_width", C.c_double),
                 ("z_binning_method", C.c_int), ("cont_logl_order", C.c_int),
                 ("cont_lam_order", C.c_int), ("turn_off_sfid", C.c_int),
-                ("max_forest_length_v", C.c_double), ("max_pixel_length_v", C.c_double)]
+                ("max_forest_length_v", C.c_double), ("max_pixel_length_v", C.c_double),
+                ("redshift_growth_on", C.c_int), ("fiducial_redshift_power", C.c_double)]
 
 
 class _ChunkIn(C.Structure):
@@ -198,6 +199,11 @@ class Binning:
     max_forest_length_v: float = 3.5e4
     number_of_chunks: int = 1
     max_pixel_length_v: float = 150.0
+    # RedshiftGrowthOn (core/global_numbers.cpp:127) and its exponent B,
+    # FiducialRedshiftPower (core/fiducial_cosmology.cpp:204, compiled-in
+    # default 3.591244 at :268-271)
+    redshift_growth_on: bool = False
+    fiducial_redshift_power: float = 3.591244
 
     def kband_edges(self) -> np.ndarray:
         nk = self.nklin + self.nklog
@@ -251,7 +257,8 @@ class QMLEngine:
                   binning.z_bin_width, binning.method, binning.cont_logl_order,
                   binning.cont_lam_order, int(bool(binning.turn_off_sfid)),
                   binning.max_forest_length_v / max(1, binning.number_of_chunks),
-                  binning.max_pixel_length_v)
+                  binning.max_pixel_length_v, int(bool(binning.redshift_growth_on)),
+                  float(binning.fiducial_redshift_power))
         _check(self._lib.qml_set_bins(self._ctx, C.byref(b)), self._lib)
         self.T = len(self._edges) - 1
         self.T *= binning.nz

@@ -44,6 +44,11 @@ class Bins:
     number_of_chunks: int = 1
     max_pixel_length_v: float = 150.0
     use_precomputed_fisher: bool = False
+    # RedshiftGrowthOn (core/global_numbers.cpp:127) and its exponent
+    # FiducialRedshiftPower (core/fiducial_cosmology.cpp:204; compiled-in
+    # PD13 value 3.591244 at :268-271)
+    redshift_growth_on: bool = False
+    fiducial_redshift_power: float = 3.591244
     kband_edges: np.ndarray = field(init=False)
     kband_centers: np.ndarray = field(init=False)
     zbin_centers: np.ndarray = field(init=False)

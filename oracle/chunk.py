@@ -369,6 +369,18 @@ class Chunk:
         t[iu] = redshift_weight(self.bins, z[iu], zm)
         return self._mirror(t)
 
+    def growth_factors(self, z, zm):
+        """_scaleDerivativesWithRedshiftGrowth, core/chunk_estimate.cpp:539-573:
+        ((1 + z_ij) / (1 + z_m))^B for j >= i, mirrored.  The reference
+        scales all N^2 entries with the z matrix it holds; away from the
+        oversampled path only its upper triangle is set (:384-398), so the
+        symmetric form here is the reference's upper triangle."""
+        iu = np.triu_indices(self.N)
+        g = np.zeros((self.N, self.N))
+        zcm = self.bins.zbin_centers[zm] + 1.0
+        g[iu] = np.power((z[iu] + 1.0) / zcm, self.bins.fiducial_redshift_power)
+        return self._mirror(g)
+
     def q_matrices(self, v, z):
         """Stored Q_k after copies and z weights, in stored order."""
         nk = self.bins.nk
@@ -380,6 +392,8 @@ class Chunk:
         for iz in range(nzq):
             zm = self.fisher_index_start // nk + iz
             w = self.redshift_weights(z, zm)
+            if self.bins.redshift_growth_on:
+                w = w * self.growth_factors(z, zm)
             for s in self.stored[iz * self.kbin_upp:(iz + 1) * self.kbin_upp]:
                 out.append(base[s % nk] * w)
         return out

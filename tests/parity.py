@@ -12,7 +12,8 @@ def oracle_bins(b: Binning) -> Bins:
                 z_bin_width=b.z_bin_width, nz=b.nz, klast=b.klast, method=b.method,
                 cont_logl_order=b.cont_logl_order, cont_lam_order=b.cont_lam_order,
                 turn_off_sfid=b.turn_off_sfid, max_forest_length_v=b.max_forest_length_v,
-                number_of_chunks=b.number_of_chunks, max_pixel_length_v=b.max_pixel_length_v)
+                number_of_chunks=b.number_of_chunks, max_pixel_length_v=b.max_pixel_length_v,
+                redshift_growth_on=b.redshift_growth_on, fiducial_redshift_power=b.fiducial_redshift_power)
 
 
 def oracle_tables(w: synth.Workload):

@@ -254,3 +254,53 @@ def test_resolution_matrix_mixed_precision_against_fp64():
         print(f"resolution matrix {precision}: F {ef:.2e} b {eb:.2e} t {et:.2e}", flush=True)
         assert P.rel_err(dm[0], d64[0]) < 1e-12
         assert ef < tol and eb < tol and et < tol
+
+
+def _with_growth(w, power=3.591244):
+    w.binning.redshift_growth_on = True
+    w.binning.fiducial_redshift_power = power
+    return w
+
+
+@pytest.mark.parametrize("config", ["desi", "rmat"])
+def test_redshift_growth_fp64_matches_oracle(config):
+    """RedshiftGrowthOn: every stored Q of z bin m carries ((1+z_ij)/(1+z_m))^B
+    on top of the bin weight, in the table and resolution-matrix modes."""
+    _print_case(f"redshift growth {config} fp64 vs oracle")
+    if config == "desi":
+        w = _with_growth(synth.desi_config(nchunks=12))
+    else:
+        w = _with_growth(synth.rmat_config(nchunks=6, n_range=(100, 170)))
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 41)
+    idx = list(range(len(w.chunks)))
+    eng, out, F, dbt = P.run_engine(w, th, idx, max_batch_chunks=5)
+    eng.close()
+    oout, oF, od = P.run_oracle(w, th, idx)
+    assert set(out) == set(oout) and len(out) >= 5
+    worst = 0.0
+    for i in out:
+        worst = max(worst, P.rel_err(out[i][0], oout[i][0]),
+                    *(P.rel_err(out[i][1][k], oout[i][1][k]) for k in range(3)))
+    print(f"redshift growth {config}: worst per-chunk relative error {worst:.2e}", flush=True)
+    assert worst < FP64_TOL
+    assert P.rel_err(F, oF) < FP64_TOL
+    for k in range(3):
+        assert P.rel_err(dbt[k], od[k]) < FP64_TOL
+    # growth changes the results (the test is not vacuous)
+    w.binning.redshift_growth_on = False
+    _, _, F0, _ = P.run_engine(w, th, idx)
+    assert P.rel_err(F0, F) > 1e-3
+
+
+def test_redshift_growth_mixed_precision_against_fp64():
+    _print_case("redshift growth mixed vs fp64")
+    w = _with_growth(synth.desi_config(nchunks=24))
+    T = w.binning.total_kz_bins
+    idx = list(range(len(w.chunks)))
+    _, _, F64, d64 = P.run_engine(w, np.zeros(T), idx, precision="fp64")
+    for precision, tol in (("fp32", 2e-4), ("bf16x3", 2e-3)):
+        _, _, Fm, dm = P.run_engine(w, np.zeros(T), idx, precision=precision)
+        ef, eb, et = P.fisher_norm_err(Fm, F64), P.rel_err(dm[1], d64[1]), P.rel_err(dm[2], d64[2])
+        print(f"redshift growth {precision}: F {ef:.2e} b {eb:.2e} t {et:.2e}", flush=True)
+        assert P.rel_err(dm[0], d64[0]) < 1e-12
+        assert ef < tol and eb < tol and et < tol

@@ -389,3 +389,44 @@ def test_rmat_preprocessing_through_the_c_abi_matches_the_oracle():
         got = rmat_oversample(dec, osamp)
         want = ResoMatrix("dia", dec, 11).oversample(osamp).values
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("power", [3.591244, 1.0, -2.5])
+def test_redshift_growth_is_a_per_pixel_congruence(power):
+    """With 1 + z_ij = sqrt(l_i l_j) (l in units of the Lya wavelength), the
+    growth factor ((1 + z_ij)/(1 + z_m))^B separates into g_i g_j with
+    g_i = (l_i / (1 + z_m))^(B/2): every growth-scaled Q of z bin m is
+    G_m Q G_m.  A wrong pair redshift (arithmetic mean), a bin centre without
+    the 1 +, or a wrong exponent breaks the identity; B = 0 must reproduce
+    the unscaled matrices exactly."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    import parity as P
+    from lyspeq_amd import synth
+    w = synth.desi_config(nchunks=6)
+    tabs = P.oracle_tables(w)
+    c = w.chunks[0]
+    b_off = P.oracle_bins(w.binning)
+    w.binning.redshift_growth_on = True
+    w.binning.fiducial_redshift_power = power
+    b_on = P.oracle_bins(w.binning)
+    w.binning.fiducial_redshift_power = 0.0
+    b_zero = P.oracle_bins(w.binning)
+    ch_off = Chunk(b_off, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+    ch_on = Chunk(b_on, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+    ch_zero = Chunk(b_zero, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+    v, z = ch_off.vz_matrices()
+    q_off, q_on, q_zero = ch_off.q_matrices(v, z), ch_on.q_matrices(v, z), ch_zero.q_matrices(v, z)
+    nk = b_off.nk
+    assert len(q_on) == len(q_off) > 0
+    nzq = len(q_off) // ch_off.kbin_upp
+    assert nzq >= 1
+    for iz in range(nzq):
+        zm = ch_off.fisher_index_start // nk + iz
+        g = np.power(ch_off.wave / (1.0 + b_off.zbin_centers[zm]), power / 2.0)
+        for s in range(iz * ch_off.kbin_upp, (iz + 1) * ch_off.kbin_upp):
+            ref = g[:, None] * q_off[s] * g[None, :]
+            scale = np.max(np.abs(ref))
+            assert scale > 0
+            assert np.max(np.abs(q_on[s] - ref)) <= 1e-13 * scale
+            assert np.array_equal(q_zero[s], q_off[s])

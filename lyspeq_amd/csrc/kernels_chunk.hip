@@ -299,7 +299,7 @@ typedef double dq4 __attribute__((ext_vector_type(4)));
 // rows of a K step (4 columns) are gathered two steps ahead into two named
 // register sets, so their latency overlaps the two intervening steps.
 template <int RBT, int CBT>
-__global__ __launch_bounds__(256) void k_dpass(const ChunkDev *cs, const double *in,
+__global__ __launch_bounds__(256, (RBT * CBT <= 3) ? 3 : 1) void k_dpass(const ChunkDev *cs, const double *in,
                                                const TableDev *tabs, BinsDev bins,
                                                const double *mat, const double *vec, double *P) {
     const ChunkDev c = cs[blockIdx.y];

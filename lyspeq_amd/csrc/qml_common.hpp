@@ -124,13 +124,19 @@ __device__ __forceinline__ double zbin_weight(const BinsDev &b, double z, int zm
     return 0.0;
 }
 
+// ((1 + z) / (1 + z_m))^B, out of line: inlined, pow's registers would count
+// against every kernel that weights a stored Q, growth on or off.
+static __device__ __attribute__((noinline)) double growth_factor(double z, double zc, double b) {
+    return pow((z + 1.0) / (zc + 1.0), b);
+}
+
 // The factor every stored Q of z bin zm carries at pixel pair redshift z:
 // the bin weight, times ((1 + z) / (1 + z_m))^B with redshift growth on
 // (core/chunk_estimate.cpp:539-573, applied after the weights of :483-536).
 __device__ __forceinline__ double zweight(const BinsDev &b, double z, int zm) {
     const double w = zbin_weight(b, z, zm);
     if (!b.growth || w == 0.0) return w;
-    return w * pow((z + 1.0) / (b.zc[zm] + 1.0), b.growth_b);
+    return w * growth_factor(z, b.zc[zm], b.growth_b);
 }
 
 // S_fid(z, v) by DiscreteInterpolation2D::evaluate with x = z (fast axis).

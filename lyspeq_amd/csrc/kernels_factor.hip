@@ -65,14 +65,38 @@ __device__ __forceinline__ void tile_zero(fd4 (&acc)[2][2]) {
         for (int y = 0; y < 2; ++y) acc[x][y] = fd4{0.0, 0.0, 0.0, 0.0};
 }
 
+// A 64x64 block in registers: thread t holds the column pairs 2 (e % 32)
+// of rows e / 32, e = t + 256 q (16-byte global loads through a
+// global-space pointer, so that they count in vmcnt only).
+struct BlockRegs {
+    dvec2 v[FB * FB / 512];
+};
+__device__ __forceinline__ void fetch_block(BlockRegs &R, const double *M, int Np, int r0, int c0) {
+#pragma unroll
+    for (int q = 0; q < FB * FB / 512; ++q) {
+        const int e = threadIdx.x + 256 * q, r = e / 32, c2 = 2 * (e % 32);
+        R.v[q] = *(GlobalD2 *)(M + (size_t)(r0 + r) * Np + c0 + c2);
+    }
+}
+// s[r * FLD + c] = block(r, c), or s[c * FLD + r] transposed
+__device__ __forceinline__ void put_block(double *s, const BlockRegs &R, bool transpose) {
+#pragma unroll
+    for (int q = 0; q < FB * FB / 512; ++q) {
+        const int e = threadIdx.x + 256 * q, r = e / 32, c2 = 2 * (e % 32);
+        if (transpose) {
+            s[c2 * FLD + r] = R.v[q].x;
+            s[(c2 + 1) * FLD + r] = R.v[q].y;
+        } else {
+            s[r * FLD + c2] = R.v[q].x;
+            s[r * FLD + c2 + 1] = R.v[q].y;
+        }
+    }
+}
 // s[r * FLD + c] = M[(r0 + r) * Np + c0 + c] (or transposed) for a 64x64 block
 __device__ __forceinline__ void load_block(double *s, const double *M, int Np, int r0, int c0, bool transpose) {
-    for (int e = threadIdx.x; e < FB * FB; e += 256) {
-        const int r = e / FB, c = e % FB;
-        const double v = M[(size_t)(r0 + r) * Np + c0 + c];
-        if (transpose) s[c * FLD + r] = v;
-        else s[r * FLD + c] = v;
-    }
+    BlockRegs R;
+    fetch_block(R, M, Np, r0, c0);
+    put_block(s, R, transpose);
 }
 
 __global__ __launch_bounds__(256) void k_chol_init(const ChunkDev *cs, double *mat) {
@@ -163,10 +187,19 @@ __global__ __launch_bounds__(256) void k_chol_panel(const ChunkDev *cs, double *
     // block row p of the inverse
     const int j = bx - ntmax;
     if (j >= p) return;
+    // L[p,k] and Li[k,j]^T for k = j .. p-1; the blocks of step k+1 are in
+    // flight in registers while step k multiplies (the last step re-reads
+    // its own blocks, unconditionally, so the waits stay exact)
+    BlockRegs ra, rb;
+    fetch_block(ra, L, Np, p0, j * FB);
+    fetch_block(rb, Li, Np, j * FB, j * FB);
     for (int k = j; k < p; ++k) {
-        load_block(sa, L, Np, p0, k * FB, false);       // L[p,k]
-        load_block(sb, Li, Np, k * FB, j * FB, true);   // Li[k,j]^T
+        put_block(sa, ra, false);
+        put_block(sb, rb, true);
         __syncthreads();
+        const int kn = min(k + 1, p - 1);
+        fetch_block(ra, L, Np, p0, kn * FB);
+        fetch_block(rb, Li, Np, kn * FB, j * FB);
         tile_nt(sa, sb, acc);
         __syncthreads();
     }

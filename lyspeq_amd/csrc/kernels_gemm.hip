@@ -489,14 +489,10 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     const int lr = tid / 4, lc = tid % 4;
     const auto arow = [&](int q) { const int r = I0 + lr + RP * q; return r < arows ? r : 0; };
     const auto brow = [&](int q) { const int r = J0 + lr + RP * q; return r < brows ? r : 0; };
-    const __bf16 *pa0, *pa1, *pb0, *pb1;
-    const auto operands = [&](const __bf16 *a, const __bf16 *b) {
-        pa0 = a + (long long)arow(0) * lda + 8 * lc;
-        pa1 = a + (long long)arow(PA - 1) * lda + 8 * lc;
-        pb0 = b + (long long)brow(0) * ldb + 8 * lc;
-        pb1 = b + (long long)brow(PB - 1) * ldb + 8 * lc;
-    };
-    operands(Ahi, Bhi);
+    // per-thread element offsets of the staged rows (below 2^31 for every
+    // operand); the operand bases are workgroup-uniform
+    const int oa0 = arow(0) * (int)lda + 8 * lc, oa1 = arow(PA - 1) * (int)lda + 8 * lc;
+    const int ob0 = brow(0) * (int)ldb + 8 * lc, ob1 = brow(PB - 1) * (int)ldb + 8 * lc;
     const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
     const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
     // two staging register sets (P, Q): stage k+2 is in flight while k+1 waits
@@ -506,19 +502,20 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     // fragment reads: row 16x + lane%16 of the wave's 64-row slab, chunk lane/16
     const int rr = lane % 16, cq = lane / 16;
     const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
-#define QML_BF3_GLOAD(R, k0)                                                     \
-    R##rah0 = *(const uint4 *)(pa0 + (k0));                                      \
-    R##ral0 = *(const uint4 *)(pa0 + alo + (k0));                                \
-    R##rbh0 = *(const uint4 *)(pb0 + (k0));                                      \
-    R##rbl0 = *(const uint4 *)(pb0 + blo + (k0));                                \
+#define QML_BF3_GLOADX(R, XA, XB, k0)                                            \
+    R##rah0 = *(const uint4 *)((XA) + oa0 + (k0));                               \
+    R##ral0 = *(const uint4 *)((XA) + alo + oa0 + (k0));                         \
+    R##rbh0 = *(const uint4 *)((XB) + ob0 + (k0));                               \
+    R##rbl0 = *(const uint4 *)((XB) + blo + ob0 + (k0));                         \
     if constexpr (PA > 1) {                                                      \
-        R##rah1 = *(const uint4 *)(pa1 + (k0));                                  \
-        R##ral1 = *(const uint4 *)(pa1 + alo + (k0));                            \
+        R##rah1 = *(const uint4 *)((XA) + oa1 + (k0));                           \
+        R##ral1 = *(const uint4 *)((XA) + alo + oa1 + (k0));                     \
     }                                                                            \
     if constexpr (PB > 1) {                                                      \
-        R##rbh1 = *(const uint4 *)(pb1 + (k0));                                  \
-        R##rbl1 = *(const uint4 *)(pb1 + blo + (k0));                            \
+        R##rbh1 = *(const uint4 *)((XB) + ob1 + (k0));                           \
+        R##rbl1 = *(const uint4 *)((XB) + blo + ob1 + (k0));                     \
     }
+#define QML_BF3_GLOAD(R, k0) QML_BF3_GLOADX(R, Ahi, Bhi, k0)
 #define QML_BF3_SSTORE(R, st)                                                    \
     {                                                                            \
         __bf16 *s0 = smem + (st) * STG;                                          \
@@ -610,13 +607,36 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         if (active) QML_BF3_SWEEP(true)                                       \
         else QML_BF3_SWEEP(false)                                             \
     }
+// The right product's two terms as one sweep of 2n stages: P rows I
+// against Li rows J, then Li rows I against P rows J (the same staged row
+// offsets with the operand bases swapped), so the second term's first stage
+// is in flight while the first term's last one is multiplied.
+#define QML_BF3_SWEEP1_2T(DO_MFMA)                                            \
+    {                                                                         \
+        const int ns = (kend - kbeg) / BK;                                    \
+        QML_BF3_GLOAD(P, kbeg)                                                \
+        QML_BF3_SSTORE(P, 0)                                                  \
+        __syncthreads();                                                      \
+        for (int s = 0; s < 2 * ns; ++s) {                                    \
+            const int sn = min(s + 1, 2 * ns - 1);                            \
+            const bool second = sn >= ns;                                     \
+            const __bf16 *xa = second ? Bhi : Ahi, *xb = second ? Ahi : Bhi;  \
+            QML_BF3_GLOADX(P, xa, xb, kbeg + (second ? sn - ns : sn) * BK)    \
+            if (DO_MFMA) QML_BF3_MFMA(0)                                      \
+            __syncthreads();                                                  \
+            if (s + 1 < 2 * ns) {                                             \
+                QML_BF3_SSTORE(P, 0)                                          \
+                __syncthreads();                                              \
+            }                                                                 \
+        }                                                                     \
+    }
     if (kbeg < kend) {
-        QML_BF3_RUN
-        if constexpr (MODE == MODE_RIGHT) {
-            // second term: Li rows I against P rows J (each sweep ends on a
-            // barrier, so the stage buffers are free)
-            operands(Bhi, Ahi);
+        if constexpr (MODE == MODE_RIGHT && NST == 1) {
+            if (active) QML_BF3_SWEEP1_2T(true)
+            else QML_BF3_SWEEP1_2T(false)
+        } else {
             QML_BF3_RUN
+            static_assert(MODE != MODE_RIGHT || NST == 1, "the right product's two terms need one stage");
         }
     }
 #undef QML_BF3_GLOAD
@@ -625,6 +645,8 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 #undef QML_BF3_SWEEP
 #undef QML_BF3_SWEEP1
 #undef QML_BF3_RUN
+#undef QML_BF3_SWEEP1_2T
+#undef QML_BF3_GLOADX
     if (!active) return;
     // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
     const int orow = lane % 16, ocol = 4 * (lane / 16);

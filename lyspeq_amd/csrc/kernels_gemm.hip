@@ -581,23 +581,24 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         }                                                                     \
     }
 // Single LDS stage: stage s+1 is loaded into registers while stage s is
-// multiplied, then stored once every wave has read stage s.
+// multiplied, then stored once every wave has read stage s.  The last stage
+// is multiplied outside the loop, so no load is left in flight into the
+// epilogue.
 #define QML_BF3_SWEEP1(DO_MFMA)                                               \
     {                                                                         \
         const int ns = (kend - kbeg) / BK;                                    \
-        const int klast = kend - BK;                                          \
         QML_BF3_GLOAD(P, kbeg)                                                \
         QML_BF3_SSTORE(P, 0)                                                  \
         __syncthreads();                                                      \
-        for (int s = 0; s < ns; ++s) {                                        \
-            QML_BF3_GLOAD(P, min(kbeg + (s + 1) * BK, klast))                 \
+        for (int s = 0; s + 1 < ns; ++s) {                                    \
+            QML_BF3_GLOAD(P, kbeg + (s + 1) * BK)                             \
             if (DO_MFMA) QML_BF3_MFMA(0)                                      \
             __syncthreads();                                                  \
-            if (s + 1 < ns) {                                                 \
-                QML_BF3_SSTORE(P, 0)                                          \
-                __syncthreads();                                              \
-            }                                                                 \
+            QML_BF3_SSTORE(P, 0)                                              \
+            __syncthreads();                                                  \
         }                                                                     \
+        if (DO_MFMA) QML_BF3_MFMA(0)                                          \
+        __syncthreads();                                                      \
     }
 #define QML_BF3_RUN                                                           \
     if constexpr (NST == 1) {                                                 \
@@ -617,18 +618,18 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
         QML_BF3_GLOAD(P, kbeg)                                                \
         QML_BF3_SSTORE(P, 0)                                                  \
         __syncthreads();                                                      \
-        for (int s = 0; s < 2 * ns; ++s) {                                    \
-            const int sn = min(s + 1, 2 * ns - 1);                            \
+        for (int s = 0; s + 1 < 2 * ns; ++s) {                                \
+            const int sn = s + 1;                                             \
             const bool second = sn >= ns;                                     \
             const __bf16 *xa = second ? Bhi : Ahi, *xb = second ? Ahi : Bhi;  \
             QML_BF3_GLOADX(P, xa, xb, kbeg + (second ? sn - ns : sn) * BK)    \
             if (DO_MFMA) QML_BF3_MFMA(0)                                      \
             __syncthreads();                                                  \
-            if (s + 1 < 2 * ns) {                                             \
-                QML_BF3_SSTORE(P, 0)                                          \
-                __syncthreads();                                              \
-            }                                                                 \
+            QML_BF3_SSTORE(P, 0)                                              \
+            __syncthreads();                                                  \
         }                                                                     \
+        if (DO_MFMA) QML_BF3_MFMA(0)                                          \
+        __syncthreads();                                                      \
     }
     if (kbeg < kend) {
         if constexpr (MODE == MODE_RIGHT && NST == 1) {

@@ -519,8 +519,11 @@ template <> struct XStore4<__bf16> {
             h[u] = (__bf16)v[u];
             l[u] = (__bf16)(v[u] - (float)h[u]);
         }
-        *(bf4 *)(X + 2 * j * msz + off) = h;
-        *(bf4 *)(X + (2 * j + 1) * msz + off) = l;
+        // streaming stores: the stack is read back only after gigabytes of
+        // other traffic
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(__builtin_bit_cast(u2, h), (u2 *)(X + 2 * j * msz + off));
+        __builtin_nontemporal_store(__builtin_bit_cast(u2, l), (u2 *)(X + (2 * j + 1) * msz + off));
     }
 };
 

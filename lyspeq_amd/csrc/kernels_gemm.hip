@@ -709,7 +709,8 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 #pragma unroll
         for (int s8 = 0; s8 < 8; ++s8) {
             const int r = 8 * s8 + lane / 8, ch = lane % 8;
-            *(uint4 *)(Cp + (long long)r * ldc + 8 * ch) = *(const uint4 *)(ep + r * EP + 8 * ch);
+            typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(*(const u4 *)(ep + r * EP + 8 * ch), (u4 *)(Cp + (long long)r * ldc + 8 * ch));
         }
     }
 }

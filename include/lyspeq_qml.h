@@ -219,6 +219,29 @@ int qml_add_chunk_rmat(qml_ctx *ctx, const qml_chunk_in *chunk, const qml_rmat *
 int qml_rmat_deconvolve(int size, int ndiags, double m, double *values);
 int qml_rmat_oversample(int size, int ndiags, const double *values, int oversampling, double *rows);
 
+/* The S/Q lookup tables of (R, dv) pairs, computed on the device: the work of
+ * SQLookupTable::computeTables (core/sq_table.cpp:201-323), i.e. of the
+ * reference's table generator, for many pairs at once.  Grids and layouts as
+ * there and as qml_add_table takes them: v_n = n L_V / (N_V - 1); z_m = z_c0 -
+ * dz + m dz (nz + 1) / (N_Z - 1) from the bins of qml_set_bins (call it first).
+ *   deriv : npairs x nk x N_V, Q_kn(v) = (1/pi) int_band W(k)^2 cos(k v) dk with
+ *           W = sinc(k dv/2) exp(-(k R)^2/2), R = c / R_fwhm / 2.35482004503 km/s
+ *           (core/fiducial_cosmology.cpp:305-331); Filon's rule over
+ *           2 filon_panels panels per band (the reference: QAWO);
+ *   signal: npairs x N_V x N_Z, the reference's real-FFT evaluation of
+ *           W^2 P_fid(k, z) (core/sq_table.cpp:221-227,296-311), P_fid the
+ *           PD13 fit with Lorentzian (core/fiducial_cosmology.cpp:273-282).
+ * Either output may be NULL. */
+typedef struct {
+    int nv;             /* NUMBER_OF_VELOCITY_POINTS (N_V >= 2) */
+    double v_length;    /* VELOCITY_LENGTH, km/s */
+    int nz;             /* NUMBER_OF_Z_POINTS (N_Z >= 2) */
+    double pd13[6];     /* FIDUCIAL_PD13_PARAMS: A, n, alpha, B, beta, lambda */
+    int filon_panels;   /* M: 2M Filon panels per k band, 1..128; 0 = 64 */
+} qml_table_spec;
+int qml_make_tables(qml_ctx *ctx, const qml_table_spec *spec, int npairs, const int *R_fwhm, const double *dv_kms,
+                    double *deriv, double *signal);
+
 /* One Newton-Raphson iteration's chunk loop.  theta is TOTAL_KZ_BINS values
  * (the ps_estimate argument of Chunk::oneQSOiteration).  Adds every chunk's
  * unhalved Fisher matrix into the upper triangle of fisher_sum (T x T,

@@ -955,4 +955,77 @@ int qml_set_chunk_active(qml_ctx *c, int id, int active) {
 int qml_num_chunks(qml_ctx *c) { return c ? (int)c->chunks.size() : 0; }
 int qml_total_kz_bins(qml_ctx *c) { return c ? c->T : 0; }
 
+int qml_make_tables(qml_ctx *c, const qml_table_spec *s, int npairs, const int *R_fwhm, const double *dv_kms,
+                    double *deriv, double *signal) {
+    if (!c || !s || npairs < 0 || (npairs > 0 && (!R_fwhm || !dv_kms)))
+        return fail(QML_ERR_ARG, "qml_make_tables: null argument");
+    if (c->kedges.empty()) return fail(QML_ERR_ARG, "qml_make_tables: set the bins first");
+    if (s->nv < 2 || s->nz < 2 || !(s->v_length > 0) || s->filon_panels < 0 || s->filon_panels > 128)
+        return fail(QML_ERR_ARG, "qml_make_tables: invalid table grid");
+    for (int p = 0; p < npairs; ++p)
+        if (R_fwhm[p] <= 0 || !(dv_kms[p] > 0)) return fail(QML_ERR_ARG, "qml_make_tables: invalid (R, dv) pair");
+    if (npairs == 0 || (!deriv && !signal)) return QML_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const int nk = (int)c->kedges.size() - 1, nv = s->nv, nz = s->nz;
+    const int M = s->filon_panels > 0 ? s->filon_panels : 64;
+    const double dxv = s->v_length / (nv - 1.0);
+    // the signal grid (core/sq_table.cpp:221-227)
+    const long long nrft = 1LL << ((int)std::ceil(std::log2(4.0 * nv)) + 1);
+    const int nhalf = (int)(nrft / 2), nmode = nhalf + 1;
+    const double length = dxv / 2.0 * (double)nrft, dk = 2.0 * 3.14159265358979323846 / length;
+    const double z1 = c->zcenters[0] - c->bins.z_bin_width;
+    const double dz = c->bins.z_bin_width * (c->bins.nz + 1) / (nz - 1.0);
+    const PD13 pd{s->pd13[0], s->pd13[1], s->pd13[2], s->pd13[3], s->pd13[4], s->pd13[5]};
+    // pairs per pass: outputs of up to 2 GiB on the device
+    const size_t per = (size_t)8 * ((deriv ? (size_t)nk * nv : 0) + (signal ? (size_t)nv * nz : 0));
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>(npairs, ((size_t)2 << 30) / per));
+    struct Bufs {
+        std::vector<void *> p;
+        ~Bufs() { for (void *x : p) hipFree(x); }
+    } bufs;
+    auto dalloc = [&](void **ptr, size_t bytes) {
+        const hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess) bufs.p.push_back(*ptr);
+        return e;
+    };
+    double *d_pairs = nullptr, *d_ke = nullptr, *d_pk = nullptr, *d_q = nullptr, *d_s = nullptr;
+    int *d_mend = nullptr;
+    HIPCHK(dalloc((void **)&d_pairs, (size_t)2 * nb * sizeof(double)));
+    HIPCHK(dalloc((void **)&d_mend, (size_t)nb * sizeof(int)));
+    HIPCHK(dalloc((void **)&d_ke, (size_t)(nk + 1) * sizeof(double)));
+    if (deriv) HIPCHK(dalloc((void **)&d_q, (size_t)nb * nk * nv * sizeof(double)));
+    if (signal) {
+        HIPCHK(dalloc((void **)&d_pk, (size_t)nz * nmode * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_s, (size_t)nb * nv * nz * sizeof(double)));
+    }
+    HIPCHK(hipMemcpy(d_ke, c->kedges.data(), (nk + 1) * sizeof(double), hipMemcpyHostToDevice));
+    if (signal) launch_pfid_grid(c->st, dk, nmode, z1, dz, nz, pd, d_pk);
+    std::vector<double> hp(2 * (size_t)nb);
+    std::vector<int> hm(nb);
+    for (int p0 = 0; p0 < npairs; p0 += nb) {
+        const int n = std::min(nb, npairs - p0);
+        for (int i = 0; i < n; ++i) {
+            const double R = SPEED_OF_LIGHT / R_fwhm[p0 + i] / 2.35482004503;
+            hp[2 * i] = R;
+            hp[2 * i + 1] = dv_kms[p0 + i];
+            // modes with W^2 < exp(-92) ~ 1e-40 of W^2(0) are left out
+            hm[i] = (int)std::min<double>(nhalf, std::ceil(9.6 / (R * dk)));
+        }
+        HIPCHK(hipMemcpyAsync(d_pairs, hp.data(), 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(d_mend, hm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->st));
+        if (deriv) launch_deriv_tables(c->st, d_pairs, n, d_ke, nk, nv, dxv, M, d_q);
+        if (signal)
+            launch_signal_tables(c->st, d_pairs, d_mend, n, d_pk, nmode, nhalf, dk, dxv, 1.0 / length, nv, nz, d_s);
+        HIPCHK(hipGetLastError());
+        if (deriv)
+            HIPCHK(hipMemcpyAsync(deriv + (size_t)p0 * nk * nv, d_q, (size_t)n * nk * nv * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->st));
+        if (signal)
+            HIPCHK(hipMemcpyAsync(signal + (size_t)p0 * nv * nz, d_s, (size_t)n * nv * nz * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return QML_OK;
+}
+
 }  // extern "C"

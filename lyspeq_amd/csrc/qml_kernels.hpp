@@ -75,4 +75,14 @@ void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, 
 void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram);
 
+// S/Q lookup tables (kernels_tables.hip)
+struct PD13 {
+    double A, n, alpha, B, beta, lambda;
+};
+void launch_deriv_tables(hipStream_t st, const double *pairs, int npairs, const double *kedges, int nk, int nv,
+                         double dxv, int M, double *out);
+void launch_pfid_grid(hipStream_t st, double dk, int nmode, double z1, double dz, int nz, PD13 pd, double *pk);
+void launch_signal_tables(hipStream_t st, const double *pairs, const int *mend, int npairs, const double *pk,
+                          int nmode, int nhalf, double dk, double dxv, double inv_len, int nv, int nz, double *out);
+
 }  // namespace qml

@@ -108,7 +108,12 @@ _lib = None
 EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
            "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
-           "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample"]
+           "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables"]
+
+
+class _TableSpec(C.Structure):
+    _fields_ = [("nv", C.c_int), ("v_length", C.c_double), ("nz", C.c_int), ("pd13", C.c_double * 6),
+                ("filon_panels", C.c_int)]
 
 
 def load_library(path: str = LIB_PATH):
@@ -130,6 +135,7 @@ def load_library(path: str = LIB_PATH):
     lib.qml_add_chunk.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(ChunkInfo), C.POINTER(C.c_int)]
     lib.qml_rmat_deconvolve.argtypes = [C.c_int, C.c_int, C.c_double, dp]
     lib.qml_rmat_oversample.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp]
+    lib.qml_make_tables.argtypes = [vp, C.POINTER(_TableSpec), C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_add_chunk_rmat.argtypes = [vp, C.POINTER(_ChunkIn), C.POINTER(_RMat), C.POINTER(ChunkInfo),
                                        C.POINTER(C.c_int)]
     lib.qml_commit.argtypes = [vp]
@@ -293,6 +299,23 @@ class QMLEngine:
                                        _dptr(deriv), sp, C.byref(r)), self._lib)
         self.tables.append((int(R_fwhm), float(dv_kms)))
         return r.value
+
+    def make_tables(self, pairs, nv: int, v_length: float, nz: int, pd13, filon_panels: int = 0,
+                    deriv: bool = True, signal: bool = True):
+        """qml_make_tables: the S/Q tables of the (R_fwhm, dv) pairs on the
+        device, as (deriv (npairs, nk, nv), signal (npairs, nv, nz)); an
+        output not asked for is None."""
+        n = len(pairs)
+        R = np.ascontiguousarray([int(r) for r, _ in pairs], dtype=np.int32)
+        dv = np.ascontiguousarray([float(d) for _, d in pairs], dtype=np.float64)
+        q = np.empty((n, self.binning.nk, nv)) if deriv else None
+        s = np.empty((n, nv, nz)) if signal else None
+        spec = _TableSpec(int(nv), float(v_length), int(nz), (C.c_double * 6)(*[float(x) for x in pd13]),
+                          int(filon_panels))
+        _check(self._lib.qml_make_tables(self._ctx, C.byref(spec), n, R.ctypes.data_as(C.POINTER(C.c_int)),
+                                         _dptr(dv), _dptr(q) if deriv else None, _dptr(s) if signal else None),
+               self._lib)
+        return q, s
 
     def add_chunk(self, wave, delta, ivar, R_fwhm: int):
         w = np.ascontiguousarray(wave, dtype=np.float64)

@@ -759,28 +759,42 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
         }
         __syncthreads();
     }
-    // the noise (H) and signal (T) members, in FP64 arithmetic
+    // the noise (H) and signal (T) members: their At rows and columns staged
+    // in LDS like the bands' (every wave is past the last band's barrier),
+    // arithmetic in CT; elements outside the chunk are exact zeros
+    const int nht = 2 * nu * (TC + TR);
+    for (int f = tid; f < nht; f += 256) {
+        const int h = f / (nu * (TC + TR)), x = (f / (TC + TR)) % nu, p = f % (TC + TR);
+        atv[0][f] = (CT)aux[c.at_off + ((size_t)(c.n + h) * MAX_NU + x) * Np + (p < TC ? b0 + p : i0 + p - TC)];
+    }
+    __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int j = c.n + h;
-        const double *At = aux + c.at_off + (size_t)j * MAX_NU * Np;
+        const CT *ab = &atv[0][h * nu * (TC + TR)];
+        CT ac[4][NUM > 0 ? NUM : 1];
+#pragma unroll
+        for (int x = 0; x < NUM; ++x)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ac[u][x] = ab[x * (TC + TR) + 4 * cg + u];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int i = i0 + rl + 16 * e;
+            CT ar[NUM > 0 ? NUM : 1];
+#pragma unroll
+            for (int x = 0; x < NUM; ++x) ar[x] = ab[x * (TC + TR) + TC + rl + 16 * e];
             CT val[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int b = bb + u;
-                double vv = 0.0;
-                if (i < N && b < N) {
-                    const double base = (h == 0) ? ((i == b) ? Npr[b] : 0.0) : S[(size_t)i * Np + b];
-                    double corr = 0.0;
-                    for (int x = 0; x < nu; ++x)
-                        corr += U[x * (size_t)Np + i] * At[(size_t)x * Np + b] + At[(size_t)x * Np + i] * U[x * (size_t)Np + b];
-                    const double tf = (!UPPER || b > i) ? 1.0 : ((b == i) ? 0.5 : 0.0);
-                    vv = D[i] * D[b] * tf * (base - corr);
-                }
-                val[u] = (CT)vv;
+                const bool ok = (okm >> (4 * e + u)) & 1u;
+                double base = 0.0;
+                if (h == 0) base = (ok && i == b) ? Npr[b] : 0.0;
+                else base = ok ? S[(size_t)i * Np + b] : 0.0;
+                CT acc = (CT)base;
+#pragma unroll
+                for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
+                val[u] = sc[e][u] * acc;
             }
             XStore4<OutT>::put(Xc, msz, j, (size_t)i * Np + bb, val);
         }

@@ -73,7 +73,7 @@ def kernel_work(shapes, prec):
     return w
 
 
-def run_engine(w, precision, steps, warmup, scratch_gb):
+def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=()):
     eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
@@ -81,6 +81,15 @@ def run_engine(w, precision, steps, warmup, scratch_gb):
     log(f"[{precision}] {len(accepted)} chunks registered and committed in {time.perf_counter() - t0:.1f} s")
     T = eng.T
     theta = np.zeros(T)
+    schedule = {"max_batch_chunks": 256, "autotuned": False}
+    if tune_caps:
+        t0 = time.perf_counter()
+        cap, tms = eng.autotune(theta, caps=tune_caps, reps=1)
+        schedule = {"max_batch_chunks": cap, "autotuned": True, "candidates": list(tune_caps),
+                    "tuned_ms": tms, "tuning_s": time.perf_counter() - t0}
+        log(f"[{precision}] autotuned batch cap {cap} ({tms:.1f} ms) in {schedule['tuning_s']:.1f} s")
+    if graph:
+        eng.set_graph(True)
     for i in range(warmup):
         F, d = np.zeros((T, T)), np.zeros((3, T))
         eng.iterate(theta, F, d)
@@ -94,6 +103,7 @@ def run_engine(w, precision, steps, warmup, scratch_gb):
         log(f"[{precision}] step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms")
     # one more (untimed) iteration with the two stages serialised on one
     # stream: its per-kernel event durations are those of each kernel alone
+    eng.set_graph(False)
     eng.set_pipeline(False)
     F2, d2 = np.zeros((T, T)), np.zeros((3, T))
     t0 = time.perf_counter()
@@ -106,7 +116,8 @@ def run_engine(w, precision, steps, warmup, scratch_gb):
     tm = eng.timings()
     shapes = chunk_shapes(eng)
     eng.close()
-    return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms)
+    return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms,
+                schedule=schedule)
 
 
 def copy_calibration(mb: int):
@@ -179,6 +190,11 @@ def main():
     ap.add_argument("--config", default="desi", choices=list(synth.CONFIGS))
     ap.add_argument("--nchunks", type=int, default=0)
     ap.add_argument("--scratch-gb", type=float, default=96.0)
+    ap.add_argument("--launch", default="eager", choices=["eager", "graph"],
+                    help="eager launches, or the iteration captured once into a HIP graph and replayed")
+    ap.add_argument("--autotune", default="", metavar="CAPS",
+                    help="comma-separated caps on chunks per batch for the autotuned schedule "
+                         "(tuned before the warm-up, outside the timed steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -198,21 +214,25 @@ def main():
     log(f"workload {w.name}: {len(w.chunks)} chunks, {len(w.pairs)} (R,dv) tables, "
         f"generated in {time.perf_counter() - t0:.1f} s")
 
+    caps = tuple(int(x) for x in args.autotune.split(",") if x.strip())
+    opt = dict(graph=args.launch == "graph", tune_caps=caps)
     res64 = None
     if not args.no_fp64:
-        res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb)
-    resm = run_engine(w, args.precision, args.steps, args.warmup, args.scratch_gb)
+        res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)
+    resm = run_engine(w, args.precision, args.steps, args.warmup, args.scratch_gb, **opt)
 
     nch = resm["n"]
     ms = 1e3 * float(np.mean(resm["times"]))
     value = nch / (ms / 1e3)
     cfg = {"workload": f"{w.name}: {w.description}", "chunks": nch, "tables": len(w.pairs),
-           "precision": args.precision, "theta": "0 (first iteration)"}
+           "precision": args.precision, "theta": "0 (first iteration)", "launch": args.launch,
+           "schedule": resm["schedule"]}
     if res64:
         ms64 = 1e3 * float(np.mean(res64["times"]))
         cfg["fp64_chunks_per_s"] = nch / (ms64 / 1e3)
         cfg["fp64_ms_per_step"] = ms64
         cfg["speedup_vs_fp64"] = ms64 / ms
+        cfg["fp64_schedule"] = res64["schedule"]
         F64, d64 = res64["F"], res64["dbt"]
         Fm, dm = resm["F"], resm["dbt"]
         dF = np.sqrt(np.abs(np.diag(F64)))

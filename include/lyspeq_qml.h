@@ -270,6 +270,29 @@ int qml_set_chunk_active(qml_ctx *ctx, int chunk_id, int active);
  * serial per MPI rank). */
 int qml_set_pipeline(qml_ctx *ctx, int on);
 
+/* Graph replay of qml_iterate (default off).  On, the next qml_iterate
+ * captures its whole launch sequence (the theta upload, both stages of every
+ * batch on their two streams, the accumulation and the copy-out of the sums)
+ * into a HIP graph, and every later iteration replays that graph with the new
+ * theta, so the host issues one launch per iteration.  The graph is captured
+ * again when qml_set_pipeline changes the stage overlap, and released by
+ * qml_set_graph(ctx, 0) or qml_destroy.  Results are identical to those of
+ * eager launches.  A replayed iteration reports total_ms only in its
+ * qml_timings (the per-stage events are not recorded).  No counterpart in the
+ * reference. */
+int qml_set_graph(qml_ctx *ctx, int on);
+
+/* Autotuned batch schedule.  For each candidate cap on chunks per batch
+ * (caps[0..ncaps-1]), the committed chunks are planned again into
+ * size-bucketed batches (sorted by padded size, cut at the cap or at the
+ * scratch budget of qml_opts), and reps + 1 iterations are run at theta, the
+ * first as warm-up; the schedule with the shortest iteration is kept and its
+ * cap and time (ms) are returned.  The per-chunk results do not depend on the
+ * schedule.  The accumulated sums of these iterations are discarded.  No
+ * counterpart in the reference (its chunk loop is serial per MPI rank). */
+int qml_autotune(qml_ctx *ctx, const double *theta, const int *caps, int ncaps, int reps, int *best_cap,
+                 double *best_ms);
+
 /* Number of chunks registered, and TOTAL_KZ_BINS. */
 int qml_num_chunks(qml_ctx *ctx);
 int qml_total_kz_bins(qml_ctx *ctx);

@@ -876,7 +876,9 @@ __global__ __launch_bounds__(256) void k_gram_finalize(const ChunkDev *cs, const
 
 // --------------------------------------------------------------- accumulate --
 // fisher_sum[(T+1) fis + j + i T] += F_chunk[j + i nq] for j >= i, and the
-// d/b/t vectors, in chunk-id order (core/chunk_estimate.cpp:893-903).
+// d/b/t vectors, in chunk-id order (core/chunk_estimate.cpp:893-903).  The
+// sums start from zero and are written, not added: the iteration needs no
+// clearing of Fsum or dbt_sum.  The lower triangle of Fsum is never written.
 __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const int *status, const int *skip,
                                                     int nchunks,
                                                     int T, const double *fc, const double *dbt,
@@ -884,7 +886,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
     // each thread owns fixed columns of row a, so the sums run in chunk order
     const int a = blockIdx.x;
     for (int b = a + threadIdx.x; b < T; b += 256) {
-        double s = Fsum[(size_t)a * T + b];
+        double s = 0.0;
         for (int ci = 0; ci < nchunks; ++ci) {
             const ChunkOut o = outs[ci];
             if (status[ci] || skip[ci] || a < o.fis || b >= o.fis + o.nq) continue;
@@ -894,7 +896,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
     }
     if (threadIdx.x < 3) {
         const int k = threadIdx.x;
-        double s = dbt_sum[k * T + a];
+        double s = 0.0;
         for (int ci = 0; ci < nchunks; ++ci) {
             const ChunkOut o = outs[ci];
             if (status[ci] || skip[ci] || a < o.fis || a >= o.fis + o.nq) continue;

@@ -156,9 +156,30 @@ struct qml_ctx {
     double *d_rin = nullptr;   // resolution-matrix rows and fine grids
     std::vector<Batch> batches;
     hipEvent_t ev_begin{}, ev_end{}, ev_acc{};
+    hipEvent_t ev_fork{}, ev_join{};   // order stage B's stream inside one iteration
     qml_timings tm{};
-    std::vector<double> hF, hdbt;
+    // pinned host staging of theta and of the sums: the iteration's copies
+    // read and write fixed addresses, so a captured sequence can replay them
+    double *h_theta = nullptr, *h_F = nullptr, *h_dbt = nullptr;
+    // qml_set_graph: the iteration's launch sequence captured once and
+    // replayed; dropped when the stage overlap setting changes
+    int use_graph = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_pipeline = -1;
 };
+
+namespace {
+void drop_graph(qml_ctx *c) {
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_pipeline = -1;
+}
+void release_plan(qml_ctx *c);
+int plan_batches(qml_ctx *c);
+}  // namespace
 
 extern "C" {
 
@@ -188,7 +209,9 @@ int qml_create(qml_ctx **out, const qml_opts *opts) {
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
         hipStreamCreate(&c->st2) != hipSuccess ||
         hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
-        hipEventCreate(&c->ev_acc) != hipSuccess) {
+        hipEventCreate(&c->ev_acc) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return fail(QML_ERR_HIP, "qml_create: stream/event creation failed");
     }
@@ -202,22 +225,17 @@ int qml_destroy(qml_ctx *c) {
     if (c->st) hipStreamSynchronize(c->st);
     if (c->st2) hipStreamSynchronize(c->st2);
     for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_q2); hipFree(t.d_s); }
-    for (auto &b : c->batches) {
-        hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
-        for (auto &e : b.evA) if (e) hipEventDestroy(e);
-        for (auto &e : b.evB) if (e) hipEventDestroy(e);
-        if (b.evR) hipEventDestroy(b.evR);
-    }
-    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip, c->d_gram,
-                    c->d_X, c->d_Y, c->d_theta, c->d_F, c->d_dbtsum, c->d_rin};
+    release_plan(c);
+    void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip,
+                    c->d_theta, c->d_F, c->d_dbtsum, c->d_rin};
     for (void *p : ptrs) if (p) hipFree(p);
-    for (int k = 0; k < 2; ++k) {
-        void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
-        for (void *p : set) if (p) hipFree(p);
-    }
     if (c->ev_begin) hipEventDestroy(c->ev_begin);
     if (c->ev_end) hipEventDestroy(c->ev_end);
     if (c->ev_acc) hipEventDestroy(c->ev_acc);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    double *pinned[] = {c->h_theta, c->h_F, c->h_dbt};
+    for (double *p : pinned) if (p) hipHostFree(p);
     if (c->st) hipStreamDestroy(c->st);
     if (c->st2) hipStreamDestroy(c->st2);
     delete c;
@@ -513,95 +531,39 @@ int qml_add_chunk_rmat(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm, q
     return add_chunk_impl(c, in, rm, info, chunk_id);
 }
 
-int qml_commit(qml_ctx *c) {
-    if (!c || !c->bins_set) return fail(QML_ERR_ARG, "qml_commit: set bins first");
-    if (c->committed) return fail(QML_ERR_ARG, "qml_commit: already committed");
-    HIPCHK(hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+// Frees the batch plan: work lists, events, scratch arenas and the graph
+// that referenced them.
+void release_plan(qml_ctx *c) {
+    drop_graph(c);
+    for (auto &b : c->batches) {
+        hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
+        for (auto &e : b.evA) if (e) hipEventDestroy(e);
+        for (auto &e : b.evB) if (e) hipEventDestroy(e);
+        if (b.evR) hipEventDestroy(b.evR);
+    }
+    c->batches.clear();
+    for (int k = 0; k < 2; ++k) {
+        void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
+        for (void *p : set) if (p) hipFree(p);
+        c->d_mat[k] = c->d_vec[k] = c->d_pj[k] = c->d_sj[k] = nullptr;
+        c->d_lic[k] = nullptr;
+    }
+    void *rest[] = {c->d_gram, c->d_X, c->d_Y};
+    for (void *p : rest) if (p) hipFree(p);
+    c->d_gram = nullptr;
+    c->d_X = c->d_Y = nullptr;
+}
+
+// Size-bucketed batch plan: chunks sorted by padded size (then stored Q
+// count), descending, are cut into batches of at most c->max_batch chunks
+// whose scratch fits c->scratch_gb; allocates the arenas of the largest
+// batch and uploads every batch's chunk descriptors and work lists.
+int plan_batches(qml_ctx *c) {
     const int nch = (int)c->chunks.size();
     const size_t es = (c->prec == 0) ? 8 : 4;
-    // tables
-    std::vector<TableDev> td(c->tables.size());
-    for (size_t r = 0; r < c->tables.size(); ++r) {
-        const auto &t = c->tables[r];
-        td[r].q = t.d_q;
-        td[r].qt = t.d_qt;
-        td[r].q2 = t.d_q2;
-        td[r].s = t.d_s;
-        td[r].nv = t.nv;
-        td[r].nz = t.nz;
-        td[r].v1 = 0.0;
-        td[r].dv = t.v_length / (t.nv - 1.0);
-        td[r].z1 = c->zcenters[0] - c->bins.z_bin_width;
-        td[r].dz = c->bins.z_bin_width * (c->bins.nz + 1) / (t.nz - 1.0);
-    }
-    if (!td.empty()) {
-        HIPCHK(hipMalloc(&c->d_tables, td.size() * sizeof(TableDev)));
-        HIPCHK(hipMemcpy(c->d_tables, td.data(), td.size() * sizeof(TableDev), hipMemcpyHostToDevice));
-    }
-    // inputs and persistent outputs
-    size_t in_total = 0, fc_total = 0, dbt_total = 0;
-    c->outs.resize(nch);
-    for (int i = 0; i < nch; ++i) {
-        auto &ch = c->chunks[i];
-        ch.Np = (int)align_up(ch.N, TILE);
-        ch.m = ch.n + 2;
-        ch.in_off = (long long)in_total;
-        in_total += (size_t)IN_SLOTS * ch.Np;
-        c->outs[i].fc_off = (long long)fc_total;
-        c->outs[i].dbt_off = (long long)dbt_total;
-        c->outs[i].nq = ch.nq;
-        c->outs[i].fis = ch.fis;
-        fc_total += (size_t)ch.nq * ch.nq;
-        dbt_total += (size_t)3 * ch.nq;
-    }
-    std::vector<double> hin(std::max<size_t>(in_total, 1), 0.0);
-    for (int i = 0; i < nch; ++i) {
-        const auto &ch = c->chunks[i];
-        double *base = hin.data() + ch.in_off;
-        const size_t Np = ch.Np;
-        for (int p = 0; p < ch.Np; ++p) {
-            const bool real = p < ch.N;
-            base[IN_WAVE * Np + p] = real ? ch.wave[p] : ch.wave[ch.N - 1];
-            base[IN_DELTA * Np + p] = real ? ch.delta[p] : 0.0;
-            const double iv = real ? ch.ivar[p] : 0.0;
-            base[IN_IVAR * Np + p] = iv;
-            base[IN_D * Np + p] = std::sqrt(iv);
-            base[IN_NPRIME * Np + p] = (iv != 0) ? 1.0 / iv : 0.0;
-            for (int t = 0; t < ch.nu; ++t) base[(IN_U0 + t) * Np + p] = real ? ch.U[(size_t)t * ch.N + p] : 0.0;
-        }
-    }
-    HIPCHK(hipMalloc(&c->d_in, hin.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(c->d_in, hin.data(), hin.size() * sizeof(double), hipMemcpyHostToDevice));
-    {   // resolution-matrix rows and grids
-        size_t rin_total = 0;
-        for (auto &ch : c->chunks)
-            if (ch.rm) {
-                ch.rin_off = (long long)rin_total;
-                rin_total += ch.rband.size() + ch.fine.size();
-            }
-        if (rin_total) {
-            std::vector<double> hr(rin_total);
-            for (const auto &ch : c->chunks)
-                if (ch.rm) {
-                    std::copy(ch.rband.begin(), ch.rband.end(), hr.begin() + ch.rin_off);
-                    std::copy(ch.fine.begin(), ch.fine.end(), hr.begin() + ch.rin_off + ch.rband.size());
-                }
-            HIPCHK(hipMalloc(&c->d_rin, rin_total * sizeof(double)));
-            HIPCHK(hipMemcpy(c->d_rin, hr.data(), rin_total * sizeof(double), hipMemcpyHostToDevice));
-        }
-    }
-    HIPCHK(hipMalloc(&c->d_outs, std::max(nch, 1) * sizeof(ChunkOut)));
-    if (nch) HIPCHK(hipMemcpy(c->d_outs, c->outs.data(), nch * sizeof(ChunkOut), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&c->d_fc, std::max<size_t>(fc_total, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_dbt, std::max<size_t>(dbt_total, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_status, std::max(nch, 1) * sizeof(int)));
-    HIPCHK(hipMemset(c->d_status, 0, std::max(nch, 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&c->d_skip, std::max(nch, 1) * sizeof(int)));
-    HIPCHK(hipMemset(c->d_skip, 0, std::max(nch, 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&c->d_theta, std::max(c->T, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_F, (size_t)c->T * c->T * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
-
     // per-chunk scratch sizes (elements of each arena)
     struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
     auto need = [&](const ChunkHost &ch) {
@@ -790,22 +752,133 @@ int qml_commit(qml_ctx *c) {
         HIPCHK(hipEventCreate(&B.evR));
         c->batches.push_back(B);
     }
-    c->hF.assign((size_t)c->T * c->T, 0.0);
-    c->hdbt.assign((size_t)3 * c->T, 0.0);
+    return QML_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qml_commit(qml_ctx *c) {
+    if (!c || !c->bins_set) return fail(QML_ERR_ARG, "qml_commit: set bins first");
+    if (c->committed) return fail(QML_ERR_ARG, "qml_commit: already committed");
+    HIPCHK(hipSetDevice(c->device));
+    const int nch = (int)c->chunks.size();
+    const size_t es = (c->prec == 0) ? 8 : 4;
+    // tables
+    std::vector<TableDev> td(c->tables.size());
+    for (size_t r = 0; r < c->tables.size(); ++r) {
+        const auto &t = c->tables[r];
+        td[r].q = t.d_q;
+        td[r].qt = t.d_qt;
+        td[r].q2 = t.d_q2;
+        td[r].s = t.d_s;
+        td[r].nv = t.nv;
+        td[r].nz = t.nz;
+        td[r].v1 = 0.0;
+        td[r].dv = t.v_length / (t.nv - 1.0);
+        td[r].z1 = c->zcenters[0] - c->bins.z_bin_width;
+        td[r].dz = c->bins.z_bin_width * (c->bins.nz + 1) / (t.nz - 1.0);
+    }
+    if (!td.empty()) {
+        HIPCHK(hipMalloc(&c->d_tables, td.size() * sizeof(TableDev)));
+        HIPCHK(hipMemcpy(c->d_tables, td.data(), td.size() * sizeof(TableDev), hipMemcpyHostToDevice));
+    }
+    // inputs and persistent outputs
+    size_t in_total = 0, fc_total = 0, dbt_total = 0;
+    c->outs.resize(nch);
+    for (int i = 0; i < nch; ++i) {
+        auto &ch = c->chunks[i];
+        ch.Np = (int)align_up(ch.N, TILE);
+        ch.m = ch.n + 2;
+        ch.in_off = (long long)in_total;
+        in_total += (size_t)IN_SLOTS * ch.Np;
+        c->outs[i].fc_off = (long long)fc_total;
+        c->outs[i].dbt_off = (long long)dbt_total;
+        c->outs[i].nq = ch.nq;
+        c->outs[i].fis = ch.fis;
+        fc_total += (size_t)ch.nq * ch.nq;
+        dbt_total += (size_t)3 * ch.nq;
+    }
+    std::vector<double> hin(std::max<size_t>(in_total, 1), 0.0);
+    for (int i = 0; i < nch; ++i) {
+        const auto &ch = c->chunks[i];
+        double *base = hin.data() + ch.in_off;
+        const size_t Np = ch.Np;
+        for (int p = 0; p < ch.Np; ++p) {
+            const bool real = p < ch.N;
+            base[IN_WAVE * Np + p] = real ? ch.wave[p] : ch.wave[ch.N - 1];
+            base[IN_DELTA * Np + p] = real ? ch.delta[p] : 0.0;
+            const double iv = real ? ch.ivar[p] : 0.0;
+            base[IN_IVAR * Np + p] = iv;
+            base[IN_D * Np + p] = std::sqrt(iv);
+            base[IN_NPRIME * Np + p] = (iv != 0) ? 1.0 / iv : 0.0;
+            for (int t = 0; t < ch.nu; ++t) base[(IN_U0 + t) * Np + p] = real ? ch.U[(size_t)t * ch.N + p] : 0.0;
+        }
+    }
+    HIPCHK(hipMalloc(&c->d_in, hin.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_in, hin.data(), hin.size() * sizeof(double), hipMemcpyHostToDevice));
+    {   // resolution-matrix rows and grids
+        size_t rin_total = 0;
+        for (auto &ch : c->chunks)
+            if (ch.rm) {
+                ch.rin_off = (long long)rin_total;
+                rin_total += ch.rband.size() + ch.fine.size();
+            }
+        if (rin_total) {
+            std::vector<double> hr(rin_total);
+            for (const auto &ch : c->chunks)
+                if (ch.rm) {
+                    std::copy(ch.rband.begin(), ch.rband.end(), hr.begin() + ch.rin_off);
+                    std::copy(ch.fine.begin(), ch.fine.end(), hr.begin() + ch.rin_off + ch.rband.size());
+                }
+            HIPCHK(hipMalloc(&c->d_rin, rin_total * sizeof(double)));
+            HIPCHK(hipMemcpy(c->d_rin, hr.data(), rin_total * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(hipMalloc(&c->d_outs, std::max(nch, 1) * sizeof(ChunkOut)));
+    if (nch) HIPCHK(hipMemcpy(c->d_outs, c->outs.data(), nch * sizeof(ChunkOut), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->d_fc, std::max<size_t>(fc_total, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_dbt, std::max<size_t>(dbt_total, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_status, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMemset(c->d_status, 0, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_skip, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMemset(c->d_skip, 0, std::max(nch, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_theta, std::max(c->T, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_F, (size_t)c->T * c->T * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_dbtsum, (size_t)3 * c->T * sizeof(double)));
+
+    {
+        const int rc = plan_batches(c);
+        if (rc != QML_OK) return rc;
+    }
+    // the accumulation writes the upper triangle of the Fisher sum only
+    HIPCHK(hipMemset(c->d_F, 0, (size_t)c->T * c->T * sizeof(double)));
+    HIPCHK(hipHostMalloc((void **)&c->h_theta, std::max(c->T, 1) * sizeof(double)));
+    HIPCHK(hipHostMalloc((void **)&c->h_F, std::max<size_t>((size_t)c->T * c->T, 1) * sizeof(double)));
+    HIPCHK(hipHostMalloc((void **)&c->h_dbt, std::max(3 * c->T, 1) * sizeof(double)));
     c->committed = true;
     return QML_OK;
 }
 
-int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt_sum) {
-    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_iterate: commit first");
-    if (!theta || !fisher_sum || !dbt_sum) return fail(QML_ERR_ARG, "qml_iterate: null argument");
-    HIPCHK(hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+// Enqueues one iteration: theta from its pinned staging buffer, both stages of
+// every batch, the accumulation and the copy of the sums into pinned host
+// memory.  Stage B's stream is forked from c->st after the theta copy and
+// joined back into it at the end, so that the sequence begins and ends on
+// c->st (what a stream capture needs).  With timing every stage boundary
+// records its event; otherwise only the events that order the two streams are
+// recorded.
+int enqueue_iteration(qml_ctx *c, bool timing) {
     const int T = c->T;
     hipStream_t sB = c->pipeline ? c->st2 : c->st;   // stream of the dense-product stage
-    HIPCHK(hipEventRecord(c->ev_begin, c->st));
-    HIPCHK(hipMemcpyAsync(c->d_theta, theta, T * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemsetAsync(c->d_F, 0, (size_t)T * T * sizeof(double), sB));
-    HIPCHK(hipMemsetAsync(c->d_dbtsum, 0, (size_t)3 * T * sizeof(double), sB));
+    auto mark = [&](hipEvent_t e, hipStream_t s) { return timing ? hipEventRecord(e, s) : hipSuccess; };
+    HIPCHK(hipMemcpyAsync(c->d_theta, c->h_theta, T * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (sB != c->st) {
+        HIPCHK(hipEventRecord(c->ev_fork, c->st));
+        HIPCHK(hipStreamWaitEvent(sB, c->ev_fork, 0));
+    }
     for (size_t bi = 0; bi < c->batches.size(); ++bi) {
         Batch &B = c->batches[bi];
         const int set = (int)(bi % 2);
@@ -832,19 +905,19 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         // stage A on st: this arena set was last read by stage B of batch bi - 2
         if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
         if (c->poison) launch_poison_lds(c->st);
-        HIPCHK(hipEventRecord(B.evA[0], c->st));
+        HIPCHK(mark(B.evA[0], c->st));
         if (!launch_rmat(L)) return fail(QML_ERR_LIMIT, "qml_iterate: no sandwich kernel for the resolution matrices");
-        HIPCHK(hipEventRecord(B.evR, c->st));
+        HIPCHK(mark(B.evR, c->st));
         launch_assemble(L, c->d_theta);
-        HIPCHK(hipEventRecord(B.evA[1], c->st));
+        HIPCHK(mark(B.evA[1], c->st));
         launch_cholesky(L, c->d_status);
-        HIPCHK(hipEventRecord(B.evA[2], c->st));
+        HIPCHK(mark(B.evA[2], c->st));
         launch_trinv(L);
-        HIPCHK(hipEventRecord(B.evA[3], c->st));
+        HIPCHK(mark(B.evA[3], c->st));
         launch_solve(L);
-        HIPCHK(hipEventRecord(B.evA[4], c->st));
+        HIPCHK(mark(B.evA[4], c->st));
         launch_dpass(L);
-        HIPCHK(hipEventRecord(B.evA[5], c->st));
+        HIPCHK(mark(B.evA[5], c->st));
         launch_small(L);
         if (c->prec != 0) launch_convert_li(L, c->d_lic[set], c->prec);
         HIPCHK(hipEventRecord(B.evA[6], c->st));
@@ -855,50 +928,102 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
         const void *li = (c->prec == 0) ? (const void *)c->d_mat[set] : c->d_lic[set];
         HIPCHK(hipStreamWaitEvent(sB, B.evA[6], 0));
         if (c->poison) launch_poison_lds(sB);
-        HIPCHK(hipEventRecord(B.evB[0], sB));
+        HIPCHK(mark(B.evB[0], sB));
         launch_xgen(L2, c->d_X, c->prec);
-        HIPCHK(hipEventRecord(B.evB[1], sB));
+        HIPCHK(mark(B.evB[1], sB));
         launch_gemm_whiten_left(sB, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
-        HIPCHK(hipEventRecord(B.evB[2], sB));
+        HIPCHK(mark(B.evB[2], sB));
         launch_gemm_whiten_right(sB, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
-        HIPCHK(hipEventRecord(B.evB[3], sB));
+        HIPCHK(mark(B.evB[3], sB));
         launch_gemm_gram(sB, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
-        HIPCHK(hipEventRecord(B.evB[4], sB));
+        HIPCHK(mark(B.evB[4], sB));
         launch_gram_finalize(L2, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
         HIPCHK(hipEventRecord(B.evB[5], sB));
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipEventRecord(c->ev_acc, sB));
+    HIPCHK(mark(c->ev_acc, sB));
     launch_accumulate(sB, c->d_outs, c->d_status, c->d_skip, (int)c->chunks.size(), T, c->d_fc, c->d_dbt, c->d_F,
                       c->d_dbtsum);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->hF.data(), c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, sB));
-    HIPCHK(hipMemcpyAsync(c->hdbt.data(), c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, sB));
-    HIPCHK(hipEventRecord(c->ev_end, sB));
+    HIPCHK(hipMemcpyAsync(c->h_F, c->d_F, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost, sB));
+    HIPCHK(hipMemcpyAsync(c->h_dbt, c->d_dbtsum, (size_t)3 * T * sizeof(double), hipMemcpyDeviceToHost, sB));
+    if (sB != c->st) {
+        HIPCHK(hipEventRecord(c->ev_join, sB));
+        HIPCHK(hipStreamWaitEvent(c->st, c->ev_join, 0));
+    }
+    return QML_OK;
+}
+
+// Captures enqueue_iteration into c->graph and instantiates it.
+int capture_iteration(qml_ctx *c) {
+    drop_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_iteration(c, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (rc != QML_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(QML_ERR_HIP, std::string("qml_iterate: stream capture: ") + hipGetErrorString(e));
+    c->graph = g;
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        drop_graph(c);
+        return fail(QML_ERR_HIP, std::string("qml_iterate: graph instantiation: ") + hipGetErrorString(ei));
+    }
+    c->graph_pipeline = c->pipeline;
+    return QML_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt_sum) {
+    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_iterate: commit first");
+    if (!theta || !fisher_sum || !dbt_sum) return fail(QML_ERR_ARG, "qml_iterate: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int T = c->T;
+    // the previous iteration has completed (both streams were synchronised)
+    std::memcpy(c->h_theta, theta, (size_t)T * sizeof(double));
+    if (c->use_graph) {
+        if (!c->gexec || c->graph_pipeline != c->pipeline) {
+            const int rc = capture_iteration(c);
+            if (rc != QML_OK) return rc;
+        }
+        HIPCHK(hipEventRecord(c->ev_begin, c->st));
+        HIPCHK(hipGraphLaunch(c->gexec, c->st));
+        HIPCHK(hipEventRecord(c->ev_end, c->st));
+    } else {
+        HIPCHK(hipEventRecord(c->ev_begin, c->st));
+        const int rc = enqueue_iteration(c, true);
+        if (rc != QML_OK) return rc;
+        HIPCHK(hipEventRecord(c->ev_end, c->st));
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipStreamSynchronize(c->st2));
     for (int i = 0; i < T; ++i)
-        for (int j = i; j < T; ++j) fisher_sum[(size_t)i * T + j] += c->hF[(size_t)i * T + j];
-    for (int k = 0; k < 3 * T; ++k) dbt_sum[k] += c->hdbt[k];
+        for (int j = i; j < T; ++j) fisher_sum[(size_t)i * T + j] += c->h_F[(size_t)i * T + j];
+    for (int k = 0; k < 3 * T; ++k) dbt_sum[k] += c->h_dbt[k];
     qml_timings tm{};
-    for (auto &B : c->batches) {
-        for (int k = 0; k < 6; ++k) {
-            float ms = 0;
-            hipEventElapsedTime(&ms, k == 0 ? B.evR : B.evA[k], B.evA[k + 1]);
-            tm.kernel_ms[k] += ms;
+    if (!c->use_graph) {   // a replayed graph records no per-stage events
+        for (auto &B : c->batches) {
+            for (int k = 0; k < 6; ++k) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, k == 0 ? B.evR : B.evA[k], B.evA[k + 1]);
+                tm.kernel_ms[k] += ms;
+            }
+            {
+                float ms = 0;
+                hipEventElapsedTime(&ms, B.evA[0], B.evR);
+                tm.kernel_ms[QML_K_RMAT] += ms;
+            }
+            for (int k = 0; k < 5; ++k) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, B.evB[k], B.evB[k + 1]);
+                tm.kernel_ms[6 + k] += ms;
+            }
         }
-        {
-            float ms = 0;
-            hipEventElapsedTime(&ms, B.evA[0], B.evR);
-            tm.kernel_ms[QML_K_RMAT] += ms;
-        }
-        for (int k = 0; k < 5; ++k) {
-            float ms = 0;
-            hipEventElapsedTime(&ms, B.evB[k], B.evB[k + 1]);
-            tm.kernel_ms[6 + k] += ms;
-        }
-    }
-    {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev_acc, c->ev_end);
         tm.kernel_ms[QML_K_ACCUMULATE] = ms;
@@ -938,6 +1063,57 @@ int qml_get_timings(qml_ctx *c, qml_timings *t) {
 int qml_set_pipeline(qml_ctx *c, int on) {
     if (!c) return fail(QML_ERR_ARG, "qml_set_pipeline: null context");
     c->pipeline = on ? 1 : 0;
+    return QML_OK;
+}
+
+int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, int reps, int *best_cap,
+                 double *best_ms) {
+    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_autotune: commit first");
+    if (!theta || !caps || ncaps < 1) return fail(QML_ERR_ARG, "qml_autotune: no theta or no candidates");
+    for (int i = 0; i < ncaps; ++i)
+        if (caps[i] < 1) return fail(QML_ERR_ARG, "qml_autotune: a candidate cap below 1");
+    HIPCHK(hipSetDevice(c->device));
+    reps = std::max(reps, 1);
+    const int T = c->T;
+    std::vector<double> F((size_t)T * T), dbt((size_t)3 * T);
+    int best = -1;
+    double best_t = 0.0;
+    for (int i = 0; i < ncaps; ++i) {
+        if (i == 0 || caps[i] != c->max_batch) {
+            release_plan(c);
+            c->max_batch = caps[i];
+            const int rc = plan_batches(c);
+            if (rc != QML_OK) return rc;
+        }
+        double t = 0.0;
+        for (int r = 0; r <= reps; ++r) {   // r = 0 warms the new plan up
+            const int rc = qml_iterate(c, theta, F.data(), dbt.data());
+            if (rc != QML_OK) return rc;
+            if (r > 0 && (r == 1 || c->tm.total_ms < t)) t = c->tm.total_ms;
+        }
+        if (best < 0 || t < best_t) {
+            best = caps[i];
+            best_t = t;
+        }
+    }
+    if (c->max_batch != best) {
+        release_plan(c);
+        c->max_batch = best;
+        const int rc = plan_batches(c);
+        if (rc != QML_OK) return rc;
+    }
+    if (best_cap) *best_cap = best;
+    if (best_ms) *best_ms = best_t;
+    return QML_OK;
+}
+
+int qml_set_graph(qml_ctx *c, int on) {
+    if (!c) return fail(QML_ERR_ARG, "qml_set_graph: null context");
+    c->use_graph = on ? 1 : 0;
+    if (!c->use_graph) {
+        HIPCHK(hipSetDevice(c->device));
+        drop_graph(c);
+    }
     return QML_OK;
 }
 

@@ -108,7 +108,8 @@ _lib = None
 EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_add_table",
            "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
-           "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables"]
+           "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
+           "qml_set_graph", "qml_autotune"]
 
 
 class _TableSpec(C.Structure):
@@ -143,6 +144,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_chunk_result.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_int)]
     lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
     lib.qml_set_pipeline.argtypes = [vp, C.c_int]
+    lib.qml_set_graph.argtypes = [vp, C.c_int]
+    lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]
     lib.qml_total_kz_bins.argtypes = [vp]
@@ -367,6 +370,25 @@ class QMLEngine:
         """Stage overlap across two streams (default on); off gives per-kernel
         durations free of contention, with identical results."""
         _check(self._lib.qml_set_pipeline(self._ctx, int(bool(on))), self._lib)
+
+    def set_graph(self, on: bool):
+        """Graph replay (default off): the next iteration captures its launch
+        sequence into a HIP graph, later iterations replay it with the new
+        theta.  Results are identical; a replay reports total_ms only."""
+        _check(self._lib.qml_set_graph(self._ctx, int(bool(on))), self._lib)
+
+    def autotune(self, theta: np.ndarray, caps=(64, 128, 256, 512), reps: int = 1):
+        """Plans the batches again under each cap on chunks per batch, times
+        reps iterations (after one warm-up) at theta and keeps the fastest
+        schedule.  Returns (best cap, its ms per iteration)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        assert th.size == self.T
+        cv = (C.c_int * len(caps))(*[int(x) for x in caps])
+        best = C.c_int(0)
+        ms = C.c_double(0.0)
+        _check(self._lib.qml_autotune(self._ctx, _dptr(th), cv, len(caps), int(reps), C.byref(best),
+                                      C.byref(ms)), self._lib)
+        return best.value, ms.value
 
     def set_chunk_active(self, cid: int, active: bool):
         """Leaves a chunk out of (or back in) the accumulated sums."""

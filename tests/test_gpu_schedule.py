@@ -1,0 +1,113 @@
+"""Graph replay and the autotuned batch schedule against eager launches of the
+default schedule (needs an MI355X).  Neither changes what is computed for a
+chunk: each kernel sums every output in an order fixed by the chunk alone, so
+the results must be bit-identical."""
+import numpy as np
+import pytest
+
+import parity as P
+from lyspeq_amd import synth
+from lyspeq_amd.qml import QMLEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def _engine(w, precision, **kw):
+    eng = QMLEngine(w.binning, precision=precision, **kw)
+    accepted = synth.load_engine(eng, w, list(range(len(w.chunks))))
+    eng.commit()
+    return eng, accepted
+
+
+def _run(eng, accepted, theta):
+    T = eng.T
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(theta, F, d)
+    per = {cid: eng.chunk_result(cid) for _, cid in accepted}
+    return F, d, per
+
+
+def _same(a, b):
+    Fa, da, pa = a
+    Fb, db, pb = b
+    assert np.array_equal(Fa, Fb) and np.array_equal(da, db)
+    assert pa.keys() == pb.keys()
+    for k in pa:
+        assert np.array_equal(pa[k][0], pb[k][0]) and np.array_equal(pa[k][1], pb[k][1])
+        assert pa[k][2] == pb[k][2]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_graph_replay_matches_eager(precision):
+    """Replays with a new theta each time, eager iterations and host
+    synchronisations between them, a new capture after the stage overlap is
+    switched off, and a chunk left out of the sums after capture."""
+    _print_case(f"graph replay {precision}")
+    w = synth.desi_config(nchunks=24)
+    tha = P.theta_perturbed(w.binning, w.pd13, 0.05, 3)
+    thb = P.theta_perturbed(w.binning, w.pd13, 0.08, 4)
+    # seven chunks per batch: several batches, both arena sets, both streams
+    eng, acc = _engine(w, precision, max_batch_chunks=7)
+    ea = _run(eng, acc, tha)
+    eb = _run(eng, acc, thb)
+    assert not np.array_equal(ea[0], eb[0])
+    eng.set_graph(True)
+    _same(_run(eng, acc, tha), ea)     # capture and first replay
+    _same(_run(eng, acc, thb), eb)     # replay with another theta
+    _same(_run(eng, acc, thb), eb)
+    assert eng.timings()["total_ms"] > 0
+    eng.set_graph(False)
+    _same(_run(eng, acc, tha), ea)     # eager between replays
+    eng.set_graph(True)
+    _same(_run(eng, acc, thb), eb)     # a new capture
+    _same(_run(eng, acc, tha), ea)
+    eng.set_pipeline(False)            # captured again on one stream
+    _same(_run(eng, acc, thb), eb)
+    _same(_run(eng, acc, tha), ea)
+    eng.set_pipeline(True)
+    cid0 = acc[0][1]
+    eng.set_chunk_active(cid0, False)  # read by the replay from device memory
+    g = _run(eng, acc, tha)
+    eng.set_graph(False)
+    e = _run(eng, acc, tha)
+    _same(g, e)
+    assert not np.array_equal(g[0], ea[0])
+    eng.close()
+    print(f"graph replay {precision}: bit-identical to eager", flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_autotuned_schedule_gives_identical_results(precision):
+    _print_case(f"autotune {precision}")
+    w = synth.desi_config(nchunks=24)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 8)
+    eng, acc = _engine(w, precision)   # default cap: the 24 chunks in one batch
+    e0 = _run(eng, acc, th)
+    best, ms = eng.autotune(th, caps=(5, 24, 9), reps=1)
+    print(f"autotune {precision}: best cap {best}, {ms:.2f} ms", flush=True)
+    assert best in (5, 24, 9) and ms > 0
+    _same(_run(eng, acc, th), e0)
+    for cap in (3, 11):                # each schedule forced by a one-candidate tune
+        b, _ = eng.autotune(th, caps=(cap,), reps=1)
+        assert b == cap
+        _same(_run(eng, acc, th), e0)
+    eng.set_graph(True)                # a graph of the tuned plan
+    _same(_run(eng, acc, th), e0)
+    _same(_run(eng, acc, th), e0)
+    eng.close()
+
+
+def test_autotune_rejects_bad_candidates():
+    _print_case("autotune arguments")
+    w = synth.desi_config(nchunks=4)
+    eng, acc = _engine(w, "fp64")
+    T = eng.T
+    with pytest.raises(Exception):
+        eng.autotune(np.zeros(T), caps=(0,))
+    with pytest.raises(Exception):
+        eng.autotune(np.zeros(T), caps=())
+    eng.close()

@@ -58,6 +58,7 @@ def build(verbose: bool = False) -> str:
 
 HOST_LIB = os.path.join(HERE, "libqmlhost.so")
 HOST_DRIVER = os.path.join(ROOT, "tests", "cpp", "host_driver")
+DRIVER_CHECK = os.path.join(ROOT, "tests", "cpp", "driver_check")
 
 
 def _run(cmd, what, verbose):
@@ -76,13 +77,15 @@ def build_host(verbose: bool = False) -> str:
     link = ["-L", HERE, "-lqmlhip", "-L", rocm_lib, f"-Wl,-rpath-link,{rocm_lib}"]
     tmp = HOST_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
-          os.path.join(HERE, "host", "qml_host.cpp"), *link, "-Wl,-rpath,$ORIGIN", "-o", tmp],
+          os.path.join(HERE, "host", "qml_host.cpp"), os.path.join(HERE, "host", "qml_driver.cpp"), *link,
+          "-Wl,-rpath,$ORIGIN", "-o", tmp],
          "host library build", verbose)
     os.replace(tmp, HOST_LIB)
-    src = os.path.join(ROOT, "tests", "cpp", "host_driver.cpp")
-    if os.path.exists(src):
-        _run(["g++", "-O2", "-std=c++17", "-Wall", *inc, src, "-L", HERE, "-lqmlhost", *link,
-              "-Wl,-rpath,$ORIGIN/../../lyspeq_amd", "-o", HOST_DRIVER], "host driver build", verbose)
+    for src, exe in ((os.path.join(ROOT, "tests", "cpp", "host_driver.cpp"), HOST_DRIVER),
+                     (os.path.join(ROOT, "tests", "cpp", "driver_check.cpp"), DRIVER_CHECK)):
+        if os.path.exists(src):
+            _run(["g++", "-O2", "-std=c++17", "-Wall", *inc, src, "-L", HERE, "-lqmlhost", *link,
+                  "-Wl,-rpath,$ORIGIN/../../lyspeq_amd", "-o", exe], f"{os.path.basename(exe)} build", verbose)
     return HOST_LIB
 
 

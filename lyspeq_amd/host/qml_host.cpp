@@ -108,13 +108,27 @@ void Engine::forget(Chunk *c) {
     queue_.erase(std::remove(queue_.begin(), queue_.end(), c), queue_.end());
 }
 
+void Engine::commit() {
+    if (committed_) return;
+    check(qml_commit(ctx_));
+    committed_ = true;
+    active_.assign(chunks_.size(), 1);
+}
+
+void Engine::setGraph(bool on) { check(qml_set_graph(ctx_, on ? 1 : 0)); }
+
+int Engine::autotune(const double *ps_estimate, const std::vector<int> &caps, int reps, double *best_ms) {
+    commit();
+    int best = 0;
+    double ms = 0.0;
+    check(qml_autotune(ctx_, ps_estimate, caps.data(), (int)caps.size(), reps, &best, &ms));
+    if (best_ms) *best_ms = ms;
+    return best;
+}
+
 void Engine::flush() {
     if (queue_.empty()) return;
-    if (!committed_) {
-        check(qml_commit(ctx_));
-        committed_ = true;
-        active_.assign(chunks_.size(), 1);
-    }
+    commit();
     // only enqueued chunks enter the sums
     std::vector<int> want(chunks_.size(), 0);
     for (Chunk *c : queue_) want[c->id_] = 1;

@@ -110,6 +110,15 @@ public:
     // call commits the engine (no chunk can be added after it).
     void flush();
 
+    // Graph replay of the iterations (qml_set_graph): each flush after the
+    // first replays one captured launch sequence.  Results are unchanged.
+    void setGraph(bool on);
+    // Commits the chunks constructed so far and picks the batch schedule by
+    // timing iterations at ps_estimate under each cap on chunks per batch
+    // (qml_autotune).  No chunk can be added afterwards.  Returns the cap.
+    int autotune(const double *ps_estimate, const std::vector<int> &caps, int reps = 1,
+                 double *best_ms = nullptr);
+
     int totalKzBins() const { return bins_.totalKzBins(); }
     const Binning &binning() const { return bins_; }
     qml_ctx *ctx() { return ctx_; }
@@ -121,6 +130,7 @@ private:
     void enqueue(Chunk *c, const double *ps, std::vector<std::unique_ptr<double[]>> *dbt, double *fisher);
     void forget(Chunk *c);
     void check(int code) const;
+    void commit();
 
     Binning bins_;
     qml_ctx *ctx_ = nullptr;

@@ -10,6 +10,8 @@
 * ``power_estimates``: theta = 0.5 * solver * (d - b - t) and the Fisher
   weighted d/b/t (core/quadratic_estimate.cpp:290-318); sigma =
   sqrt(diag(inverse)) (core/quadratic_estimate.cpp:717-740).
+* ``has_converged``: the relative-change statistics and the two chi^2/dof
+  tests (core/quadratic_estimate.cpp:548-614).
 
 Pins: tests/truth/test_it1_quadratic_power_estimate_detailed.dat (ThetaP,
 ErrorP, Fisher-weighted d/b/t) and test_it1_inversefisher_matrix.dat.
@@ -75,3 +77,21 @@ def power_estimates(fisher_sum: np.ndarray, dbt_sum: np.ndarray):
     sigma = np.sqrt(np.diag(sinv))
     return dict(fisher=fs, solver=solver, inverse=sinv, theta=theta, sigma=sigma,
                 weighted=weighted, damped=damped, damp=damp, dof=dof)
+
+
+def has_converged(current, previous, inverse, fisher, dof, eps):
+    """Returns (abs_mean, abs_max, chi2_diag, chi2_full, converged)."""
+    current, previous = np.asarray(current, float), np.asarray(previous, float)
+    p1, p2 = np.abs(current), np.abs(previous)
+    keep = ~((p1 == 0) & (p2 == 0))
+    r = np.abs(p1 - p2)[keep] / np.maximum(p1, p2)[keep]
+    abs_mean = float(np.sum(r / dof))
+    abs_max = float(r.max()) if r.size else 0.0
+    t = previous - current
+    e = np.diag(inverse)
+    ok = e > 0
+    chi2_diag = float(np.sqrt(np.sum(t[ok] ** 2 / e[ok]) / dof))
+    up = np.triu(fisher)
+    sym = up + np.triu(up, 1).T          # dsymv reads the upper triangle
+    chi2_full = float(np.sqrt(abs(t @ sym @ t) / dof))
+    return abs_mean, abs_max, chi2_diag, chi2_full, bool(chi2_diag < eps or chi2_full < eps)

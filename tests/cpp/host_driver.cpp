@@ -19,6 +19,15 @@
 // the last iteration), int32 nchunks, then per chunk (in construction order)
 // int32 spectrum, fisher_index_start, n_q, status, f64 fisher[n_q*n_q],
 // f64 dbt[3*n_q].
+//
+// Options after the two file names:
+//   graph     Engine::setGraph(true) before the first iteration
+//   autotune  Engine::autotune at theta over caps {2, 5, 1000} first
+//   nr        the Newton-Raphson loop (PowerEstimate, qml_driver.hpp) from
+//             theta, each iteration's theta feeding the next, no smoother and
+//             no early stop; the output is then int32 T, int32 iterations run,
+//             f64 theta[iterations][T], f64 sigma[T] of the last iteration.
+#include "qml_driver.hpp"
 #include "qml_host.hpp"
 
 #include <cstdint>
@@ -46,9 +55,22 @@ struct Reader {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s workload.bin out.bin\n", argv[0]);
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s workload.bin out.bin [graph] [autotune] [nr]\n", argv[0]);
         return 2;
+    }
+    bool graph = false, tune = false, nr = false;
+    for (int a = 3; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "graph"))
+            graph = true;
+        else if (!std::strcmp(argv[a], "autotune"))
+            tune = true;
+        else if (!std::strcmp(argv[a], "nr"))
+            nr = true;
+        else {
+            std::fprintf(stderr, "unknown option %s\n", argv[a]);
+            return 2;
+        }
     }
     try {
         FILE *f = std::fopen(argv[1], "rb");
@@ -98,6 +120,29 @@ int main(int argc, char **argv) {
         const int T = eng.totalKzBins();
         const std::vector<double> theta = r.vec(T);
         std::fclose(f);
+
+        if (tune) {
+            double ms = 0.0;
+            const int cap = eng.autotune(theta.data(), {2, 5, 1000}, 1, &ms);
+            std::printf("host driver: autotuned batch cap %d (%.2f ms)\n", cap, ms);
+        }
+        if (graph) eng.setGraph(true);
+        if (nr) {
+            std::vector<OneQSOEstimate *> ptrs;
+            for (auto &q : qsos) ptrs.push_back(q.get());
+            PowerEstimate pe(eng, ptrs, iters, 0.0);
+            const int run = pe.iterate(theta);
+            FILE *o = std::fopen(argv[2], "wb");
+            if (!o) throw std::runtime_error("cannot open output");
+            const int32_t hdr[2] = {T, run};
+            std::fwrite(hdr, 4, 2, o);
+            for (const auto &th : pe.history) std::fwrite(th.data(), 8, T, o);
+            std::fwrite(pe.power.sigma.data(), 8, T, o);
+            std::fclose(o);
+            std::printf("host driver: Newton-Raphson, %d iteration(s), chi2 diag %.3e full %.3e\n", run,
+                        pe.convergence.chi2_diag, pe.convergence.chi2_full);
+            return 0;
+        }
 
         std::vector<double> F;
         std::vector<std::unique_ptr<double[]>> dbt;

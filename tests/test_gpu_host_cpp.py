@@ -106,8 +106,8 @@ def _pieces(w):
     return out
 
 
-def _python_path(w, theta, precision):
-    """The same chunks registered through the Python binding."""
+def _python_engine(w, precision):
+    """The same chunks registered through the Python binding and committed."""
     from lyspeq_amd.qml import QMLEngine, QMLError
     eng = QMLEngine(w.binning, precision=precision)
     for i, (R, dv) in enumerate(w.pairs):
@@ -122,6 +122,12 @@ def _python_path(w, theta, precision):
             if e.code not in (-3, -4, -5):
                 raise
     eng.commit()
+    return eng, ids
+
+
+def _python_path(w, theta, precision):
+    """One iteration of the same chunks through the Python binding."""
+    eng, ids = _python_engine(w, precision)
     T = eng.T
     F, d = np.zeros((T, T)), np.zeros((3, T))
     eng.iterate(theta, F, d)
@@ -244,3 +250,66 @@ def test_cpp_resolution_matrix_spectra_preprocessed_like_the_reader(tmp_path):
     assert P.rel_err(F, Fo) < FP64_TOL
     for k in range(3):
         assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_cpp_newton_raphson_loop_with_graph_replay_and_autotuned_schedule(tmp_path, precision):
+    """PowerEstimate (lyspeq_amd/host/qml_driver.hpp) runs three Newton-Raphson
+    iterations, each theta feeding the next, over an Engine with graph replay
+    and an autotuned batch schedule.  Iteration k is repeated through the
+    Python binding with the C++ theta of iteration k - 1 as its fits, so both
+    sides hold bit-identical sums.  The C++ theta must then solve the
+    reference's equations, F_s theta = (d - b - t) / 2, or with damping
+    (F_s F_s + damp I) theta = F_s (d - b - t) / 2, with a backward error no
+    worse than 100 times that of the Python driver's theta (LAPACK LU) or
+    1e-13, whichever is larger: a bar set by the arithmetic, not by the
+    conditioning of F_s.  Both must agree within 1e-3 sigma.  The workload is
+    the 64-chunk CPU configuration (20 bands): a few chunks spread over the
+    455 bands of the DESI binning leave F_s near-singular, and the first
+    iteration's theta then makes no covariance positive definite."""
+    from lyspeq_amd.driver import power_estimates
+    print(f"[case] C++ Newton-Raphson loop, graph + autotune, {precision}", flush=True)
+    w = synth.cpu_reference_config(nchunks=64)
+    T = w.binning.total_kz_bins
+    iters = 3
+    wl, out = tmp_path / "workload.bin", tmp_path / "out.bin"
+    _write_workload(wl, w, np.zeros(T), precision, iters)
+    r = subprocess.run([HOST_DRIVER, str(wl), str(out), "graph", "autotune", "nr"], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    buf = out.read_bytes()
+    T2, run = struct.unpack_from("<ii", buf, 0)
+    assert T2 == T and run == iters
+    hist = np.frombuffer(buf, "<f8", run * T, 8).reshape(run, T)
+    sigma = np.frombuffer(buf, "<f8", T, 8 + 8 * run * T)
+    eng, ids = _python_engine(w, precision)
+    for it in range(iters):
+        fits = np.zeros(T) if it == 0 else hist[it - 1]
+        F, d = np.zeros((T, T)), np.zeros((3, T))
+        eng.iterate(fits, F, d)
+        assert sum(eng.chunk_result(cid)[2] == 0 for cid in ids) >= 60
+        res = power_estimates(F, d)
+        A = res["fisher"].copy()
+        A[np.diag(A) == 0, np.diag(A) == 0] = 1.0
+        rhs = 0.5 * (d[0] - d[1] - d[2])
+        th = hist[it]
+        if res["damp"] != 0:
+            lhs, want = (A @ A + res["damp"] * np.eye(T)) @ th, A @ rhs
+            scale = np.abs(A @ A).max() * np.abs(th).max() + np.abs(want).max()
+        else:
+            lhs, want = A @ th, rhs
+            scale = np.abs(A).max() * np.abs(th).max() + np.abs(want).max()
+        backward = np.abs(lhs - want).max() / scale
+        thp = res["theta"]
+        lhs_p = (A @ A + res["damp"] * np.eye(T)) @ thp if res["damp"] != 0 else A @ thp
+        backward_p = np.abs(lhs_p - want).max() / scale
+        ok = np.isfinite(res["sigma"]) & (res["sigma"] > 0)
+        dev = np.abs(th - thp)[ok] / res["sigma"][ok]
+        print(f"iteration {it}: damp {res['damp']:.2e}, backward error C++ {backward:.2e} "
+              f"Python {backward_p:.2e}, max |d theta| / sigma {dev.max():.2e}", flush=True)
+        assert backward < max(100 * backward_p, 1e-13)
+        assert dev.max() < 1e-3
+    assert np.allclose(sigma[ok], res["sigma"][ok], rtol=1e-4, atol=0)
+    assert not np.allclose(hist[0], hist[1])   # theta moved between iterations
+    eng.close()

@@ -27,7 +27,7 @@ from lyspeq_amd.driver import power_estimates                  # noqa: E402
 from lyspeq_amd.qml import QMLEngine, KERNEL_NAMES             # noqa: E402
 
 METRIC = "chunks/s (F,d,b) mixed-prec vs FP64 on 1 MI355X; max rel err of F and P1D"
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "bf16x4": 2.5e15}
 HBM_PEAK = 8.0e12
 
 
@@ -61,7 +61,7 @@ def kernel_work(shapes, prec):
     for N, n, nt in shapes:
         m = n + 2
         N = float(N)
-        if prec == "bf16x3":
+        if prec in ("bf16x3", "bf16x4"):
             # X_j = M + M^T: P = Li M (lower x lower), G = P Li^T + Li P^T (upper half)
             w["whiten_left"] += m * N * (N + 1) * (N + 2) / 3.0
             w["whiten_right"] += 2.0 * m * N * (N + 1) * (N + 2) / 3.0
@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "bf16x4"])
     ap.add_argument("--config", default="desi", choices=list(synth.CONFIGS))
     ap.add_argument("--nchunks", type=int, default=0)
     ap.add_argument("--scratch-gb", type=float, default=96.0)
@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--autotune", default="", metavar="CAPS",
                     help="comma-separated caps on chunks per batch for the autotuned schedule "
                          "(tuned before the warm-up, outside the timed steps)")
+    ap.add_argument("--save-sums", default="", metavar="NPZ",
+                    help="write the FP64 and mixed Fisher and d, b, t sums to this .npz")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -252,6 +254,8 @@ def main():
         errs["P1D_rel"] = float(np.max(np.abs(pm["theta"] - p64["theta"])[ok]) /
                                 np.max(np.abs(p64["theta"][ok])))
         cfg["max_rel_err_vs_fp64"] = errs
+        if args.save_sums:
+            np.savez(args.save_sums, F64=F64, dbt64=d64, Fm=Fm, dbtm=dm)
         log("errors vs fp64:", json.dumps(errs))
         log("fp64 kernel ms:", json.dumps({k: round(v, 2) for k, v in res64["timings"]["kernel_ms"].items()}))
     log(f"{args.precision} kernel ms:", json.dumps({k: round(v, 2) for k, v in resm["timings"]["kernel_ms"].items()}))

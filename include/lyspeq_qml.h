@@ -50,6 +50,8 @@ extern "C" {
 #define QML_PREC_FP64 0
 #define QML_PREC_FP32 1    /* FP32 MFMA, FP32 accumulation */
 #define QML_PREC_BF16X3 2  /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulation */
+#define QML_PREC_BF16X4 3  /* split-BF16 with lo*lo as well: about half the error
+                              of BF16X3 for a few percent more time */
 
 typedef struct qml_ctx qml_ctx;
 

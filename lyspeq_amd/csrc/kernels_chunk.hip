@@ -969,7 +969,7 @@ void launch_xgen(const LaunchCtx &L, void *X, int prec) {
 }
 void launch_convert_li(const LaunchCtx &L, void *lic, int prec) {
     dim3 grid(64, L.nchunks);
-    if (prec == 2)
+    if (prec >= 2)
         k_convert_li<__bf16><<<grid, 256, 0, L.stream>>>(L.chunks, L.mat, (__bf16 *)lic);
     else
         k_convert_li<float><<<grid, 256, 0, L.stream>>>(L.chunks, L.mat, (float *)lic);

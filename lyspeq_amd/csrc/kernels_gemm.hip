@@ -19,7 +19,8 @@
 //   FP32   v_mfma_f32_16x16x4f32, FP32 operands and accumulation;
 //   BF16x3 v_mfma_f32_16x16x32_bf16 three times per block with the FP32
 //          operands split into bf16 hi + lo at LDS staging:
-//          a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation.
+//          a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation;
+//   BF16x4 the same with a_lo b_lo added first (four MFMAs per block).
 // The work list holds (chunk, member or split, I, J) per workgroup; work is
 // assigned so that consecutive items stay on one XCD (blockIdx % 8 selects
 // the XCD), keeping a chunk's operands in that XCD's L2.
@@ -342,15 +343,15 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     }
 }
 
-void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
+void launch_bf3(hipStream_t st, int mode, int nprod, const ChunkDev *chunks, const Work *work, int nwork,
                 const void *A, const void *B, void *C);
 
 template <int MODE>
 static void launch_mode(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                         const void *A, const void *B, void *C) {
     if (nwork <= 0) return;
-    if (prec == 2) {
-        launch_bf3(st, MODE, chunks, work, nwork, A, B, C);
+    if (prec >= 2) {   // split BF16: three products, or four with lo * lo
+        launch_bf3(st, MODE, prec == 3 ? 4 : 3, chunks, work, nwork, A, B, C);
         return;
     }
     const int grid = 8 * ((nwork + 7) / 8);
@@ -385,7 +386,9 @@ namespace qml {
 //
 // A workgroup computes a 128x128 tile with one wave per 64x64 quarter
 // (4x4 blocks of 16x16, three v_mfma_f32_16x16x32_bf16 per block and K step:
-// lo*hi, hi*lo, hi*hi).  Waves whose quarter lies outside the matrix, or
+// lo*hi, hi*lo, hi*hi; NPROD = 4 adds lo*lo first).  The lo*lo term is as
+// large as what the two-plane split leaves out of each operand (both about
+// 2^-16 of the product); adding it about halves the error of the products.  Waves whose quarter lies outside the matrix, or
 // strictly below the diagonal for the upper-only products, skip their MFMAs
 // (a separate copy of the loop) but still stage their share of the operands.
 // NST LDS stages: the whitening products use one (168 VGPRs and 36 KB of
@@ -414,7 +417,7 @@ template <> struct Bf3Shape<MODE_LEFT> { static constexpr int BM = 128, BN = 128
 template <> struct Bf3Shape<MODE_RIGHT> { static constexpr int BM = 128, BN = 128; };
 template <> struct Bf3Shape<MODE_GRAM> { static constexpr int BM = 128, BN = 128; };
 
-template <int MODE, int NST>
+template <int MODE, int NST, int NPROD>
 __global__ __launch_bounds__(64 * (Bf3Shape<MODE>::BM / 64) * (Bf3Shape<MODE>::BN / 64), NST == 1 ? 3 : 2)
 void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *Abase, const __bf16 *Bbase,
                 void *Cbase) {
@@ -544,6 +547,8 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
             const bf8 ah = *(const bf8 *)(s0 + aoff + x * 16 * BK);                                   \
             const bf8 al = *(const bf8 *)(s0 + PLA + aoff + x * 16 * BK);                             \
             _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                           \
+                if constexpr (NPROD == 4)                                                             \
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], al, acc[x][y], 0, 0, 0); \
                 acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al, acc[x][y], 0, 0, 0);   \
                 acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah, acc[x][y], 0, 0, 0);   \
                 acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah, acc[x][y], 0, 0, 0);   \
@@ -715,19 +720,28 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     }
 }
 
-void launch_bf3(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
-                const void *A, const void *B, void *C) {
-    if (nwork <= 0) return;
+template <int NPROD>
+static void launch_bf3_np(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
+                          const __bf16 *a, const __bf16 *b, void *C) {
     const int grid = 8 * ((nwork + 7) / 8);
-    const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
     // whitening products: one LDS stage (three workgroups per CU); the Gram,
     // whose K sweeps are longer: two stages
     if (mode == MODE_LEFT)
-        k_gemm_bf3<MODE_LEFT, 1><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_LEFT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
     else if (mode == MODE_RIGHT)
-        k_gemm_bf3<MODE_RIGHT, 1><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_RIGHT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
     else
-        k_gemm_bf3<MODE_GRAM, 2><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        k_gemm_bf3<MODE_GRAM, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+}
+
+void launch_bf3(hipStream_t st, int mode, int nprod, const ChunkDev *chunks, const Work *work, int nwork,
+                const void *A, const void *B, void *C) {
+    if (nwork <= 0) return;
+    const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
+    if (nprod == 4)
+        launch_bf3_np<4>(st, mode, chunks, work, nwork, a, b, C);
+    else
+        launch_bf3_np<3>(st, mode, chunks, work, nwork, a, b, C);
 }
 
 }  // namespace qml

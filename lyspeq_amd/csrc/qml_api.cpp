@@ -198,7 +198,7 @@ int qml_create(qml_ctx **out, const qml_opts *opts) {
         if (opts->scratch_gb > 0) c->scratch_gb = opts->scratch_gb;
         if (opts->max_batch_chunks > 0) c->max_batch = opts->max_batch_chunks;
     }
-    if (c->prec < 0 || c->prec > 2) {
+    if (c->prec < 0 || c->prec > 3) {
         delete c;
         return fail(QML_ERR_ARG, "qml_create: unknown precision");
     }
@@ -644,7 +644,7 @@ int plan_batches(qml_ctx *c) {
             d.nvu = ch.nvu;
             d.sj_off = (long long)acc.sj;
             // the split form stores two bf16 planes per element: offsets in bf16 units
-            const long long su = (c->prec == 2) ? 2 : 1;
+            const long long su = (c->prec >= 2) ? 2 : 1;
             d.stk_off = (long long)acc.stk * su;
             d.gram_off = (long long)acc.gram;
             d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic * su;
@@ -715,11 +715,11 @@ int plan_batches(qml_ctx *c) {
             }
             // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16,
             // whose left product P = Li M is lower triangular (J <= I only)
-            const int g = (c->prec == 2) ? 2 : 1;
+            const int g = (c->prec >= 2) ? 2 : 1;
             const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
             for (int j = 0; j < d.m; ++j)
                 for (int I = 0; I < nt; ++I)
-                    for (int J = 0; J < ((c->prec == 2) ? I + 1 : nt); ++J) wl.push_back({li, j, I, J});
+                    for (int J = 0; J < ((c->prec >= 2) ? I + 1 : nt); ++J) wl.push_back({li, j, I, J});
             for (int j = 0; j < d.m; ++j)
                 for (int I = 0; I < nt; ++I)
                     for (int J = I; J < nt; ++J) wr.push_back({li, j, I, J});

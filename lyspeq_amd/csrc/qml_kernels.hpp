@@ -64,7 +64,7 @@ void launch_poison_lds(hipStream_t st);
 void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, const int *skip, int nchunks,
                        int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
 
-// kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3.
+// kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3, 3 BF16x4.
 // Y_j = Li X_j for every member (lower-triangular Li).
 void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                              const void *li, const void *X, void *Y);

@@ -21,8 +21,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqmlhip.so")
 
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4 = 0, 1, 2, 3
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "bf16x4": PREC_BF16X4}
 
 QML_OK = 0
 QML_ERR_ARG, QML_ERR_HIP, QML_ERR_SHORT_CHUNK, QML_ERR_NO_TABLE = -1, -2, -3, -4

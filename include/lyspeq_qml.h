@@ -284,6 +284,35 @@ int qml_set_pipeline(qml_ctx *ctx, int on);
  * reference. */
 int qml_set_graph(qml_ctx *ctx, int on);
 
+/* Poisson bootstrap of the last iteration's per-chunk results, in the
+ * reference's FastBootstrap form (specifics::FAST_BOOTSTRAP, the default;
+ * PoissonBootstrapper::_fastBootstrap and _meanBootstrap, core/bootstrapper.hpp:
+ * 183-221,282-375,392-399; OneQSOEstimate::collapseBootstrap and
+ * addBootPowerOnly, core/one_qso_estimate.cpp:200-239,
+ * core/one_qso_estimate.hpp:55-60).
+ *   qso_of_chunk: for every registered chunk, its spectrum's position in the
+ *     driver's queue (0 .. nqso-1), or -1 to leave the chunk out.  Every
+ *     spectrum 0 .. nqso-1 is one the reference's queue still holds.
+ *   Spectrum q's collapsed vector theta_q (ndim values at istart) is the sum,
+ *   in chunk order, of its chunks' (d - b) - t at their fisher_index_start.
+ *   Its weights are draws q nboots .. (q+1) nboots - 1 of qml_poisson_draws(seed).
+ *   realizations (nboots x TOTAL_KZ_BINS, may be NULL): R[b] = sum over q of
+ *     p_qb theta_q (not normalised: 0.5 solver R[b] is a power realisation);
+ *   mean (T) and cov (T x T): the mean of the realisations and their sample
+ *     covariance with 1/(nboots - 1), as _calcuateMean / _calcuateCovariance.
+ *     The reference then writes 0.25 cov as the bootstrap Fisher matrix and
+ *     sandwiches it with the solver (_saveData, _sandwichInvFisher).
+ * Weights, sums and their order follow the reference; the seed it uses is the
+ * MPI rank (0 on one process). */
+int qml_bootstrap(qml_ctx *ctx, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
+                  double *realizations, double *mean, double *cov);
+
+/* n unit-mean Poisson draws of PoissonRNG(seed) (core/bootstrapper.hpp:19-37):
+ * std::poisson_distribution<int> over std::mt19937_64 seeded once.  With the
+ * same C++ standard library as the reference's build the draws are the
+ * reference's.  Host code; no context or device is needed. */
+int qml_poisson_draws(unsigned long long seed, long long n, int *out);
+
 /* Autotuned batch schedule.  For each candidate cap on chunks per batch
  * (caps[0..ncaps-1]), the committed chunks are planned again into
  * size-bucketed batches (sorted by padded size, cut at the cap or at the

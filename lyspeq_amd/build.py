@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqmlhip.so")
 
 SOURCES = ["kernels_chunk.hip", "kernels_factor.hip", "kernels_gemm.hip", "kernels_rmat.hip", "kernels_tables.hip",
-           "qml_api.cpp",
+           "kernels_boot.hip", "qml_api.cpp",
            "rmat_prep.cpp"]
 
 HIPCC_FLAGS = [

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -1126,6 +1127,77 @@ int qml_set_chunk_active(qml_ctx *c, int id, int active) {
     HIPCHK(hipStreamSynchronize(c->st2));
     HIPCHK(hipMemcpy(c->d_skip + id, &skip, sizeof(int), hipMemcpyHostToDevice));
     return QML_OK;
+}
+
+int qml_poisson_draws(unsigned long long seed, long long n, int *out) {
+    if (n < 0 || (n > 0 && !out)) return fail(QML_ERR_ARG, "qml_poisson_draws: bad count or null output");
+    // PoissonRNG of core/bootstrapper.hpp:19-37: a 64-bit Mersenne twister
+    // seeded once, unit-mean Poisson draws from the standard library
+    std::mt19937_64 rng(seed);
+    std::poisson_distribution<int> pd;
+    for (long long i = 0; i < n; ++i) out[i] = pd(rng);
+    return QML_OK;
+}
+
+int qml_bootstrap(qml_ctx *c, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
+                  double *realizations, double *mean, double *cov) {
+    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_bootstrap: commit and iterate first");
+    if (nboots < 2 || nqso < 1 || !qso_of_chunk || !mean || !cov)
+        return fail(QML_ERR_ARG, "qml_bootstrap: nboots >= 2, nqso >= 1 and the outputs are required");
+    const int nch = (int)c->chunks.size(), T = c->T;
+    std::vector<int> qptr(nqso + 1, 0), qchunk;
+    for (int i = 0; i < nch; ++i) {
+        const int q = qso_of_chunk[i];
+        if (q < -1 || q >= nqso) return fail(QML_ERR_ARG, "qml_bootstrap: spectrum index out of range");
+        if (q >= 0) ++qptr[q + 1];
+    }
+    for (int q = 0; q < nqso; ++q) qptr[q + 1] += qptr[q];
+    qchunk.resize(qptr[nqso]);
+    {
+        std::vector<int> fill(qptr.begin(), qptr.end() - 1);
+        for (int i = 0; i < nch; ++i)
+            if (qso_of_chunk[i] >= 0) qchunk[fill[qso_of_chunk[i]]++] = i;   // chunk order within a spectrum
+    }
+    // the weights of spectrum q are draws q * nboots .. (q + 1) * nboots - 1
+    std::vector<int> P((size_t)nqso * nboots);
+    {
+        const int rc = qml_poisson_draws(seed, (long long)P.size(), P.data());
+        if (rc != QML_OK) return rc;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    int *d_qptr = nullptr, *d_qchunk = nullptr, *d_P = nullptr;
+    double *d_theta = nullptr, *d_R = nullptr, *d_mean = nullptr, *d_cov = nullptr;
+    auto release = [&]() {
+        void *ps[] = {d_qptr, d_qchunk, d_P, d_theta, d_R, d_mean, d_cov};
+        for (void *p : ps) if (p) hipFree(p);
+    };
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc(&d_qptr, qptr.size() * sizeof(int)));
+        HIPCHK(hipMalloc(&d_qchunk, std::max<size_t>(qchunk.size(), 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&d_P, P.size() * sizeof(int)));
+        HIPCHK(hipMalloc(&d_theta, (size_t)nqso * T * sizeof(double)));
+        HIPCHK(hipMalloc(&d_R, (size_t)nboots * T * sizeof(double)));
+        HIPCHK(hipMalloc(&d_mean, (size_t)T * sizeof(double)));
+        HIPCHK(hipMalloc(&d_cov, (size_t)T * T * sizeof(double)));
+        HIPCHK(hipMemcpy(d_qptr, qptr.data(), qptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!qchunk.empty())
+            HIPCHK(hipMemcpy(d_qchunk, qchunk.data(), qchunk.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_P, P.data(), P.size() * sizeof(int), hipMemcpyHostToDevice));
+        launch_bootstrap(c->st, c->d_outs, c->d_dbt, d_qptr, d_qchunk, nqso, T, d_P, nboots, d_theta, d_R, d_mean,
+                         d_cov);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (realizations)
+            HIPCHK(hipMemcpy(realizations, d_R, (size_t)nboots * T * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mean, d_mean, (size_t)T * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cov, d_cov, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost));
+        return QML_OK;
+    };
+    const int rc = run();
+    release();
+    return rc;
 }
 
 int qml_num_chunks(qml_ctx *c) { return c ? (int)c->chunks.size() : 0; }

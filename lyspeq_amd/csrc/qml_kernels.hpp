@@ -75,6 +75,13 @@ void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, 
 void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram);
 
+// kernels_boot.hip: FastBootstrap realisations, their mean and covariance.
+// qptr/qchunk: CSR list of each spectrum's chunk ids (nqso + 1 offsets); P is
+// nqso x nboots Poisson weights; theta nqso x T and R nboots x T scratch.
+void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, const int *qptr, const int *qchunk,
+                      int nqso, int T, const int *P, int nboots, double *theta, double *R, double *mean,
+                      double *cov);
+
 // S/Q lookup tables (kernels_tables.hip)
 struct PD13 {
     double A, n, alpha, B, beta, lambda;

@@ -166,6 +166,20 @@ void Engine::flush() {
 
 void Engine::timings(qml_timings *t) { check(qml_get_timings(ctx_, t)); }
 
+void Engine::bootstrap(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed,
+                       double *mean, double *cov, double *realizations) {
+    commit();
+    std::vector<int> qso(chunks_.size(), -1);
+    int nq = 0;
+    for (OneQSOEstimate *q : queue) {
+        if (q->chunks.empty()) continue;   // erased from the reference's queue
+        for (auto &c : q->chunks) qso[c->engineId()] = nq;
+        ++nq;
+    }
+    if (nq == 0) throw QMLError(QML_ERR_ARG, "Engine::bootstrap: no spectrum holds a chunk");
+    check(qml_bootstrap(ctx_, nboots, seed, nq, qso.data(), realizations, mean, cov));
+}
+
 // ------------------------------------------------------------------- Chunk --
 Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2, const ResolutionRows *rr) : eng_(&eng) {
     ResolutionRows own;

@@ -89,6 +89,7 @@ struct Binning {
 };
 
 class Chunk;
+class OneQSOEstimate;
 
 // One device context: tables, chunks, and the batched iteration.
 class Engine {
@@ -118,6 +119,13 @@ public:
     // (qml_autotune).  No chunk can be added afterwards.  Returns the cap.
     int autotune(const double *ps_estimate, const std::vector<int> &caps, int reps = 1,
                  double *best_ms = nullptr);
+
+    // PoissonBootstrapper::run with FastBootstrap (qml_bootstrap) over the
+    // driver's queue after the last flush: the spectra in queue order, each
+    // with the chunks it still holds.  mean is TOTAL_KZ_BINS values, cov
+    // TOTAL_KZ_BINS^2; realizations (nboots x TOTAL_KZ_BINS) may be null.
+    void bootstrap(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed, double *mean,
+                   double *cov, double *realizations = nullptr);
 
     int totalKzBins() const { return bins_.totalKzBins(); }
     const Binning &binning() const { return bins_; }

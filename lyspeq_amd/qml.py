@@ -109,7 +109,7 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
-           "qml_set_graph", "qml_autotune"]
+           "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws"]
 
 
 class _TableSpec(C.Structure):
@@ -145,6 +145,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
     lib.qml_set_pipeline.argtypes = [vp, C.c_int]
     lib.qml_set_graph.argtypes = [vp, C.c_int]
+    lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
+    lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]
@@ -173,6 +175,14 @@ def rmat_deconvolve(values, m: float) -> np.ndarray:
     if code != QML_OK:
         raise QMLError(code, "qml_rmat_deconvolve: invalid arguments")
     return v
+
+
+def poisson_draws(seed: int, n: int) -> np.ndarray:
+    """n unit-mean Poisson draws of the reference's PoissonRNG(seed) (host)."""
+    lib = load_library()
+    out = np.zeros(int(n), dtype=np.int32)
+    _check(lib.qml_poisson_draws(int(seed), int(n), out.ctypes.data_as(C.POINTER(C.c_int))), lib)
+    return out
 
 
 def rmat_oversample(values, oversampling: int) -> np.ndarray:
@@ -389,6 +399,18 @@ class QMLEngine:
         _check(self._lib.qml_autotune(self._ctx, _dptr(th), cv, len(caps), int(reps), C.byref(best),
                                       C.byref(ms)), self._lib)
         return best.value, ms.value
+
+    def bootstrap(self, nboots: int, qso_of_chunk, nqso: int, seed: int = 0, realizations: bool = True):
+        """FastBootstrap of the last iteration's per-chunk results: returns
+        (realizations nboots x T or None, mean T, covariance T x T)."""
+        q = np.ascontiguousarray(qso_of_chunk, dtype=np.int32)
+        assert q.size == len(self.chunk_infos)
+        R = np.zeros((nboots, self.T)) if realizations else None
+        mean, cov = np.zeros(self.T), np.zeros((self.T, self.T))
+        _check(self._lib.qml_bootstrap(self._ctx, int(nboots), int(seed), int(nqso),
+                                       q.ctypes.data_as(C.POINTER(C.c_int)), _dptr(R) if R is not None else None,
+                                       _dptr(mean), _dptr(cov)), self._lib)
+        return R, mean, cov
 
     def set_chunk_active(self, cid: int, active: bool):
         """Leaves a chunk out of (or back in) the accumulated sums."""

@@ -27,6 +27,8 @@
 //             theta, each iteration's theta feeding the next, no smoother and
 //             no early stop; the output is then int32 T, int32 iterations run,
 //             f64 theta[iterations][T], f64 sigma[T] of the last iteration.
+//   boot      after the iterations, Engine::bootstrap over the spectra (500
+//             realisations, seed 7); out.bin.boot receives f64 mean[T], cov[T*T].
 #include "qml_driver.hpp"
 #include "qml_host.hpp"
 
@@ -34,6 +36,7 @@
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 using namespace lyspeq_amd;
@@ -59,7 +62,7 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "usage: %s workload.bin out.bin [graph] [autotune] [nr]\n", argv[0]);
         return 2;
     }
-    bool graph = false, tune = false, nr = false;
+    bool graph = false, tune = false, nr = false, boot = false;
     for (int a = 3; a < argc; ++a) {
         if (!std::strcmp(argv[a], "graph"))
             graph = true;
@@ -67,6 +70,8 @@ int main(int argc, char **argv) {
             tune = true;
         else if (!std::strcmp(argv[a], "nr"))
             nr = true;
+        else if (!std::strcmp(argv[a], "boot"))
+            boot = true;
         else {
             std::fprintf(stderr, "unknown option %s\n", argv[a]);
             return 2;
@@ -157,6 +162,17 @@ int main(int argc, char **argv) {
             eng.flush();
         }
 
+        if (boot) {
+            std::vector<OneQSOEstimate *> queue;
+            for (auto &q : qsos) queue.push_back(q.get());
+            std::vector<double> mean(T), cov((size_t)T * T);
+            eng.bootstrap(queue, 500, 7, mean.data(), cov.data());
+            FILE *bo = std::fopen((std::string(argv[2]) + ".boot").c_str(), "wb");
+            if (!bo) throw std::runtime_error("cannot open bootstrap output");
+            std::fwrite(mean.data(), 8, T, bo);
+            std::fwrite(cov.data(), 8, cov.size(), bo);
+            std::fclose(bo);
+        }
         FILE *o = std::fopen(argv[2], "wb");
         if (!o) throw std::runtime_error("cannot open output");
         const int32_t T32 = T;

@@ -313,3 +313,42 @@ def test_cpp_newton_raphson_loop_with_graph_replay_and_autotuned_schedule(tmp_pa
     assert np.allclose(sigma[ok], res["sigma"][ok], rtol=1e-4, atol=0)
     assert not np.allclose(hist[0], hist[1])   # theta moved between iterations
     eng.close()
+
+
+def test_cpp_bootstrap_over_the_queue_matches_python_binding(tmp_path):
+    """Engine::bootstrap numbers the spectra that hold chunks in queue order
+    and maps each chunk to its spectrum; the same mapping through the Python
+    binding must give the same mean and covariance bit for bit."""
+    from lyspeq_amd.qml import QMLEngine, QMLError
+    print("[case] C++ bootstrap over the queue", flush=True)
+    w = synth.desi_config(nchunks=16)
+    T = w.binning.total_kz_bins
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 13)
+    wl, out = tmp_path / "workload.bin", tmp_path / "out.bin"
+    _write_workload(wl, w, th, "fp64", 1)
+    r = subprocess.run([HOST_DRIVER, str(wl), str(out), "boot"], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    buf = np.fromfile(str(out) + ".boot", "<f8")
+    mean_c, cov_c = buf[:T], buf[T:].reshape(T, T)
+    eng = QMLEngine(w.binning, precision="fp64")
+    for i, (R, dv) in enumerate(w.pairs):
+        eng.add_table(R, dv, w.deriv[i], w.signal[i], w.v_length)
+    spec_of = []
+    for s, i1, i2 in _pieces(w):
+        c = w.chunks[s]
+        try:
+            eng.add_chunk(c["wave"][i1:i2], c["delta"][i1:i2], c["ivar"][i1:i2], c["R"])
+            spec_of.append(s)
+        except QMLError as e:
+            if e.code not in (-3, -4, -5):
+                raise
+    eng.commit()
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(th, F, d)
+    order = {s: k for k, s in enumerate(sorted(set(spec_of)))}
+    qso = np.array([order[s] for s in spec_of], dtype=np.int32)
+    _, mean, cov = eng.bootstrap(500, qso, len(order), seed=7, realizations=False)
+    eng.close()
+    assert np.array_equal(mean, mean_c) and np.array_equal(cov, cov_c)
+    assert np.abs(cov).max() > 0

@@ -81,13 +81,14 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
     log(f"[{precision}] {len(accepted)} chunks registered and committed in {time.perf_counter() - t0:.1f} s")
     T = eng.T
     theta = np.zeros(T)
-    schedule = {"max_batch_chunks": 256, "autotuned": False}
+    schedule = dict(eng.schedule(), autotuned=False)
     if tune_caps:
         t0 = time.perf_counter()
         cap, tms = eng.autotune(theta, caps=tune_caps, reps=1)
-        schedule = {"max_batch_chunks": cap, "autotuned": True, "candidates": list(tune_caps),
-                    "tuned_ms": tms, "tuning_s": time.perf_counter() - t0}
-        log(f"[{precision}] autotuned batch cap {cap} ({tms:.1f} ms) in {schedule['tuning_s']:.1f} s")
+        schedule = dict(eng.schedule(), autotuned=True, candidates=list(tune_caps), tuned_ms=tms,
+                        tuning_s=time.perf_counter() - t0)
+        log(f"[{precision}] autotuned batch cap {cap}, LDS stages left {schedule['stages_left']} "
+            f"gram {schedule['stages_gram']} ({tms:.1f} ms) in {schedule['tuning_s']:.1f} s")
     if graph:
         eng.set_graph(True)
     for i in range(warmup):

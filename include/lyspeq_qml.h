@@ -317,12 +317,33 @@ int qml_poisson_draws(unsigned long long seed, long long n, int *out);
  * (caps[0..ncaps-1]), the committed chunks are planned again into
  * size-bucketed batches (sorted by padded size, cut at the cap or at the
  * scratch budget of qml_opts), and reps + 1 iterations are run at theta, the
- * first as warm-up; the schedule with the shortest iteration is kept and its
- * cap and time (ms) are returned.  The per-chunk results do not depend on the
- * schedule.  The accumulated sums of these iterations are discarded.  No
- * counterpart in the reference (its chunk loop is serial per MPI rank). */
+ * first as warm-up; the plan with the shortest iteration is kept.  In the
+ * split-BF16 precisions the kernel shapes are tuned next, on that plan: the
+ * LDS stages (1 or 2) of the left whitening product and of the Gram, every
+ * combination timed the same way (qml_set_kernel_stages).  The chosen cap and
+ * the best time (ms) are returned; qml_get_schedule reports the whole choice.
+ * The per-chunk results do not depend on any of these choices.  The
+ * accumulated sums of these iterations are discarded.  No counterpart in the
+ * reference (its chunk loop is serial per MPI rank). */
 int qml_autotune(qml_ctx *ctx, const double *theta, const int *caps, int ncaps, int reps, int *best_cap,
                  double *best_ms);
+
+/* Kernel shapes of the split-BF16 products: LDS stages of the left
+ * whitening product and of the Gram, 1 (the next stage waits in registers,
+ * three workgroups per CU) or 2 (two workgroups per CU).  Defaults 1 and 2.
+ * Results are identical for every choice.  No counterpart in the reference. */
+int qml_set_kernel_stages(qml_ctx *ctx, int stages_left, int stages_gram);
+
+/* The current launch schedule. */
+typedef struct {
+    int max_batch_chunks;  /* cap on chunks per batch of the current plan */
+    int n_batches;         /* batches of the current plan */
+    int stages_left;       /* LDS stages of the split-BF16 left product */
+    int stages_gram;       /* LDS stages of the split-BF16 Gram */
+    int pipeline;          /* two-stream stage overlap (qml_set_pipeline) */
+    int graph;             /* graph replay (qml_set_graph) */
+} qml_schedule;
+int qml_get_schedule(qml_ctx *ctx, qml_schedule *schedule);
 
 /* Number of chunks registered, and TOTAL_KZ_BINS. */
 int qml_num_chunks(qml_ctx *ctx);

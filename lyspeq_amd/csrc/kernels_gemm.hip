@@ -343,15 +343,15 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     }
 }
 
-void launch_bf3(hipStream_t st, int mode, int nprod, const ChunkDev *chunks, const Work *work, int nwork,
+void launch_bf3(hipStream_t st, int mode, int nprod, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                 const void *A, const void *B, void *C);
 
 template <int MODE>
-static void launch_mode(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+static void launch_mode(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                         const void *A, const void *B, void *C) {
     if (nwork <= 0) return;
     if (prec >= 2) {   // split BF16: three products, or four with lo * lo
-        launch_bf3(st, MODE, prec == 3 ? 4 : 3, chunks, work, nwork, A, B, C);
+        launch_bf3(st, MODE, prec == 3 ? 4 : 3, nst, chunks, work, nwork, A, B, C);
         return;
     }
     const int grid = 8 * ((nwork + 7) / 8);
@@ -361,17 +361,17 @@ static void launch_mode(hipStream_t st, int prec, const ChunkDev *chunks, const 
         k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
 }
 
-void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
-                             const void *li, const void *X, void *Y) {
-    launch_mode<MODE_LEFT>(st, prec, chunks, work, nwork, li, X, Y);
+void launch_gemm_whiten_left(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
+                             int nwork, const void *li, const void *X, void *Y) {
+    launch_mode<MODE_LEFT>(st, prec, nst, chunks, work, nwork, li, X, Y);
 }
 void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                               const void *li, const void *Y, void *G) {
-    launch_mode<MODE_RIGHT>(st, prec, chunks, work, nwork, Y, li, G);
+    launch_mode<MODE_RIGHT>(st, prec, 1, chunks, work, nwork, Y, li, G);
 }
-void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+void launch_gemm_gram(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram) {
-    launch_mode<MODE_GRAM>(st, prec, chunks, work, nwork, G, G, gram);
+    launch_mode<MODE_GRAM>(st, prec, nst, chunks, work, nwork, G, G, gram);
 }
 
 }  // namespace qml
@@ -721,27 +721,37 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 }
 
 template <int NPROD>
-static void launch_bf3_np(hipStream_t st, int mode, const ChunkDev *chunks, const Work *work, int nwork,
+static void launch_bf3_np(hipStream_t st, int mode, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                           const __bf16 *a, const __bf16 *b, void *C) {
     const int grid = 8 * ((nwork + 7) / 8);
-    // whitening products: one LDS stage (three workgroups per CU); the Gram,
-    // whose K sweeps are longer: two stages
-    if (mode == MODE_LEFT)
-        k_gemm_bf3<MODE_LEFT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
-    else if (mode == MODE_RIGHT)
+    // nst LDS stages for the left product and the Gram (one: three
+    // workgroups per CU, the next stage in registers; two: two workgroups per
+    // CU, a second register set two stages ahead); the right product's
+    // two-term sweep uses one.  Each output sums its K steps in the same
+    // order either way, so the results are identical.
+    if (mode == MODE_LEFT) {
+        if (nst == 2)
+            k_gemm_bf3<MODE_LEFT, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        else
+            k_gemm_bf3<MODE_LEFT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+    } else if (mode == MODE_RIGHT) {
         k_gemm_bf3<MODE_RIGHT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
-    else
-        k_gemm_bf3<MODE_GRAM, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+    } else {
+        if (nst == 1)
+            k_gemm_bf3<MODE_GRAM, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        else
+            k_gemm_bf3<MODE_GRAM, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+    }
 }
 
-void launch_bf3(hipStream_t st, int mode, int nprod, const ChunkDev *chunks, const Work *work, int nwork,
+void launch_bf3(hipStream_t st, int mode, int nprod, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                 const void *A, const void *B, void *C) {
     if (nwork <= 0) return;
     const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
     if (nprod == 4)
-        launch_bf3_np<4>(st, mode, chunks, work, nwork, a, b, C);
+        launch_bf3_np<4>(st, mode, nst, chunks, work, nwork, a, b, C);
     else
-        launch_bf3_np<3>(st, mode, chunks, work, nwork, a, b, C);
+        launch_bf3_np<3>(st, mode, nst, chunks, work, nwork, a, b, C);
 }
 
 }  // namespace qml

@@ -130,6 +130,7 @@ struct qml_ctx {
     double scratch_gb = 32.0;
     int max_batch = 256;
     int pipeline = 1;
+    int nst_left = 1, nst_gram = 2;   // LDS stages of the split-BF16 left product and Gram
     int poison = 0;   // LYSPEQ_QML_POISON=1 at commit: NaN-filled scratch and LDS
     hipStream_t st = nullptr, st2 = nullptr;
     bool bins_set = false, committed = false;
@@ -932,11 +933,11 @@ int enqueue_iteration(qml_ctx *c, bool timing) {
         HIPCHK(mark(B.evB[0], sB));
         launch_xgen(L2, c->d_X, c->prec);
         HIPCHK(mark(B.evB[1], sB));
-        launch_gemm_whiten_left(sB, c->prec, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
+        launch_gemm_whiten_left(sB, c->prec, c->nst_left, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
         HIPCHK(mark(B.evB[2], sB));
         launch_gemm_whiten_right(sB, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
         HIPCHK(mark(B.evB[3], sB));
-        launch_gemm_gram(sB, c->prec, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
+        launch_gemm_gram(sB, c->prec, c->nst_gram, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
         HIPCHK(mark(B.evB[4], sB));
         launch_gram_finalize(L2, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
         HIPCHK(hipEventRecord(B.evB[5], sB));
@@ -1103,8 +1104,57 @@ int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, in
         const int rc = plan_batches(c);
         if (rc != QML_OK) return rc;
     }
+    if (c->prec >= 2) {
+        // kernel shapes of the split-BF16 products: LDS stages of the left
+        // product and of the Gram, timed on the chosen plan
+        static const int cand[4][2] = {{1, 2}, {2, 2}, {1, 1}, {2, 1}};
+        int bl = c->nst_left, bg = c->nst_gram;
+        double bt = -1.0;
+        for (const auto &v : cand) {
+            const int rs = qml_set_kernel_stages(c, v[0], v[1]);
+            if (rs != QML_OK) return rs;
+            double t = 0.0;
+            for (int r = 0; r <= reps; ++r) {   // r = 0 warms the variant up
+                const int rc = qml_iterate(c, theta, F.data(), dbt.data());
+                if (rc != QML_OK) return rc;
+                if (r > 0 && (r == 1 || c->tm.total_ms < t)) t = c->tm.total_ms;
+            }
+            if (bt < 0 || t < bt) {
+                bt = t;
+                bl = v[0];
+                bg = v[1];
+            }
+        }
+        const int rs = qml_set_kernel_stages(c, bl, bg);
+        if (rs != QML_OK) return rs;
+        best_t = bt;
+    }
     if (best_cap) *best_cap = best;
     if (best_ms) *best_ms = best_t;
+    return QML_OK;
+}
+
+int qml_set_kernel_stages(qml_ctx *c, int stages_left, int stages_gram) {
+    if (!c) return fail(QML_ERR_ARG, "qml_set_kernel_stages: null context");
+    if (stages_left < 1 || stages_left > 2 || stages_gram < 1 || stages_gram > 2)
+        return fail(QML_ERR_ARG, "qml_set_kernel_stages: stages must be 1 or 2");
+    if (stages_left != c->nst_left || stages_gram != c->nst_gram) {
+        HIPCHK(hipSetDevice(c->device));
+        drop_graph(c);   // a captured sequence holds the launches of the old kernels
+    }
+    c->nst_left = stages_left;
+    c->nst_gram = stages_gram;
+    return QML_OK;
+}
+
+int qml_get_schedule(qml_ctx *c, qml_schedule *s) {
+    if (!c || !s) return fail(QML_ERR_ARG, "qml_get_schedule: null argument");
+    s->max_batch_chunks = c->max_batch;
+    s->n_batches = (int)c->batches.size();
+    s->stages_left = c->nst_left;
+    s->stages_gram = c->nst_gram;
+    s->pipeline = c->pipeline;
+    s->graph = c->use_graph;
     return QML_OK;
 }
 

@@ -65,14 +65,16 @@ void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, 
                        int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
 
 // kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3, 3 BF16x4.
+// nst: LDS stages of the split-BF16 left product and Gram (1 or 2; the
+// FP64/FP32 products and the right product ignore it).
 // Y_j = Li X_j for every member (lower-triangular Li).
-void launch_gemm_whiten_left(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
-                             const void *li, const void *X, void *Y);
+void launch_gemm_whiten_left(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
+                             int nwork, const void *li, const void *X, void *Y);
 // G_j = Y_j Li^T on upper tiles, written tile-packed.
 void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
                               const void *li, const void *Y, void *G);
 // Gram partials E_s = sum over G tile s of G_j G_j'.
-void launch_gemm_gram(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
+void launch_gemm_gram(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram);
 
 // kernels_boot.hip: FastBootstrap realisations, their mean and covariance.

@@ -109,7 +109,13 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_add_chunk", "qml_add_chunk_rmat", "qml_commit", "qml_iterate", "qml_get_chunk_result",
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
-           "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws"]
+           "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws", "qml_set_kernel_stages",
+           "qml_get_schedule"]
+
+
+class _Schedule(C.Structure):
+    _fields_ = [("max_batch_chunks", C.c_int), ("n_batches", C.c_int), ("stages_left", C.c_int),
+                ("stages_gram", C.c_int), ("pipeline", C.c_int), ("graph", C.c_int)]
 
 
 class _TableSpec(C.Structure):
@@ -145,6 +151,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
     lib.qml_set_pipeline.argtypes = [vp, C.c_int]
     lib.qml_set_graph.argtypes = [vp, C.c_int]
+    lib.qml_set_kernel_stages.argtypes = [vp, C.c_int, C.c_int]
+    lib.qml_get_schedule.argtypes = [vp, C.POINTER(_Schedule)]
     lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
@@ -411,6 +419,15 @@ class QMLEngine:
                                        q.ctypes.data_as(C.POINTER(C.c_int)), _dptr(R) if R is not None else None,
                                        _dptr(mean), _dptr(cov)), self._lib)
         return R, mean, cov
+
+    def set_kernel_stages(self, left: int, gram: int):
+        """LDS stages (1 or 2) of the split-BF16 left product and Gram."""
+        _check(self._lib.qml_set_kernel_stages(self._ctx, int(left), int(gram)), self._lib)
+
+    def schedule(self) -> dict:
+        s = _Schedule()
+        _check(self._lib.qml_get_schedule(self._ctx, C.byref(s)), self._lib)
+        return {k: getattr(s, k) for k, _ in s._fields_}
 
     def set_chunk_active(self, cid: int, active: bool):
         """Leaves a chunk out of (or back in) the accumulated sums."""

@@ -101,6 +101,36 @@ def test_autotuned_schedule_gives_identical_results(precision):
     eng.close()
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16x4"])
+def test_kernel_stage_variants_give_identical_results(precision):
+    """The split-BF16 left product and Gram in one or two LDS stages: every
+    combination, with a graph captured again after each change, must give
+    the default's results bit for bit."""
+    _print_case(f"kernel stage variants {precision}")
+    w = synth.desi_config(nchunks=24)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 9)
+    eng, acc = _engine(w, precision, max_batch_chunks=10)
+    s = eng.schedule()
+    assert (s["stages_left"], s["stages_gram"]) == (1, 2) and s["n_batches"] >= 2
+    e0 = _run(eng, acc, th)
+    for left, gram in ((2, 2), (1, 1), (2, 1)):
+        eng.set_kernel_stages(left, gram)
+        _same(_run(eng, acc, th), e0)
+    eng.set_graph(True)
+    for left, gram in ((1, 2), (2, 1)):
+        eng.set_kernel_stages(left, gram)
+        _same(_run(eng, acc, th), e0)
+        _same(_run(eng, acc, th), e0)
+    with pytest.raises(Exception):
+        eng.set_kernel_stages(3, 1)
+    best, ms = eng.autotune(th, caps=(10,), reps=1)
+    s = eng.schedule()
+    print(f"autotuned stages {precision}: left {s['stages_left']} gram {s['stages_gram']}, {ms:.2f} ms", flush=True)
+    assert s["stages_left"] in (1, 2) and s["stages_gram"] in (1, 2) and s["max_batch_chunks"] == 10
+    _same(_run(eng, acc, th), e0)
+    eng.close()
+
+
 def test_autotune_rejects_bad_candidates():
     _print_case("autotune arguments")
     w = synth.desi_config(nchunks=4)

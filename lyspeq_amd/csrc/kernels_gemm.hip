@@ -13,13 +13,14 @@
 //   gram        :  E_s = sum_{k in G tile s} G_j[k] G_j'[k]   (split-K partials,
 //                                             x2 for off-diagonal G tiles)
 //
-// One 256-thread workgroup computes a 64x64 output tile; each of its four
-// waves owns a 32x32 quarter made of 2x2 16x16 MFMA blocks.  Precision:
+// FP64 and FP32 (k_gemm): one 256-thread workgroup computes a 64x64 output
+// tile; each of its four waves owns a 32x32 quarter made of 2x2 16x16 MFMA
+// blocks:
 //   FP64   v_mfma_f64_16x16x4f64, FP64 operands and accumulation;
-//   FP32   v_mfma_f32_16x16x4f32, FP32 operands and accumulation;
-//   BF16x3 v_mfma_f32_16x16x32_bf16 three times per block with the FP32
-//          operands split into bf16 hi + lo at LDS staging:
-//          a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation;
+//   FP32   v_mfma_f32_16x16x4f32, FP32 operands and accumulation.
+// Split BF16 (k_gemm_bf3): 128x128 output tiles on operands stored as two
+// bf16 planes (hi, lo), v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+//   BF16x3 a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi;
 //   BF16x4 the same with a_lo b_lo added first (four MFMAs per block).
 // The work list holds (chunk, member or split, I, J) per workgroup; work is
 // assigned so that consecutive items stay on one XCD (blockIdx % 8 selects
@@ -142,100 +143,6 @@ __device__ __forceinline__ void mainloop_k4(const Prob<T> &p, int I0, int J0,
     }
 }
 
-// ----------------------------------------------------------- BF16x3 path --
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
-}
-
-__device__ __forceinline__ void mainloop_bf16x3(const Prob<float> &p, int I0, int J0, f4 (&acc)[2][2]) {
-    constexpr int KB = 32, LD = 40;   // row pitch 80 bytes keeps 16-byte alignment
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[64][LD];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[64][LD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[64][LD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[64][LD];
-    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int wm = w / 2, wn = w % 2;
-    const int ar = tid / 4, ak = (tid % 4) * 8;      // A / NT pattern: 8 consecutive k
-    const int bk = tid / 8, bn = (tid % 8) * 8;      // NN pattern: 8 consecutive n
-    float ra[8], rb[8];
-    auto gload = [&](int k0) {
-        const bool aok = (I0 + ar) < p.arows;
-        const float *pa = p.A + (size_t)(I0 + ar) * p.lda + k0 + ak;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) ra[u] = aok ? pa[u] : 0.f;
-        if (p.nt) {
-            const bool bok = (J0 + ar) < p.brows;
-            const float *pb = p.B + (size_t)(J0 + ar) * p.ldb + k0 + ak;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) rb[u] = bok ? pb[u] : 0.f;
-        } else {
-            const float *pb = p.B + (size_t)(k0 + bk) * p.ldb + J0 + bn;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) rb[u] = pb[u];
-        }
-    };
-    auto sstore = [&]() {
-        bf8 h, l;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            __bf16 a, b;
-            split_bf16(ra[u], a, b);
-            h[u] = a;
-            l[u] = b;
-        }
-        *(bf8 *)&Ah[ar][ak] = h;
-        *(bf8 *)&Al[ar][ak] = l;
-        if (p.nt) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                __bf16 a, b;
-                split_bf16(rb[u], a, b);
-                h[u] = a;
-                l[u] = b;
-            }
-            *(bf8 *)&Bh[ar][ak] = h;
-            *(bf8 *)&Bl[ar][ak] = l;
-        } else {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                __bf16 a, b;
-                split_bf16(rb[u], a, b);
-                Bh[bn + u][bk] = a;
-                Bl[bn + u][bk] = b;
-            }
-        }
-    };
-    if (p.kbeg >= p.kend) return;
-    gload(p.kbeg);
-    sstore();
-    __syncthreads();
-    for (int k0 = p.kbeg; k0 < p.kend; k0 += KB) {
-        const bool more = k0 + KB < p.kend;
-        if (more) gload(k0 + KB);
-        bf8 ah[2], al[2], bh[2], bl[2];
-        const int kq = 8 * (lane / 16);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            ah[q] = *(const bf8 *)&Ah[wm * 32 + q * 16 + lane % 16][kq];
-            al[q] = *(const bf8 *)&Al[wm * 32 + q * 16 + lane % 16][kq];
-            bh[q] = *(const bf8 *)&Bh[wn * 32 + q * 16 + lane % 16][kq];
-            bl[q] = *(const bf8 *)&Bl[wn * 32 + q * 16 + lane % 16][kq];
-        }
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
-            }
-        __syncthreads();
-        if (more) sstore();
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------------ kernel --
 template <int PREC, int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *work, int nwork,
@@ -290,10 +197,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc[x][y][u] = 0;
-    if constexpr (PREC == 2)
-        mainloop_bf16x3(p, I0, J0, acc);
-    else
-        mainloop_k4<T>(p, I0, J0, acc);
+    mainloop_k4<T>(p, I0, J0, acc);
 
     const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     const int wm = w / 2, wn = w % 2;

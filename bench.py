@@ -73,7 +73,13 @@ def kernel_work(shapes, prec):
     return w
 
 
-def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=()):
+def fiducial_power(w):
+    """P_fid(k_c, z_m) of the workload's bands, in the driver's band order."""
+    kc, zc = w.binning.kband_centers(), w.binning.zbin_centers()
+    return np.array([[synth.pd13_power(k, z, *w.pd13) for k in kc] for z in zc]).reshape(-1)
+
+
+def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0):
     eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
@@ -116,9 +122,30 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
         raise RuntimeError("serialised and overlapped iterations differ")
     tm = eng.timings()
     shapes = chunk_shapes(eng)
+    # Newton-Raphson chain (untimed): each iteration's theta from the
+    # previous one's sums by the driver arithmetic, no smoother
+    chain = None
+    if nr > 0:
+        th = np.zeros(T)
+        chain = dict(ms=[], theta=[], sigma=None, rejected=[], damp=[])
+        for i in range(nr):
+            Fi, di = np.zeros((T, T)), np.zeros((3, T))
+            t0 = time.perf_counter()
+            eng.iterate(th, Fi, di)
+            chain["ms"].append((time.perf_counter() - t0) * 1e3)
+            res = power_estimates(Fi, di)
+            th = res["theta"]
+            chain["theta"].append(th)
+            if nr_floor > 0:   # stand-in for the reference's smoother: keep S + theta Q a covariance
+                th = np.maximum(th, -nr_floor * fiducial_power(w))
+            chain["sigma"] = res["sigma"]
+            chain["rejected"].append(sum(eng.chunk_result(cid)[2] != 0 for _, cid in accepted))
+            chain["damp"].append(float(res["damp"]))
+            log(f"[{precision}] Newton-Raphson iteration {i + 1}/{nr}: {chain['ms'][-1]:.1f} ms, "
+                f"{chain['rejected'][-1]} chunks rejected, damping {chain['damp'][-1]:.3g}")
     eng.close()
     return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms,
-                schedule=schedule)
+                schedule=schedule, chain=chain)
 
 
 def copy_calibration(mb: int):
@@ -196,6 +223,11 @@ def main():
     ap.add_argument("--autotune", default="", metavar="CAPS",
                     help="comma-separated caps on chunks per batch for the autotuned schedule "
                          "(tuned before the warm-up, outside the timed steps)")
+    ap.add_argument("--nr", type=int, default=0, metavar="N",
+                    help="after the timed steps, run N Newton-Raphson iterations (theta fed forward, no "
+                         "smoother) per precision and report the final P1D deviation of the mixed chain")
+    ap.add_argument("--nr-floor", type=float, default=0.0, metavar="F",
+                    help="feed theta forward as max(theta, -F P_fid) (a stand-in for the reference's smoother)")
     ap.add_argument("--save-sums", default="", metavar="NPZ",
                     help="write the FP64 and mixed Fisher and d, b, t sums to this .npz")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -218,7 +250,7 @@ def main():
         f"generated in {time.perf_counter() - t0:.1f} s")
 
     caps = tuple(int(x) for x in args.autotune.split(",") if x.strip())
-    opt = dict(graph=args.launch == "graph", tune_caps=caps)
+    opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor)
     res64 = None
     if not args.no_fp64:
         res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)
@@ -255,6 +287,19 @@ def main():
         errs["P1D_rel"] = float(np.max(np.abs(pm["theta"] - p64["theta"])[ok]) /
                                 np.max(np.abs(p64["theta"][ok])))
         cfg["max_rel_err_vs_fp64"] = errs
+        if args.nr and res64["chain"] and resm["chain"]:
+            c64, cm = res64["chain"], resm["chain"]
+            sg = c64["sigma"]
+            okc = np.isfinite(sg) & (sg > 0)
+            devs = [float(np.max(np.abs(a - b)[okc] / sg[okc])) for a, b in zip(cm["theta"], c64["theta"])]
+            cfg["newton_raphson"] = {"iterations": args.nr, "theta_floor_times_pfid": args.nr_floor,
+                                     "fp64_ms": c64["ms"], "mixed_ms": cm["ms"],
+                                     "P1D_max_dev_over_sigma_per_iteration": devs,
+                                     "rejected_chunks_fp64": c64["rejected"], "rejected_chunks_mixed": cm["rejected"],
+                                     "damping_fp64": c64["damp"], "damping_mixed": cm["damp"],
+                                     "theta_rms_over_sigma_final": float(np.sqrt(np.mean(
+                                         (c64["theta"][-1][okc] / sg[okc]) ** 2)))}
+            log("Newton-Raphson chain:", json.dumps(cfg["newton_raphson"]))
         if args.save_sums:
             np.savez(args.save_sums, F64=F64, dbt64=d64, Fm=Fm, dbtm=dm)
         log("errors vs fp64:", json.dumps(errs))

@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from lyspeq_amd import synth                                   # noqa: E402
-from lyspeq_amd.driver import power_estimates                  # noqa: E402
+from lyspeq_amd.driver import power_estimates, smooth_power    # noqa: E402
 from lyspeq_amd.qml import QMLEngine, KERNEL_NAMES             # noqa: E402
 
 METRIC = "chunks/s (F,d,b) mixed-prec vs FP64 on 1 MI355X; max rel err of F and P1D"
@@ -79,7 +79,8 @@ def fiducial_power(w):
     return np.array([[synth.pd13_power(k, z, *w.pd13) for k in kc] for z in zc]).reshape(-1)
 
 
-def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0):
+def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0,
+               nr_smooth=False):
     eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
@@ -127,7 +128,7 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
     chain = None
     if nr > 0:
         th = np.zeros(T)
-        chain = dict(ms=[], theta=[], sigma=None, rejected=[], damp=[])
+        chain = dict(ms=[], theta=[], sigma=[], rejected=[], damp=[])
         for i in range(nr):
             Fi, di = np.zeros((T, T)), np.zeros((3, T))
             t0 = time.perf_counter()
@@ -136,9 +137,12 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
             res = power_estimates(Fi, di)
             th = res["theta"]
             chain["theta"].append(th)
-            if nr_floor > 0:   # stand-in for the reference's smoother: keep S + theta Q a covariance
+            if nr_smooth:      # the reference's smoothing of the fits (py/smbivspline.py)
+                th = smooth_power(th, res["sigma"], w.binning.kband_centers(), w.binning.zbin_centers(),
+                                  fiducial_power(w))
+            elif nr_floor > 0:   # keep S + theta Q a covariance without smoothing
                 th = np.maximum(th, -nr_floor * fiducial_power(w))
-            chain["sigma"] = res["sigma"]
+            chain["sigma"].append(res["sigma"])
             chain["rejected"].append(sum(eng.chunk_result(cid)[2] != 0 for _, cid in accepted))
             chain["damp"].append(float(res["damp"]))
             log(f"[{precision}] Newton-Raphson iteration {i + 1}/{nr}: {chain['ms'][-1]:.1f} ms, "
@@ -228,6 +232,8 @@ def main():
                          "smoother) per precision and report the final P1D deviation of the mixed chain")
     ap.add_argument("--nr-floor", type=float, default=0.0, metavar="F",
                     help="feed theta forward as max(theta, -F P_fid) (a stand-in for the reference's smoother)")
+    ap.add_argument("--nr-smooth", action="store_true",
+                    help="feed the smoothed fits forward, as the reference's driver does")
     ap.add_argument("--save-sums", default="", metavar="NPZ",
                     help="write the FP64 and mixed Fisher and d, b, t sums to this .npz")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -250,7 +256,8 @@ def main():
         f"generated in {time.perf_counter() - t0:.1f} s")
 
     caps = tuple(int(x) for x in args.autotune.split(",") if x.strip())
-    opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor)
+    opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor,
+               nr_smooth=args.nr_smooth)
     res64 = None
     if not args.no_fp64:
         res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)
@@ -289,10 +296,14 @@ def main():
         cfg["max_rel_err_vs_fp64"] = errs
         if args.nr and res64["chain"] and resm["chain"]:
             c64, cm = res64["chain"], resm["chain"]
-            sg = c64["sigma"]
+            devs = []
+            for a, b, sg in zip(cm["theta"], c64["theta"], c64["sigma"]):   # each iteration's own sigma
+                okc = np.isfinite(sg) & (sg > 0)
+                devs.append(float(np.max(np.abs(a - b)[okc] / sg[okc])))
+            sg = c64["sigma"][-1]
             okc = np.isfinite(sg) & (sg > 0)
-            devs = [float(np.max(np.abs(a - b)[okc] / sg[okc])) for a, b in zip(cm["theta"], c64["theta"])]
-            cfg["newton_raphson"] = {"iterations": args.nr, "theta_floor_times_pfid": args.nr_floor,
+            cfg["newton_raphson"] = {"iterations": args.nr, "smoothed_fits": args.nr_smooth,
+                                     "theta_floor_times_pfid": args.nr_floor,
                                      "fp64_ms": c64["ms"], "mixed_ms": cm["ms"],
                                      "P1D_max_dev_over_sigma_per_iteration": devs,
                                      "rejected_chunks_fp64": c64["rejected"], "rejected_chunks_mixed": cm["rejected"],

@@ -61,3 +61,37 @@ def power_estimates(fisher_sum: np.ndarray, dbt_sum: np.ndarray) -> dict:
         sigma = np.sqrt(np.diag(sinv))
     return dict(fisher=fs, solver=solver, inverse=sinv, theta=theta, sigma=sigma,
                 weighted=weighted, damp=damp, dof=dof)
+
+
+def smooth_power(theta: np.ndarray, sigma: np.ndarray, kcenters: np.ndarray, zcenters: np.ndarray,
+                 pfid: np.ndarray, interp_log: bool = False) -> np.ndarray:
+    """The fits the next iteration uses, as OneDQuadraticPowerEstimate::
+    _smoothPowerSpectra makes them (core/quadratic_estimate.cpp:353-396):
+    writeSpectrumEstimates (:624-650) lists z ("%.3lf"), k, P = theta + P_fid
+    and e = sqrt((F^-1)_ii) ("%e") per band, z-major; py/smbivspline.py fits
+    scipy's SmoothBivariateSpline(z, k, P, w=1/e, s=len(e)), or with
+    interp_log (SmoothLogkLogP) one in (ln(1+z), ln k, ln P) with weights
+    P/e over the bands with P, e > 0, and evaluates it at every band;
+    _readScriptOutput (:340-348) subtracts P_fid again.  Bands with e <= 0
+    (no data), whose weight the script would make infinite, are left out of
+    the fit here.  kcenters (nk) and zcenters (nz) give the band grid; theta,
+    sigma and pfid have nz * nk values."""
+    from scipy.interpolate import SmoothBivariateSpline
+    nk, nz = len(kcenters), len(zcenters)
+    assert theta.size == sigma.size == pfid.size == nk * nz
+    zz = np.repeat(np.asarray(zcenters, float), nk)
+    kk = np.tile(np.asarray(kcenters, float), nz)
+    # the values the script reads back from the text file
+    z = np.array([float(f"{v:.3f}") for v in zz])
+    k = np.array([float(f"{v:e}") for v in kk])
+    p = np.array([float(f"{v:e}") for v in theta + pfid])
+    e = np.array([float(f"{v:e}") for v in np.nan_to_num(sigma, nan=0.0)])
+    if interp_log:
+        m = (p > 0) & (e > 0)
+        spl = SmoothBivariateSpline(np.log(1 + z[m]), np.log(k[m]), np.log(p[m]), w=p[m] / e[m], s=int(m.sum()))
+        smooth = np.exp(spl(np.log(1 + z), np.log(k), grid=False))
+    else:
+        m = e > 0
+        spl = SmoothBivariateSpline(z[m], k[m], p[m], w=1.0 / e[m], s=int(m.sum()))
+        smooth = spl(z, k, grid=False)
+    return smooth - pfid

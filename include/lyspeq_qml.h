@@ -307,6 +307,25 @@ int qml_set_graph(qml_ctx *ctx, int on);
 int qml_bootstrap(qml_ctx *ctx, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
                   double *realizations, double *mean, double *cov);
 
+/* The bootstrap's complete form (FastBootstrap off; PoissonBootstrapper::
+ * _oneBoot, core/bootstrapper.hpp:223-262, OneQSOEstimate::addBoot,
+ * core/one_qso_estimate.hpp:43-53): for every realisation b, one weight per
+ * spectrum in queue order (draws b nqso .. (b+1) nqso - 1 of
+ * qml_poisson_draws(seed), or weights[b nqso + q] if weights is not NULL),
+ * F_b = sum of p F_q over the spectra with p != 0 (F_q the spectrum's chunk
+ * Fisher matrices summed at their fisher_index_start) and the power
+ * realisation x_b solving F_b x = sum of p theta_q by Cholesky, bands without
+ * data left at zero (LAPACKE_safeSolveCho, mathtools/matrix_helper.cpp:
+ * 385-396).  A realisation whose F_b is not positive definite is invalid.
+ *   realizations (nboots x T, may be NULL): the valid x_b, in order, first
+ *     *nvalid rows; valid (nboots, may be NULL): 1 for a valid realisation;
+ *   mean, cov: over the valid realisations, as for qml_bootstrap.
+ * With unit weights every x_b is the driver's power estimate
+ * F_sum^-1 (d - b - t) of the included chunks. */
+int qml_bootstrap_complete(qml_ctx *ctx, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
+                           const int *weights, double *realizations, int *valid, int *nvalid, double *mean,
+                           double *cov);
+
 /* n unit-mean Poisson draws of PoissonRNG(seed) (core/bootstrapper.hpp:19-37):
  * std::poisson_distribution<int> over std::mt19937_64 seeded once.  With the
  * same C++ standard library as the reference's build the draws are the

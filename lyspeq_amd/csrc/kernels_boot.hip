@@ -1,4 +1,4 @@
-// Poisson bootstrap of the per-chunk results, FastBootstrap form
+// Poisson bootstrap of the per-chunk results.  FastBootstrap form
 // (core/bootstrapper.hpp:183-221,282-375; core/one_qso_estimate.cpp:200-239,
 // core/one_qso_estimate.hpp:55-60):
 //
@@ -10,6 +10,9 @@
 //   k_boot_mean       mean[i] = (sum over b of R[b][i]) / nboots, as sum * (1/n)
 //   k_boot_cov        cov[i][j] = sum over b of (R[b][i] - mean[i])(R[b][j] -
 //                     mean[j]), times 1/(nboots - 1) (dsyr per realisation)
+//
+// The complete form (k_boot_collapse_fisher, k_boot_build, the chunk
+// engine's batched Cholesky, k_boot_solve) follows below.
 //
 // All FP64.  Every output is summed in the reference's order (chunks, spectra,
 // realisations ascending), one thread per output element of a 64 x 64 tile,
@@ -136,14 +139,146 @@ __global__ __launch_bounds__(256) void k_boot_cov(const double *R, const double 
         }
 }
 
+// ------------------------------------------------ complete bootstrap --
+// The complete form (PoissonBootstrapper::_oneBoot, core/bootstrapper.hpp:
+// 223-262; OneQSOEstimate::addBoot, core/one_qso_estimate.hpp:43-53): per
+// realisation b, F_b = sum over spectra of p_qb F_q (upper) and x_b the
+// solution of F_b x = sum p_qb theta_q (LAPACKE_safeSolveCho,
+// mathtools/matrix_helper.cpp:385-396: empty diagonals set to 1, their x
+// zeroed).
+
+// Each spectrum's collapsed Fisher matrix (OneQSOEstimate::collapseBootstrap,
+// core/one_qso_estimate.cpp:224-234): the upper triangle of ndim x ndim at
+// fq_off[q], summed over its chunks in chunk order.  One block per spectrum.
+__global__ __launch_bounds__(256) void k_boot_collapse_fisher(const ChunkOut *outs, const double *fc,
+                                                              const int *qptr, const int *qchunk,
+                                                              const int *qstart, const int *qdim,
+                                                              const long long *fq_off, double *fq) {
+    const int q = blockIdx.x;
+    const int c0 = qptr[q], c1 = qptr[q + 1], is = qstart[q], nd = qdim[q];
+    double *F = fq + fq_off[q];
+    for (int e = threadIdx.x; e < nd * nd; e += blockDim.x) {
+        const int i = e / nd, j = e - i * nd;
+        double acc = 0.0;
+        if (j >= i) {
+            for (int k = c0; k < c1; ++k) {
+                const ChunkOut o = outs[qchunk[k]];
+                const int a = i + is - o.fis, b = j + is - o.fis;
+                if (a < 0 || b >= o.nq) continue;   // a <= b
+                acc += fc[o.fc_off + (size_t)a * o.nq + b];
+            }
+        }
+        F[e] = acc;
+    }
+}
+
+// F_b (full symmetric, Np x Np with identity padding, empty diagonals set to
+// 1) at mat + b * 3 Np^2 and the right-hand side at rhs + b * Np, for the
+// realisations b0 .. b0 + nb - 1 of the batch.  Grid (Np / 64, Np / 64, nb),
+// upper tiles only.  Element (i, j), j >= i, sums over the spectra covering
+// band i (band_q[band_ptr[i] ..], in queue order) that also cover j.
+__global__ __launch_bounds__(256) void k_boot_build(const int *W, int nqso, int b0, int T, int Np,
+                                                    const int *band_ptr, const int *band_q, const int *qstart,
+                                                    const int *qdim, const long long *fq_off, const double *fq,
+                                                    const double *theta, double *mat, double *rhs) {
+    const int I = blockIdx.y, J = blockIdx.x, b = blockIdx.z;
+    if (J < I) return;
+    const int *w = W + (size_t)(b0 + b) * nqso;
+    double *A = mat + (size_t)b * 3 * Np * Np;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int i = I * 64 + e / 64, j = J * 64 + e % 64;
+        if (j < i) continue;
+        double acc = 0.0;
+        if (i < T && j < T) {
+            for (int k = band_ptr[i]; k < band_ptr[i + 1]; ++k) {
+                const int q = band_q[k];
+                const int p = w[q];
+                const int is = qstart[q];
+                if (p == 0 || j >= is + qdim[q]) continue;
+                acc += (double)p * fq[fq_off[q] + (size_t)(i - is) * qdim[q] + (j - is)];
+            }
+            if (i == j && acc == 0.0) acc = 1.0;
+        } else if (i == j) {
+            acc = 1.0;
+        }
+        A[(size_t)i * Np + j] = acc;
+        A[(size_t)j * Np + i] = acc;
+    }
+    if (I == 0 && J == 0) {
+        for (int i = threadIdx.x; i < Np; i += 256) {
+            double acc = 0.0;
+            if (i < T)
+                for (int k = band_ptr[i]; k < band_ptr[i + 1]; ++k) {
+                    const int q = band_q[k];
+                    const int p = w[q];
+                    if (p != 0) acc += (double)p * theta[(size_t)q * T + i];
+                }
+            rhs[(size_t)b * Np + i] = acc;
+        }
+    }
+}
+
+// x = Li^T (Li rhs) per realisation (one block each), written to R[b0 + b]
+// (T values).  A band no weighted spectrum covers has a zero row and a zero
+// right-hand side with its diagonal set to 1, so its x is zero, as
+// safeSolveCho leaves it.
+__global__ __launch_bounds__(256) void k_boot_solve(const double *mat, const double *rhs, int b0, int T, int Np,
+                                                    double *R) {
+    const int b = blockIdx.x;
+    const double *Li = mat + (size_t)b * 3 * Np * Np + 2 * (size_t)Np * Np;
+    extern __shared__ double sh[];
+    double *y = sh;          // Np
+    const double *r = rhs + (size_t)b * Np;
+    for (int i = threadIdx.x; i < Np; i += 256) {
+        double s = 0.0;
+        for (int k = 0; k <= i; ++k) s += Li[(size_t)i * Np + k] * r[k];
+        y[i] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T; i += 256) {
+        double s = 0.0;
+        for (int k = i; k < Np; ++k) s += Li[(size_t)k * Np + i] * y[k];
+        R[(size_t)(b0 + b) * T + i] = s;
+    }
+}
+
+void launch_boot_collapse_fisher(hipStream_t st, const ChunkOut *outs, const double *fc, const int *qptr,
+                                 const int *qchunk, const int *qstart, const int *qdim, const long long *fq_off,
+                                 int nqso, double *fq) {
+    k_boot_collapse_fisher<<<nqso, 256, 0, st>>>(outs, fc, qptr, qchunk, qstart, qdim, fq_off, fq);
+}
+
+void launch_boot_build(hipStream_t st, const int *W, int nqso, int b0, int nb, int T, int Np, const int *band_ptr,
+                       const int *band_q, const int *qstart, const int *qdim, const long long *fq_off,
+                       const double *fq, const double *theta, double *mat, double *rhs) {
+    const int t = Np / 64;
+    k_boot_build<<<dim3(t, t, nb), 256, 0, st>>>(W, nqso, b0, T, Np, band_ptr, band_q, qstart, qdim, fq_off, fq,
+                                                 theta, mat, rhs);
+}
+
+void launch_boot_solve(hipStream_t st, const double *mat, const double *rhs, int b0, int nb, int T, int Np,
+                       double *R) {
+    k_boot_solve<<<nb, 256, (size_t)Np * sizeof(double), st>>>(mat, rhs, b0, T, Np, R);
+}
+
+void launch_boot_collapse(hipStream_t st, const ChunkOut *outs, const double *dbt, const int *qptr,
+                          const int *qchunk, int nqso, int T, double *theta) {
+    k_boot_collapse<<<nqso, 256, 0, st>>>(outs, dbt, qptr, qchunk, T, theta);
+}
+
+void launch_boot_meancov(hipStream_t st, const double *R, int nboots, int T, double *mean, double *cov) {
+    const int tt = (T + BT - 1) / BT;
+    k_boot_mean<<<(T + 255) / 256, 256, 0, st>>>(R, nboots, T, mean);
+    k_boot_cov<<<dim3(tt, tt), 256, 0, st>>>(R, mean, nboots, T, cov);
+}
+
 void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, const int *qptr, const int *qchunk,
                       int nqso, int T, const int *P, int nboots, double *theta, double *R, double *mean,
                       double *cov) {
-    k_boot_collapse<<<nqso, 256, 0, st>>>(outs, dbt, qptr, qchunk, T, theta);
+    launch_boot_collapse(st, outs, dbt, qptr, qchunk, nqso, T, theta);
     const int tt = (T + BT - 1) / BT;
     k_boot_realize<<<dim3(tt, (nboots + BT - 1) / BT), 256, 0, st>>>(P, theta, nqso, nboots, T, R);
-    k_boot_mean<<<(T + 255) / 256, 256, 0, st>>>(R, nboots, T, mean);
-    k_boot_cov<<<dim3(tt, tt), 256, 0, st>>>(R, mean, nboots, T, cov);
+    launch_boot_meancov(st, R, nboots, T, mean, cov);
 }
 
 }  // namespace qml

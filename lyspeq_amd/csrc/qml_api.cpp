@@ -1189,24 +1189,44 @@ int qml_poisson_draws(unsigned long long seed, long long n, int *out) {
     return QML_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// The chunks of each spectrum as a CSR list in chunk order (qptr: nqso + 1
+// offsets into qchunk); every spectrum must hold a chunk.
+int spectra_lists(const qml_ctx *c, int nqso, const int *qso_of_chunk, std::vector<int> &qptr,
+                  std::vector<int> &qchunk, const char *who) {
+    const int nch = (int)c->chunks.size();
+    qptr.assign(nqso + 1, 0);
+    for (int i = 0; i < nch; ++i) {
+        const int q = qso_of_chunk[i];
+        if (q < -1 || q >= nqso) return fail(QML_ERR_ARG, std::string(who) + ": spectrum index out of range");
+        if (q >= 0) ++qptr[q + 1];
+    }
+    for (int q = 0; q < nqso; ++q) {
+        if (qptr[q + 1] == 0) return fail(QML_ERR_ARG, std::string(who) + ": a spectrum without chunks");
+        qptr[q + 1] += qptr[q];
+    }
+    qchunk.resize(qptr[nqso]);
+    std::vector<int> fill(qptr.begin(), qptr.end() - 1);
+    for (int i = 0; i < nch; ++i)
+        if (qso_of_chunk[i] >= 0) qchunk[fill[qso_of_chunk[i]]++] = i;   // chunk order within a spectrum
+    return QML_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int qml_bootstrap(qml_ctx *c, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
                   double *realizations, double *mean, double *cov) {
     if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_bootstrap: commit and iterate first");
     if (nboots < 2 || nqso < 1 || !qso_of_chunk || !mean || !cov)
         return fail(QML_ERR_ARG, "qml_bootstrap: nboots >= 2, nqso >= 1 and the outputs are required");
-    const int nch = (int)c->chunks.size(), T = c->T;
-    std::vector<int> qptr(nqso + 1, 0), qchunk;
-    for (int i = 0; i < nch; ++i) {
-        const int q = qso_of_chunk[i];
-        if (q < -1 || q >= nqso) return fail(QML_ERR_ARG, "qml_bootstrap: spectrum index out of range");
-        if (q >= 0) ++qptr[q + 1];
-    }
-    for (int q = 0; q < nqso; ++q) qptr[q + 1] += qptr[q];
-    qchunk.resize(qptr[nqso]);
+    const int T = c->T;
+    std::vector<int> qptr, qchunk;
     {
-        std::vector<int> fill(qptr.begin(), qptr.end() - 1);
-        for (int i = 0; i < nch; ++i)
-            if (qso_of_chunk[i] >= 0) qchunk[fill[qso_of_chunk[i]]++] = i;   // chunk order within a spectrum
+        const int rc = spectra_lists(c, nqso, qso_of_chunk, qptr, qchunk, "qml_bootstrap");
+        if (rc != QML_OK) return rc;
     }
     // the weights of spectrum q are draws q * nboots .. (q + 1) * nboots - 1
     std::vector<int> P((size_t)nqso * nboots);
@@ -1247,6 +1267,152 @@ int qml_bootstrap(qml_ctx *c, int nboots, unsigned long long seed, int nqso, con
     };
     const int rc = run();
     release();
+    return rc;
+}
+
+int qml_bootstrap_complete(qml_ctx *c, int nboots, unsigned long long seed, int nqso, const int *qso_of_chunk,
+                           const int *weights, double *realizations, int *valid, int *nvalid, double *mean,
+                           double *cov) {
+    if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_bootstrap_complete: commit and iterate first");
+    if (nboots < 2 || nqso < 1 || !qso_of_chunk || !mean || !cov || !nvalid)
+        return fail(QML_ERR_ARG, "qml_bootstrap_complete: nboots >= 2, nqso >= 1 and the outputs are required");
+    const int T = c->T, Np = (int)align_up(T, 64);
+    std::vector<int> qptr, qchunk;
+    {
+        const int rc = spectra_lists(c, nqso, qso_of_chunk, qptr, qchunk, "qml_bootstrap_complete");
+        if (rc != QML_OK) return rc;
+    }
+    // istart and ndim of each spectrum (collapseBootstrap), its Fisher block
+    // offsets, and the spectra covering each band, in queue order
+    std::vector<int> qstart(nqso), qdim(nqso);
+    std::vector<long long> fq_off(nqso + 1, 0);
+    for (int q = 0; q < nqso; ++q) {
+        int lo = T, hi = 0;
+        for (int k = qptr[q]; k < qptr[q + 1]; ++k) {
+            const ChunkOut &o = c->outs[qchunk[k]];
+            lo = std::min(lo, o.fis);
+            hi = std::max(hi, o.fis + o.nq);
+        }
+        qstart[q] = lo;
+        qdim[q] = hi - lo;
+        fq_off[q + 1] = fq_off[q] + (long long)qdim[q] * qdim[q];
+    }
+    std::vector<int> band_ptr(T + 1, 0), band_q;
+    for (int q = 0; q < nqso; ++q)
+        for (int i = qstart[q]; i < qstart[q] + qdim[q]; ++i) ++band_ptr[i + 1];
+    for (int i = 0; i < T; ++i) band_ptr[i + 1] += band_ptr[i];
+    band_q.resize(band_ptr[T]);
+    {
+        std::vector<int> fill(band_ptr.begin(), band_ptr.end() - 1);
+        for (int q = 0; q < nqso; ++q)
+            for (int i = qstart[q]; i < qstart[q] + qdim[q]; ++i) band_q[fill[i]++] = q;
+    }
+    // weights: realisation-major, one draw per spectrum (_oneBoot draws one
+    // per spectrum in queue order for every realisation)
+    std::vector<int> W((size_t)nboots * nqso);
+    if (weights) {
+        std::copy(weights, weights + W.size(), W.begin());
+    } else {
+        const int rc = qml_poisson_draws(seed, (long long)W.size(), W.data());
+        if (rc != QML_OK) return rc;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    const int BATCH = std::min(nboots, 128);
+    std::vector<void *> allocs;
+    auto dalloc = [&](void **p, size_t bytes) -> hipError_t {
+        const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess) allocs.push_back(*p);
+        return e;
+    };
+    std::vector<double> R((size_t)nboots * T);
+    std::vector<int> status(nboots, 0);
+    auto run = [&]() -> int {
+        int *d_qptr, *d_qchunk, *d_qstart, *d_qdim, *d_bptr, *d_bq, *d_W, *d_status;
+        long long *d_fqoff;
+        double *d_fq, *d_theta, *d_mat, *d_rhs, *d_R, *d_mean, *d_cov;
+        ChunkDev *d_cd;
+        HIPCHK(dalloc((void **)&d_qptr, qptr.size() * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_qchunk, qchunk.size() * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_qstart, nqso * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_qdim, nqso * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_fqoff, (nqso + 1) * sizeof(long long)));
+        HIPCHK(dalloc((void **)&d_bptr, band_ptr.size() * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_bq, band_q.size() * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_W, W.size() * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_fq, (size_t)fq_off[nqso] * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_theta, (size_t)nqso * T * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_mat, (size_t)BATCH * 3 * Np * Np * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_rhs, (size_t)BATCH * Np * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_R, R.size() * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_status, BATCH * sizeof(int)));
+        HIPCHK(dalloc((void **)&d_mean, (size_t)T * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_cov, (size_t)T * T * sizeof(double)));
+        HIPCHK(dalloc((void **)&d_cd, BATCH * sizeof(ChunkDev)));
+        HIPCHK(hipMemcpy(d_qptr, qptr.data(), qptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_qchunk, qchunk.data(), qchunk.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_qstart, qstart.data(), nqso * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_qdim, qdim.data(), nqso * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_fqoff, fq_off.data(), (nqso + 1) * sizeof(long long), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_bptr, band_ptr.data(), band_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!band_q.empty())
+            HIPCHK(hipMemcpy(d_bq, band_q.data(), band_q.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_W, W.data(), W.size() * sizeof(int), hipMemcpyHostToDevice));
+        // realisation b of a batch is factorised as a "chunk" of size Np at
+        // b * 3 Np^2 (A, L, Li), its status at b
+        std::vector<ChunkDev> cd(BATCH);
+        for (int b = 0; b < BATCH; ++b) {
+            cd[b] = ChunkDev{};
+            cd[b].N = T;
+            cd[b].Np = Np;
+            cd[b].nt = Np / TILE;
+            cd[b].gidx = b;
+            cd[b].mat_off = (long long)b * 3 * Np * Np;
+        }
+        HIPCHK(hipMemcpy(d_cd, cd.data(), BATCH * sizeof(ChunkDev), hipMemcpyHostToDevice));
+        launch_boot_collapse(c->st, c->d_outs, c->d_dbt, d_qptr, d_qchunk, nqso, T, d_theta);
+        launch_boot_collapse_fisher(c->st, c->d_outs, c->d_fc, d_qptr, d_qchunk, d_qstart, d_qdim, d_fqoff, nqso,
+                                    d_fq);
+        HIPCHK(hipGetLastError());
+        LaunchCtx L{};
+        L.stream = c->st;
+        L.chunks = d_cd;
+        L.np_max = Np;
+        L.mat = d_mat;
+        for (int b0 = 0; b0 < nboots; b0 += BATCH) {
+            const int nb = std::min(BATCH, nboots - b0);
+            L.nchunks = nb;
+            launch_boot_build(c->st, d_W, nqso, b0, nb, T, Np, d_bptr, d_bq, d_qstart, d_qdim, d_fqoff, d_fq,
+                              d_theta, d_mat, d_rhs);
+            launch_cholesky(L, d_status);
+            launch_boot_solve(c->st, d_mat, d_rhs, b0, nb, T, Np, d_R);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(status.data() + b0, d_status, nb * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        HIPCHK(hipMemcpy(R.data(), d_R, R.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // the valid realisations, in order (allpowers + ii T, ii counting the valid ones)
+        int nv = 0;
+        for (int b = 0; b < nboots; ++b) {
+            if (valid) valid[b] = status[b] == 0;
+            if (status[b] != 0) continue;
+            if (nv != b) std::copy(R.begin() + (size_t)b * T, R.begin() + (size_t)(b + 1) * T, R.begin() + (size_t)nv * T);
+            ++nv;
+        }
+        *nvalid = nv;
+        if (realizations) std::copy(R.begin(), R.begin() + (size_t)nv * T, realizations);
+        if (nv < 2) return fail(QML_ERR_ARG, "qml_bootstrap_complete: fewer than two valid realisations");
+        HIPCHK(hipMemcpy(d_R, R.data(), (size_t)nv * T * sizeof(double), hipMemcpyHostToDevice));
+        launch_boot_meancov(c->st, d_R, nv, T, d_mean, d_cov);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipMemcpy(mean, d_mean, (size_t)T * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cov, d_cov, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost));
+        return QML_OK;
+    };
+    const int rc = run();
+    for (void *p : allocs) hipFree(p);
     return rc;
 }
 

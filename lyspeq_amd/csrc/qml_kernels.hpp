@@ -84,6 +84,22 @@ void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, c
                       int nqso, int T, const int *P, int nboots, double *theta, double *R, double *mean,
                       double *cov);
 
+void launch_boot_collapse(hipStream_t st, const ChunkOut *outs, const double *dbt, const int *qptr,
+                          const int *qchunk, int nqso, int T, double *theta);
+void launch_boot_meancov(hipStream_t st, const double *R, int nboots, int T, double *mean, double *cov);
+// The complete form: per-spectrum collapsed Fisher upper blocks (ndim x ndim
+// at fq_off); per batch of nb realisations from b0, F_b and its right-hand
+// side in the matrix layout of the batched Cholesky (A, L, Li per
+// realisation, Np x Np each), then x = Li^T Li rhs into R.
+void launch_boot_collapse_fisher(hipStream_t st, const ChunkOut *outs, const double *fc, const int *qptr,
+                                 const int *qchunk, const int *qstart, const int *qdim, const long long *fq_off,
+                                 int nqso, double *fq);
+void launch_boot_build(hipStream_t st, const int *W, int nqso, int b0, int nb, int T, int Np, const int *band_ptr,
+                       const int *band_q, const int *qstart, const int *qdim, const long long *fq_off,
+                       const double *fq, const double *theta, double *mat, double *rhs);
+void launch_boot_solve(hipStream_t st, const double *mat, const double *rhs, int b0, int nb, int T, int Np,
+                       double *R);
+
 // S/Q lookup tables (kernels_tables.hip)
 struct PD13 {
     double A, n, alpha, B, beta, lambda;

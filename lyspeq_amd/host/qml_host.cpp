@@ -180,6 +180,23 @@ void Engine::bootstrap(const std::vector<OneQSOEstimate *> &queue, int nboots, u
     check(qml_bootstrap(ctx_, nboots, seed, nq, qso.data(), realizations, mean, cov));
 }
 
+int Engine::bootstrapComplete(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed,
+                              double *mean, double *cov, double *realizations) {
+    commit();
+    std::vector<int> qso(chunks_.size(), -1);
+    int nq = 0;
+    for (OneQSOEstimate *q : queue) {
+        if (q->chunks.empty()) continue;
+        for (auto &c : q->chunks) qso[c->engineId()] = nq;
+        ++nq;
+    }
+    if (nq == 0) throw QMLError(QML_ERR_ARG, "Engine::bootstrapComplete: no spectrum holds a chunk");
+    int nvalid = 0;
+    check(qml_bootstrap_complete(ctx_, nboots, seed, nq, qso.data(), nullptr, realizations, nullptr, &nvalid, mean,
+                                 cov));
+    return nvalid;
+}
+
 // ------------------------------------------------------------------- Chunk --
 Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2, const ResolutionRows *rr) : eng_(&eng) {
     ResolutionRows own;

@@ -126,6 +126,11 @@ public:
     // TOTAL_KZ_BINS^2; realizations (nboots x TOTAL_KZ_BINS) may be null.
     void bootstrap(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed, double *mean,
                    double *cov, double *realizations = nullptr);
+    // The same with FastBootstrap off (qml_bootstrap_complete): a Cholesky
+    // solve per realisation; returns the number of valid realisations, whose
+    // solutions fill the first rows of realizations (may be null).
+    int bootstrapComplete(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed,
+                          double *mean, double *cov, double *realizations = nullptr);
 
     int totalKzBins() const { return bins_.totalKzBins(); }
     const Binning &binning() const { return bins_; }

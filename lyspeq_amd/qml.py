@@ -110,7 +110,7 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
            "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws", "qml_set_kernel_stages",
-           "qml_get_schedule"]
+           "qml_get_schedule", "qml_bootstrap_complete"]
 
 
 class _Schedule(C.Structure):
@@ -155,6 +155,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_schedule.argtypes = [vp, C.POINTER(_Schedule)]
     lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
+    ip = C.POINTER(C.c_int)
+    lib.qml_bootstrap_complete.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, ip, ip, dp, ip, ip, dp, dp]
     lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]
@@ -419,6 +421,26 @@ class QMLEngine:
                                        q.ctypes.data_as(C.POINTER(C.c_int)), _dptr(R) if R is not None else None,
                                        _dptr(mean), _dptr(cov)), self._lib)
         return R, mean, cov
+
+    def bootstrap_complete(self, nboots: int, qso_of_chunk, nqso: int, seed: int = 0, weights=None):
+        """The bootstrap's complete form (a Cholesky solve per realisation):
+        returns (valid realisations nvalid x T, valid flags, mean, cov)."""
+        ip = C.POINTER(C.c_int)
+        q = np.ascontiguousarray(qso_of_chunk, dtype=np.int32)
+        assert q.size == len(self.chunk_infos)
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.int32)
+            assert w.shape == (nboots, nqso)
+        R = np.zeros((nboots, self.T))
+        valid = np.zeros(nboots, dtype=np.int32)
+        nv = C.c_int(0)
+        mean, cov = np.zeros(self.T), np.zeros((self.T, self.T))
+        _check(self._lib.qml_bootstrap_complete(self._ctx, int(nboots), int(seed), int(nqso), q.ctypes.data_as(ip),
+                                                w.ctypes.data_as(ip) if w is not None else None, _dptr(R),
+                                                valid.ctypes.data_as(ip), C.byref(nv), _dptr(mean), _dptr(cov)),
+               self._lib)
+        return R[:nv.value], valid.astype(bool), mean, cov
 
     def set_kernel_stages(self, left: int, gram: int):
         """LDS stages (1 or 2) of the split-BF16 left product and Gram."""

@@ -14,13 +14,20 @@ product path never imports this module).
   (core/bootstrapper.hpp:282-375): mean = (sum of R[b]) * (1/nboots); cov =
   sum of (R[b] - mean)(R[b] - mean)^T, times 1/(nboots - 1).
 
-Pins (tests/test_bootstrap_oracle.py): with unit weights every realisation is
-the accumulated d - b - t sum of the included chunks and the covariance
-vanishes; with Poisson(1) weights the covariance tends to sum_q theta_q
+* ``collapse_fisher`` and ``complete_realizations``: the complete form
+  (FastBootstrap off), a Cholesky solve per realisation.
+
+Pins (tests/test_bootstrap_oracle.py).  Fast form: with unit weights every
+realisation is the accumulated d - b - t sum of the included chunks and the
+covariance vanishes; with Poisson(1) weights the covariance tends to sum_q theta_q
 theta_q^T (independent unit-variance weights), which a dropped mean
 subtraction or a wrong normalisation fails; a brute-force element loop on a
-tiny case.  The reference's Poisson draws themselves come from the C++
-standard library; no reference fixture holds them.
+tiny case.  Complete form: with unit weights every realisation is the
+driver's power estimate (F_sum)^-1 (d - b - t), which the driver restatement
+computes by its own LU path (pinned by the golden iteration files); a
+realisation whose F_b does not factorise is dropped.  The reference's Poisson
+draws themselves come from the C++ standard library; no reference fixture
+holds them.
 """
 from __future__ import annotations
 
@@ -56,3 +63,52 @@ def mean_covariance(R: np.ndarray):
     v = R - mean
     cov = (v.T @ v) * (1.0 / (n - 1))
     return mean, cov
+
+
+def collapse_fisher(chunks: Sequence[Tuple[int, int, np.ndarray]]):
+    """The spectrum's Fisher block (collapseBootstrap without FastBootstrap,
+    core/one_qso_estimate.cpp:224-234): chunks are (fisher_index_start, n_q,
+    F upper n_q x n_q); returns (istart, ndim x ndim upper)."""
+    istart = min(f for f, _, _ in chunks)
+    iend = max(f + n for f, n, _ in chunks)
+    F = np.zeros((iend - istart, iend - istart))
+    for f, n, fc in chunks:
+        F[f - istart:f - istart + n, f - istart:f - istart + n] += np.triu(fc)
+    return istart, F
+
+
+def complete_realizations(W: np.ndarray, collapsed, fishers, T: int, return_systems: bool = False):
+    """PoissonBootstrapper::_oneBoot (core/bootstrapper.hpp:223-262) with
+    OneQSOEstimate::addBoot (core/one_qso_estimate.hpp:43-53) and
+    LAPACKE_safeSolveCho (mathtools/matrix_helper.cpp:385-396).  W is
+    nboots x nqso (one weight per spectrum per realisation).  Returns the
+    valid realisations in order and the validity flags, and with
+    return_systems the (F_b, rhs) each valid one solves."""
+    from scipy.linalg import cho_factor, cho_solve
+    out, valid, systems = [], [], []
+    for w in W:
+        F = np.zeros((T, T))
+        r = np.zeros(T)
+        for q, p in enumerate(w):
+            if p == 0:
+                continue
+            istart, th = collapsed[q]
+            _, Fq = fishers[q]
+            n = th.size
+            F[istart:istart + n, istart:istart + n] += p * Fq
+            r[istart:istart + n] += p * th
+        S = np.triu(F) + np.triu(F, 1).T
+        empty = np.diag(S) == 0
+        S[empty, empty] = 1.0
+        try:
+            x = cho_solve(cho_factor(S, lower=False), r)
+        except np.linalg.LinAlgError:
+            valid.append(False)
+            continue
+        x[empty] = 0.0
+        out.append(x)
+        valid.append(True)
+        systems.append((S, r))
+    if return_systems:
+        return np.array(out).reshape(-1, T), np.array(valid), systems
+    return np.array(out).reshape(-1, T), np.array(valid)

@@ -104,3 +104,53 @@ def test_brute_force_element_loop():
         for j in range(T):
             c = sum((R[b, i] - mean[i]) * (R[b, j] - mean[j]) for b in range(nboots)) / (nboots - 1)
             assert abs(cov[i, j] - c) <= 1e-12 * max(1.0, abs(c))
+
+
+def _spd_chunks(rng, nqso, T, nk):
+    spectra = []
+    for _ in range(nqso):
+        chunks = []
+        for _ in range(rng.integers(1, 3)):
+            fis = int(rng.integers(0, T - 2 * nk))
+            nq = int(rng.integers(nk, 2 * nk + 1))
+            a = rng.standard_normal((nq, nq + 4))
+            chunks.append((fis, nq, np.triu(a @ a.T + nq * np.eye(nq)), rng.standard_normal((3, nq))))
+        spectra.append(chunks)
+    return spectra
+
+
+def test_complete_form_with_unit_weights_is_the_power_estimate():
+    """Every weight 1: F_b is the whole Fisher sum and x_b the driver's
+    theta = 0.5 (0.5 F)^-1 (d - b - t) (oracle/driver.py, its own LU path)."""
+    from oracle.driver import power_estimates
+    rng = np.random.default_rng(9)
+    T, nqso = 50, 8
+    spectra = _spd_chunks(rng, nqso, T, 6)
+    col = [ob.collapse([(f, n, dbt) for f, n, _, dbt in s]) for s in spectra]
+    fis = [ob.collapse_fisher([(f, n, fc) for f, n, fc, _ in s]) for s in spectra]
+    X, valid = ob.complete_realizations(np.ones((3, nqso), dtype=np.int32), col, fis, T)
+    assert valid.all() and X.shape == (3, T)
+    Fsum, dsum = np.zeros((T, T)), np.zeros((3, T))
+    for s in spectra:
+        for f, n, fc, dbt in s:
+            Fsum[f:f + n, f:f + n] += fc
+            dsum[:, f:f + n] += dbt
+    res = power_estimates(Fsum, dsum)
+    assert res["damp"] == 0
+    for x in X:
+        assert np.max(np.abs(x - res["theta"])) <= 1e-10 * np.max(np.abs(res["theta"]))
+
+
+def test_complete_form_drops_realisations_that_do_not_factorise():
+    rng = np.random.default_rng(10)
+    T, nqso = 30, 4
+    spectra = _spd_chunks(rng, nqso, T, 5)
+    col = [ob.collapse([(f, n, dbt) for f, n, _, dbt in s]) for s in spectra]
+    fis = [ob.collapse_fisher([(f, n, fc) for f, n, fc, _ in s]) for s in spectra]
+    W = np.ones((4, nqso), dtype=np.int32)
+    W[2, 0] = -3          # a negative weight makes F_b indefinite
+    X, valid = ob.complete_realizations(W, col, fis, T)
+    assert list(valid) == [True, True, False, True] and X.shape == (3, T)
+    W[1] = 0              # no spectrum: every band empty, x = 0
+    X, valid = ob.complete_realizations(W, col, fis, T)
+    assert valid[1] and not X[1].any()

@@ -88,3 +88,73 @@ def test_bootstrap_arguments_are_checked():
     with pytest.raises(Exception):
         eng.bootstrap(10, np.full(nch, 3), 2)              # spectrum index beyond nqso
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_complete_bootstrap_matches_oracle(precision):
+    """The complete form on the device (a Cholesky solve per realisation)
+    against the oracle's, on the same per-chunk results and draws; with unit
+    weights every realisation is the solution of the whole Fisher sum."""
+    print(f"[case] complete bootstrap {precision}", flush=True)
+    w = synth.desi_config(nchunks=24)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 14)
+    eng = QMLEngine(w.binning, precision=precision, max_batch_chunks=9)
+    synth.load_engine(eng, w, list(range(len(w.chunks))))
+    eng.commit()
+    T = eng.T
+    F, dbt = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(th, F, dbt)
+    nch = len(eng.chunk_infos)
+    qso = (np.arange(nch) // 3).astype(np.int32)       # spectra of three chunks
+    nqso = int(qso.max()) + 1
+    res = [eng.chunk_result(c) for c in range(nch)]
+    infos = eng.chunk_infos
+    col = [ob.collapse([(infos[c].fisher_index_start, infos[c].n_q_matrices, res[c][1])
+                        for c in range(nch) if qso[c] == s]) for s in range(nqso)]
+    fis = [ob.collapse_fisher([(infos[c].fisher_index_start, infos[c].n_q_matrices, res[c][0])
+                               for c in range(nch) if qso[c] == s]) for s in range(nqso)]
+    nboots, seed = 100, 5
+    X, valid, mean, cov = eng.bootstrap_complete(nboots, qso, nqso, seed=seed)
+    Wd = qml.poisson_draws(seed, nboots * nqso).reshape(nboots, nqso)
+    Xo, vo, systems = ob.complete_realizations(Wd, col, fis, T, return_systems=True)
+    assert np.array_equal(valid, vo) and X.shape == Xo.shape
+    # two correct Cholesky solves agree to about n cond eps, with the
+    # condition number of the diagonally equilibrated F_b (Cholesky is
+    # insensitive to diagonal scaling), per band relative to the band's own
+    # scale; the device solves through the explicit triangular inverse
+    eps = np.finfo(float).eps
+
+    def _kappa(S):
+        d = np.sqrt(np.diag(S))
+        return np.linalg.cond(S / np.outer(d, d))
+
+    worst, emax, kmax = 0.0, 0.0, 0.0
+    for xb, xo, (S, r) in zip(X, Xo, systems):
+        kappa = _kappa(S)
+        kmax = max(kmax, kappa)
+        d = np.sqrt(np.diag(S))
+        e = float(np.max(np.abs((xb - xo) * d)) / np.max(np.abs(xo * d)))
+        emax = max(emax, e)
+        bar = 20 * T * kappa * eps
+        worst = max(worst, e / bar)
+        assert e < bar
+    # mean and covariance of the device's realisations
+    mg, cg = ob.mean_covariance(X)
+    em, ec = _rel(mean, mg), _rel(cov, cg)
+    print(f"complete bootstrap {precision}: {valid.sum()} of {nboots} valid; equilibrated cond(F_b) up to "
+          f"{kmax:.1e}; realisations differ by {emax:.1e} (scaled), {worst:.3f} of the 20 n cond eps bar; "
+          f"mean {em:.1e} cov {ec:.1e}", flush=True)
+    assert em < 1e-12 and ec < 1e-11
+    # unit weights: every realisation is (sum of the chunk Fisher matrices)^-1 (d - b - t)
+    X1, v1, _, _ = eng.bootstrap_complete(4, qso, nqso, weights=np.ones((4, nqso), dtype=np.int32))
+    S = np.triu(F) + np.triu(F, 1).T
+    empty = np.diag(S) == 0
+    S[empty, empty] = 1.0
+    x = np.linalg.solve(S, dbt[0] - dbt[1] - dbt[2])
+    x[empty] = 0.0
+    assert v1.all()
+    d = np.sqrt(np.diag(S))
+    tol = 20 * T * _kappa(S) * eps
+    for xb in X1:
+        assert np.max(np.abs((xb - x) * d)) / np.max(np.abs(x * d)) < tol
+    eng.close()

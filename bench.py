@@ -80,7 +80,7 @@ def fiducial_power(w):
 
 
 def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0,
-               nr_smooth=False):
+               nr_smooth=False, boots=0):
     eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
@@ -147,9 +147,23 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
             chain["damp"].append(float(res["damp"]))
             log(f"[{precision}] Newton-Raphson iteration {i + 1}/{nr}: {chain['ms'][-1]:.1f} ms, "
                 f"{chain['rejected'][-1]} chunks rejected, damping {chain['damp'][-1]:.3g}")
+    boot = None
+    if boots > 0:   # untimed in the metric: the FastBootstrap and the complete form on the last sums
+        eng.iterate(np.zeros(T), np.zeros((T, T)), np.zeros((3, T)))
+        qso = np.arange(len(accepted), dtype=np.int32)
+        t0 = time.perf_counter()
+        eng.bootstrap(boots, qso, len(accepted), seed=0, realizations=False)
+        t_fast = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        X, valid, _, _ = eng.bootstrap_complete(boots, qso, len(accepted), seed=0)
+        t_full = time.perf_counter() - t0
+        boot = {"realisations": boots, "spectra": len(accepted), "fast_s": t_fast, "complete_s": t_full,
+                "complete_valid": int(valid.sum())}
+        log(f"[{precision}] bootstrap of {boots}: fast {t_fast:.2f} s, complete {t_full:.2f} s "
+            f"({boot['complete_valid']} valid)")
     eng.close()
     return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms,
-                schedule=schedule, chain=chain)
+                schedule=schedule, chain=chain, boot=boot)
 
 
 def copy_calibration(mb: int):
@@ -234,6 +248,9 @@ def main():
                     help="feed theta forward as max(theta, -F P_fid) (a stand-in for the reference's smoother)")
     ap.add_argument("--nr-smooth", action="store_true",
                     help="feed the smoothed fits forward, as the reference's driver does")
+    ap.add_argument("--bootstrap", type=int, default=0, metavar="N",
+                    help="after the timed steps, time both bootstrap forms with N realisations "
+                         "(every chunk its own spectrum)")
     ap.add_argument("--save-sums", default="", metavar="NPZ",
                     help="write the FP64 and mixed Fisher and d, b, t sums to this .npz")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -257,7 +274,7 @@ def main():
 
     caps = tuple(int(x) for x in args.autotune.split(",") if x.strip())
     opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor,
-               nr_smooth=args.nr_smooth)
+               nr_smooth=args.nr_smooth, boots=args.bootstrap)
     res64 = None
     if not args.no_fp64:
         res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)
@@ -340,6 +357,8 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": (ach / 1e9) if ach else None, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": (ach / HBM_PEAK) if ach else None, "traffic": None,
                 "launches": nb, "avg_launch_ms": km[dom] / nb}
+    if resm.get("boot"):
+        cfg["bootstrap"] = resm["boot"]
     step_flops = algorithmic_flops(resm["shapes"])
     cfg["step_algorithmic_tflop"] = step_flops / 1e12
     cfg["step_tflops_achieved"] = step_flops / (ms / 1e3) / 1e12

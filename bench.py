@@ -195,6 +195,18 @@ def measured_traffic(kernel: str, precision: str, workload: str):
     return float(t["hbm_bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
+def measured_mfma_busy(kernel: str, precision: str):
+    """MFMA-busy fraction of a dense product from the committed counter
+    summary (profiles/mfma_util_products_desi-256.json), if recorded."""
+    path = os.path.join(TRAFFIC_DIR, "mfma_util_products_desi-256.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    v = t.get("mfma_busy", {}).get(precision, {}).get(kernel)
+    return v, (os.path.relpath(path, ROOT) if v is not None else None)
+
+
 def cpu_baseline(w, budget_s=20.0, max_chunks=48):
     """The CPU oracle (numpy + threaded LAPACK/BLAS) on the first chunks of
     the same batch, until budget_s of CPU work is spent."""
@@ -345,6 +357,11 @@ def main():
         roof = {"kernel": dom, "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
                 "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": traffic,
                 "launches": nb, "avg_launch_ms": km[dom] / nb}
+        busy, bsrc = measured_mfma_busy(dom, args.precision)
+        if bsrc:
+            # issued MFMA-pipe occupancy from counters (three MFMAs per split product)
+            roof["mfma_busy_issued"] = busy
+            roof["mfma_busy_source"] = bsrc
         if tsrc:
             roof["traffic_source"] = tsrc
             # the same launches against the HBM roof: at split-BF16 operand

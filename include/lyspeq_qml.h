@@ -35,7 +35,10 @@ extern "C" {
 #define QML_ERR_NO_TABLE -4     /* (R, dv) not among the tables
                                    (core/chunk_estimate.cpp:198-206) */
 #define QML_ERR_EMPTY -5        /* all pixels masked (io/qso_file.cpp:150-153) */
-#define QML_ERR_MEMORY -6       /* device memory budget too small for a chunk */
+#define QML_ERR_MEMORY -6       /* the chunk's matrices exceed the device scratch budget:
+                                   Chunk::_setStoredMatrices' "Not all matrices are
+                                   stored." (core/chunk_estimate.cpp:258-282); the
+                                   chunk is not registered */
 #define QML_ERR_LIMIT -7        /* a size beyond this build's limits */
 
 /* Per-chunk status after an iteration (qml_get_chunk_result). */
@@ -180,8 +183,9 @@ int qml_add_table(qml_ctx *ctx, int R_fwhm, double dv_kms, int nv, double v_leng
                   int nz, const double *deriv, const double *signal, int *r_index);
 
 /* Registers a chunk (the work of Chunk::Chunk).  On QML_ERR_SHORT_CHUNK,
- * QML_ERR_NO_TABLE or QML_ERR_EMPTY the chunk is not registered: the caller
- * skips it, as OneQSOEstimate does (core/one_qso_estimate.cpp:115-125).
+ * QML_ERR_NO_TABLE, QML_ERR_EMPTY or QML_ERR_MEMORY the chunk is not
+ * registered: the caller skips it, as OneQSOEstimate does
+ * (core/one_qso_estimate.cpp:115-125).
  * *chunk_id receives the chunk's index. */
 int qml_add_chunk(qml_ctx *ctx, const qml_chunk_in *in, qml_chunk_info *info, int *chunk_id);
 

@@ -319,6 +319,34 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
     return QML_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Per-chunk scratch (elements of each arena) at padded size Np with m stack members.
+struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
+Need chunk_need(int prec, const ChunkHost &ch, size_t Np, size_t m) {
+    Need n{};
+    const size_t nt = Np / TILE, ntu = nt * (nt + 1) / 2;
+    const size_t mt = (m + TILE - 1) / TILE;
+    // A, L, Li, S, then for a resolution-matrix chunk its kbu Q_k and S
+    n.mat = (4 + (ch.rm ? (size_t)ch.kbu + 1 : 0)) * Np * Np;
+    n.vec = (size_t)VEC_SLOTS * Np;
+    n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
+    n.sj = m * MAX_NU * MAX_NU + ch.n;
+    n.stk = m * Np * Np;
+    n.lic = (prec == 0) ? 0 : Np * Np;
+    n.gram = ntu * (mt * (mt + 1) / 2) * TILE * TILE;
+    return n;
+}
+size_t need_bytes(int prec, const Need &n) {
+    const size_t es = (prec == 0) ? 8 : 4;
+    // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
+    return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
+}
+}  // namespace
+
+extern "C" {
+
 static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm, qml_chunk_info *info,
                           int *chunk_id) {
     if (!c || !in || !c->bins_set) return fail(QML_ERR_ARG, "qml_add_chunk: set bins first");
@@ -509,6 +537,17 @@ static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm
     inf.upper_z = zhi;
     inf.median_z = zmed;
     if (info) *info = inf;
+    {   // the matrices must fit: Chunk::_setStoredMatrices (core/chunk_estimate.cpp:258-282)
+        // throws "Not all matrices are stored.", here against the device scratch budget
+        const size_t bt = need_bytes(c->prec, chunk_need(c->prec, ch, align_up(N, TILE), (size_t)ch.n + 2));
+        if (bt > (size_t)(c->scratch_gb * 1073741824.0)) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "Not all matrices are stored: the chunk needs %.2f GB of device scratch, above the "
+                          "%.2f GB budget", bt / 1073741824.0, c->scratch_gb);
+            return fail(QML_ERR_MEMORY, msg);
+        }
+    }
     c->chunks.push_back(std::move(ch));
     if (chunk_id) *chunk_id = (int)c->chunks.size() - 1;
     return QML_OK;
@@ -566,26 +605,8 @@ void release_plan(qml_ctx *c) {
 int plan_batches(qml_ctx *c) {
     const int nch = (int)c->chunks.size();
     const size_t es = (c->prec == 0) ? 8 : 4;
-    // per-chunk scratch sizes (elements of each arena)
-    struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
-    auto need = [&](const ChunkHost &ch) {
-        Need n{};
-        const size_t Np = ch.Np, m = ch.m, nt = Np / TILE, ntu = nt * (nt + 1) / 2;
-        const size_t mt = (m + TILE - 1) / TILE;
-        // A, L, Li, S, then for a resolution-matrix chunk its kbu Q_k and S
-        n.mat = (4 + (ch.rm ? (size_t)ch.kbu + 1 : 0)) * Np * Np;
-        n.vec = (size_t)VEC_SLOTS * Np;
-        n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
-        n.sj = m * MAX_NU * MAX_NU + ch.n;
-        n.stk = m * Np * Np;
-        n.lic = (c->prec == 0) ? 0 : Np * Np;
-        n.gram = ntu * (mt * (mt + 1) / 2) * TILE * TILE;
-        return n;
-    };
-    auto bytes = [&](const Need &n) {
-        // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
-        return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
-    };
+    auto need = [&](const ChunkHost &ch) { return chunk_need(c->prec, ch, ch.Np, ch.m); };
+    auto bytes = [&](const Need &n) { return need_bytes(c->prec, n); };
     std::vector<int> order(nch);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {

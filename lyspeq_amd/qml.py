@@ -47,8 +47,13 @@ class EmptyChunkError(QMLError):
     """io/qso_file.cpp:150-153."""
 
 
+class ChunkMemoryError(QMLError):
+    """The chunk's matrices do not fit the device scratch budget
+    (Chunk::_setStoredMatrices, core/chunk_estimate.cpp:258-282)."""
+
+
 _ERRORS = {QML_ERR_SHORT_CHUNK: ShortChunkError, QML_ERR_NO_TABLE: TableNotFoundError,
-           QML_ERR_EMPTY: EmptyChunkError}
+           QML_ERR_EMPTY: EmptyChunkError, QML_ERR_MEMORY: ChunkMemoryError}
 
 
 class _Opts(C.Structure):

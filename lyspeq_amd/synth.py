@@ -384,7 +384,7 @@ def load_engine(engine, w: Workload, chunk_ids: Optional[List[int]] = None):
                 cid, _ = engine.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
             ok.append((i, cid))
         except QMLError as e:
-            if e.code in (-3, -4, -5):
+            if e.code in (-3, -4, -5, -6):   # the reference's constructor rejections
                 continue
             raise
     return ok

@@ -136,3 +136,37 @@ def test_rejected_factorisation_leaves_the_chunk_out(precision):
         if s == 30.0:
             assert nfail == len(acc) and not F.any() and not d.any()
     eng.close()
+
+
+def test_chunk_beyond_the_scratch_budget_is_refused_at_registration():
+    """Chunk::_setStoredMatrices (core/chunk_estimate.cpp:258-282) throws
+    "Not all matrices are stored." and the chunk is skipped; here a chunk
+    whose scratch exceeds the engine's budget is refused by qml_add_chunk,
+    and a smaller one is committed and iterated as usual.  Under a 0.1 GB
+    budget a DESI chunk of N >= 500 needs about 0.25 GB or more in split
+    BF16, one of N <= 250 a few tens of MB."""
+    from lyspeq_amd.qml import ChunkMemoryError
+    _print_case("scratch budget")
+    wd = synth.desi_config(nchunks=24)
+    by_size = sorted(wd.chunks, key=lambda c: len(c["wave"]))
+    c = by_size[-1]
+    assert len(c["wave"]) >= 500
+    eng = QMLEngine(wd.binning, precision="bf16x3", scratch_gb=0.1)
+    for i, (R, dv) in enumerate(wd.pairs):
+        eng.add_table(R, dv, wd.deriv[i], wd.signal[i], wd.v_length)
+    with pytest.raises(ChunkMemoryError, match="Not all matrices are stored"):
+        eng.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
+    cid = None
+    for small in by_size[:4]:          # the shortest chunk the constructor accepts
+        try:
+            cid, info = eng.add_chunk(small["wave"], small["delta"], small["ivar"], small["R"])
+            break
+        except (ShortChunkError, TableNotFoundError, EmptyChunkError):
+            continue
+    assert cid is not None and info.size <= 250
+    eng.commit()
+    T = eng.T
+    F, d = np.zeros((T, T)), np.zeros((3, T))
+    eng.iterate(np.zeros(T), F, d)
+    assert np.abs(F).max() > 0 and eng.chunk_result(cid)[2] == 0 and cid == 0
+    eng.close()

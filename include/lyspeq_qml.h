@@ -330,6 +330,20 @@ int qml_bootstrap_complete(qml_ctx *ctx, int nboots, unsigned long long seed, in
                            const int *weights, double *realizations, int *valid, int *nvalid, double *mean,
                            double *cov);
 
+/* The median bootstrap of saved realisations (PoissonBootstrapper::
+ * _medianBootstrap, core/bootstrapper.hpp:301-346,376-390, reached through
+ * PoissonBootstrapper::run() as lyspeqBootStats.cpp:35-36 calls it; medians
+ * by stats::medianOfUnsortedVector, mathtools/stats.cpp:8-28).
+ *   realizations: nboots x T, one realisation per row (host);
+ *   median (T): per band the median over the realisations: the middle value,
+ *     or the half-sum of the two middle values for even nboots;
+ *   mad_cov (T x T): for j >= i the median over the realisations of
+ *     (R[b][i] - median[i]) (R[b][j] - median[j]), mirrored to j < i.
+ * The selected values are the sorted arrays' bit for bit.  Needs a context
+ * only for its device and stream (no chunks or commit); T <= 65535. */
+int qml_bootstrap_median(qml_ctx *ctx, int nboots, int T, const double *realizations, double *median,
+                         double *mad_cov);
+
 /* n unit-mean Poisson draws of PoissonRNG(seed) (core/bootstrapper.hpp:19-37):
  * std::poisson_distribution<int> over std::mt19937_64 seeded once.  With the
  * same C++ standard library as the reference's build the draws are the

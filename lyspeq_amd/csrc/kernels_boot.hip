@@ -281,4 +281,167 @@ void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, c
     launch_boot_meancov(st, R, nboots, T, mean, cov);
 }
 
+// ---------------------------------------------------------------------------
+// Median bootstrap (PoissonBootstrapper::_medianBootstrap,
+// core/bootstrapper.hpp:301-346,376-390, with stats::medianOfUnsortedVector,
+// mathtools/stats.cpp:8-28): per band the median over the realisations, the
+// deviations from it, and per band pair (i <= j) the median of the products
+// of the deviations, mirrored to the lower triangle.  A median of n values is
+// the value of rank n/2 for odd n and the half-sum of ranks n/2 - 1 and n/2
+// for even n, the ranks of the ascending order.
+//
+// Each median is a radix select over order-preserving 64-bit keys of the
+// FP64 values: eight passes of 8 bits, most significant first, each a
+// 256-bin LDS histogram of the values still matching the selected prefix and
+// a block scan that picks the digit holding the rank; for even n one more
+// pass counts the values at or below the selected one and finds the next
+// larger value.  The selected values are the sorted array's, bit for bit;
+// products are formed the same way in every pass (one FP64 multiply).
+// Realisations are transposed first so that a band's values are contiguous.
+
+namespace {
+
+// Ascending keys are ascending values; -0 sorts below +0 (the two are equal
+// values, so the median does not depend on it).
+__device__ __forceinline__ unsigned long long order_key(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k) {
+    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+template <bool PROD>
+__device__ __forceinline__ unsigned long long median_term(const double *a, const double *b, int i) {
+    return order_key(PROD ? a[i] * b[i] : a[i]);
+}
+
+struct SelectLds {
+    unsigned int hist[256];
+    unsigned int wave_tot[4];
+    unsigned long long wave_min[4];
+    unsigned int digit, rank, count_le;
+};
+
+// The median of the n values a[i] (PROD false) or a[i] * b[i] (PROD true).
+// Called by all 256 threads of the block; every thread returns the median.
+template <bool PROD>
+__device__ double block_median(const double *a, const double *b, int n, SelectLds &s) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned int r_lo = (n % 2 == 0) ? (unsigned)(n / 2 - 1) : (unsigned)(n / 2);
+    unsigned long long prefix = 0, mask = 0;
+    unsigned int r = r_lo;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        s.hist[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += 256) {
+            const unsigned long long k = median_term<PROD>(a, b, i);
+            if ((k & mask) == prefix) atomicAdd(&s.hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        // inclusive scan of the 256 bins: within the wave, then across waves
+        const unsigned int c = s.hist[tid];
+        unsigned int incl = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) s.wave_tot[wave] = incl;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) incl += s.wave_tot[w];
+        const unsigned int excl = incl - c;
+        if (excl <= r && r < incl) {   // exactly one bin holds the rank
+            s.digit = (unsigned)tid;
+            s.rank = r - excl;
+        }
+        __syncthreads();
+        prefix |= (unsigned long long)s.digit << shift;
+        mask |= 0xffull << shift;
+        r = s.rank;
+        __syncthreads();
+    }
+    const double lo = key_value(prefix);
+    if (n % 2 != 0) return lo;
+    // even n: the value of rank n/2 is lo itself when more than n/2 values
+    // are at or below it, else the smallest value above it
+    if (tid == 0) s.count_le = 0;
+    __syncthreads();
+    unsigned int cnt = 0;
+    unsigned long long mn = ~0ull;
+    for (int i = tid; i < n; i += 256) {
+        const unsigned long long k = median_term<PROD>(a, b, i);
+        if (k <= prefix) ++cnt;
+        else mn = k < mn ? k : mn;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        cnt += __shfl_xor(cnt, d, 64);
+        const unsigned long long o = __shfl_xor(mn, d, 64);
+        mn = o < mn ? o : mn;
+    }
+    if (lane == 0) {
+        s.wave_tot[wave] = cnt;
+        s.wave_min[wave] = mn;
+    }
+    __syncthreads();
+    unsigned int tot = 0;
+    unsigned long long gmin = ~0ull;
+    for (int w = 0; w < 4; ++w) {
+        tot += s.wave_tot[w];
+        gmin = s.wave_min[w] < gmin ? s.wave_min[w] : gmin;
+    }
+    __syncthreads();
+    const double hi = (tot >= (unsigned)(n / 2 + 1)) ? lo : key_value(gmin);
+    return (lo + hi) / 2;
+}
+
+}  // namespace
+
+// R (nboots x T, one realisation per row) to X (T x nboots), 64 x 64 tiles.
+__global__ __launch_bounds__(256) void k_boot_transpose(const double *R, int nboots, int T, double *X) {
+    __shared__ double tile[64][65];
+    const int b0 = blockIdx.y * 64, i0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int b = b0 + rr, i = i0 + tx;
+        tile[rr][tx] = (b < nboots && i < T) ? R[(size_t)b * T + i] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int i = i0 + rr, b = b0 + tx;
+        if (i < T && b < nboots) X[(size_t)i * nboots + b] = tile[tx][rr];
+    }
+}
+
+// One block per band: median[i], then X[i][:] -= median[i] (_calcuateMedian,
+// and the subtraction at the start of _calcuateMadCovariance).
+__global__ __launch_bounds__(256) void k_boot_median(double *X, int nboots, double *median) {
+    __shared__ SelectLds s;
+    double *x = X + (size_t)blockIdx.x * nboots;
+    const double m = block_median<false>(x, x, nboots, s);
+    if (threadIdx.x == 0) median[blockIdx.x] = m;
+    for (int i = threadIdx.x; i < nboots; i += 256) x[i] -= m;
+}
+
+// One block per band pair j >= i (blocks with j < i return at once):
+// mad[i][j] = mad[j][i] = median over b of X[i][b] X[j][b].
+__global__ __launch_bounds__(256) void k_boot_mad(const double *X, int nboots, int T, double *mad) {
+    const int j = blockIdx.x, i = blockIdx.y;
+    if (j < i) return;
+    __shared__ SelectLds s;
+    const double m = block_median<true>(X + (size_t)i * nboots, X + (size_t)j * nboots, nboots, s);
+    if (threadIdx.x == 0) {
+        mad[(size_t)i * T + j] = m;
+        mad[(size_t)j * T + i] = m;
+    }
+}
+
+void launch_boot_median(hipStream_t st, const double *R, int nboots, int T, double *X, double *median,
+                        double *mad) {
+    k_boot_transpose<<<dim3((T + 63) / 64, (nboots + 63) / 64), 256, 0, st>>>(R, nboots, T, X);
+    k_boot_median<<<T, 256, 0, st>>>(X, nboots, median);
+    k_boot_mad<<<dim3(T, T), 256, 0, st>>>(X, nboots, T, mad);
+}
+
 }  // namespace qml

@@ -1437,6 +1437,35 @@ int qml_bootstrap_complete(qml_ctx *c, int nboots, unsigned long long seed, int 
     return rc;
 }
 
+int qml_bootstrap_median(qml_ctx *c, int nboots, int T, const double *realizations, double *median,
+                         double *mad_cov) {
+    if (!c || nboots < 1 || T < 1 || !realizations || !median || !mad_cov)
+        return fail(QML_ERR_ARG, "qml_bootstrap_median: a context, nboots >= 1, T >= 1 and every array are required");
+    if (T > 65535) return fail(QML_ERR_LIMIT, "qml_bootstrap_median: more than 65535 bands");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    const size_t nrt = (size_t)nboots * T;
+    double *d_R = nullptr, *d_X = nullptr, *d_med = nullptr, *d_mad = nullptr;
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc(&d_R, nrt * sizeof(double)));
+        HIPCHK(hipMalloc(&d_X, nrt * sizeof(double)));
+        HIPCHK(hipMalloc(&d_med, (size_t)T * sizeof(double)));
+        HIPCHK(hipMalloc(&d_mad, (size_t)T * T * sizeof(double)));
+        HIPCHK(hipMemcpy(d_R, realizations, nrt * sizeof(double), hipMemcpyHostToDevice));
+        launch_boot_median(c->st, d_R, nboots, T, d_X, d_med, d_mad);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipMemcpy(median, d_med, (size_t)T * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mad_cov, d_mad, (size_t)T * T * sizeof(double), hipMemcpyDeviceToHost));
+        return QML_OK;
+    };
+    const int rc = run();
+    void *ps[] = {d_R, d_X, d_med, d_mad};
+    for (void *p : ps) if (p) hipFree(p);
+    return rc;
+}
+
 int qml_num_chunks(qml_ctx *c) { return c ? (int)c->chunks.size() : 0; }
 int qml_total_kz_bins(qml_ctx *c) { return c ? c->T : 0; }
 

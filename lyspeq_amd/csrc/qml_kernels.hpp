@@ -87,6 +87,10 @@ void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, c
 void launch_boot_collapse(hipStream_t st, const ChunkOut *outs, const double *dbt, const int *qptr,
                           const int *qchunk, int nqso, int T, double *theta);
 void launch_boot_meancov(hipStream_t st, const double *R, int nboots, int T, double *mean, double *cov);
+// Median bootstrap: R nboots x T (device) -> median (T) and the median
+// covariance (T x T); X is T x nboots scratch (deviations from the median).
+void launch_boot_median(hipStream_t st, const double *R, int nboots, int T, double *X, double *median,
+                        double *mad);
 // The complete form: per-spectrum collapsed Fisher upper blocks (ndim x ndim
 // at fq_off); per batch of nb realisations from b0, F_b and its right-hand
 // side in the matrix layout of the batched Cholesky (A, L, Li per

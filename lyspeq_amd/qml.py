@@ -115,7 +115,7 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
            "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws", "qml_set_kernel_stages",
-           "qml_get_schedule", "qml_bootstrap_complete"]
+           "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median"]
 
 
 class _Schedule(C.Structure):
@@ -162,6 +162,7 @@ def load_library(path: str = LIB_PATH):
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     ip = C.POINTER(C.c_int)
     lib.qml_bootstrap_complete.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, ip, ip, dp, ip, ip, dp, dp]
+    lib.qml_bootstrap_median.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
     lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]
@@ -446,6 +447,16 @@ class QMLEngine:
                                                 valid.ctypes.data_as(ip), C.byref(nv), _dptr(mean), _dptr(cov)),
                self._lib)
         return R[:nv.value], valid.astype(bool), mean, cov
+
+    def bootstrap_median(self, realizations):
+        """The median bootstrap of realisations (nboots x T): returns (median
+        T, median covariance T x T).  Needs no chunks or commit."""
+        R = np.ascontiguousarray(realizations, dtype=np.float64)
+        nboots, T = R.shape
+        med, mad = np.zeros(T), np.zeros((T, T))
+        _check(self._lib.qml_bootstrap_median(self._ctx, int(nboots), int(T), _dptr(R), _dptr(med), _dptr(mad)),
+               self._lib)
+        return med, mad
 
     def set_kernel_stages(self, left: int, gram: int):
         """LDS stages (1 or 2) of the split-BF16 left product and Gram."""

@@ -16,6 +16,13 @@ product path never imports this module).
 
 * ``collapse_fisher`` and ``complete_realizations``: the complete form
   (FastBootstrap off), a Cholesky solve per realisation.
+* ``median`` and ``median_covariance``: _medianBootstrap, _calcuateMedian and
+  _calcuateMadCovariance (core/bootstrapper.hpp:301-346,376-390) with
+  stats::medianOfUnsortedVector / medianOfSortedArray (mathtools/stats.cpp:
+  8-28): a band's median over the realisations (middle value, or the half-sum
+  of the two middle values for even counts), the deviations from it, and per
+  band pair j >= i the median of the products of the deviations
+  (vector_multiply, mathtools/matrix_helper.hpp:47-52), mirrored.
 
 Pins (tests/test_bootstrap_oracle.py).  Fast form: with unit weights every
 realisation is the accumulated d - b - t sum of the included chunks and the
@@ -25,7 +32,11 @@ subtraction or a wrong normalisation fails; a brute-force element loop on a
 tiny case.  Complete form: with unit weights every realisation is the
 driver's power estimate (F_sum)^-1 (d - b - t), which the driver restatement
 computes by its own LU path (pinned by the golden iteration files); a
-realisation whose F_b does not factorise is dropped.  The reference's Poisson
+realisation whose F_b does not factorise is dropped.  Median form: a brute
+loop over sorted() lists on a tiny case; numpy's median (the same
+definition); for odd counts the diagonal is the squared median absolute
+deviation (scipy.stats.median_abs_deviation, scale 1), which a dropped median
+subtraction fails.  The reference's Poisson
 draws themselves come from the C++ standard library; no reference fixture
 holds them.
 """
@@ -112,3 +123,31 @@ def complete_realizations(W: np.ndarray, collapsed, fishers, T: int, return_syst
     if return_systems:
         return np.array(out).reshape(-1, T), np.array(valid), systems
     return np.array(out).reshape(-1, T), np.array(valid)
+
+
+def _median_rows(X: np.ndarray) -> np.ndarray:
+    """Median of each row: the sorted middle value, or (s[n/2-1] + s[n/2]) / 2."""
+    n = X.shape[-1]
+    if n % 2:
+        return np.partition(X, n // 2, axis=-1)[..., n // 2]
+    p = np.partition(X, (n // 2 - 1, n // 2), axis=-1)
+    return (p[..., n // 2 - 1] + p[..., n // 2]) / 2
+
+
+def median(R: np.ndarray) -> np.ndarray:
+    """R: nboots x T realisations; per band median."""
+    return _median_rows(np.ascontiguousarray(R.T))
+
+
+def median_covariance(R: np.ndarray):
+    """(median T, median covariance T x T) of realisations R (nboots x T)."""
+    X = np.ascontiguousarray(R.T, dtype=np.float64)
+    med = _median_rows(X)
+    X = X - med[:, None]
+    T = X.shape[0]
+    mad = np.zeros((T, T))
+    for i in range(T):
+        row = _median_rows(X[i][None, :] * X[i:])
+        mad[i, i:] = row
+        mad[i:, i] = row
+    return med, mad

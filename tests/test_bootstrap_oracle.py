@@ -154,3 +154,42 @@ def test_complete_form_drops_realisations_that_do_not_factorise():
     W[1] = 0              # no spectrum: every band empty, x = 0
     X, valid = ob.complete_realizations(W, col, fis, T)
     assert valid[1] and not X[1].any()
+
+
+def _brute_median(vals):
+    s = sorted(vals)
+    n = len(s)
+    return s[n // 2] if n % 2 else (s[n // 2 - 1] + s[n // 2]) / 2
+
+
+@pytest.mark.parametrize("nboots", [7, 8])
+def test_median_bootstrap_brute_force(nboots):
+    """Sorted Python lists, band by band and pair by pair, as
+    _calcuateMedian and _calcuateMadCovariance state it."""
+    rng = np.random.default_rng(nboots)
+    R = np.round(rng.standard_normal((nboots, 4)), 1)        # ties included
+    med, mad = ob.median_covariance(R)
+    T = R.shape[1]
+    bm = [_brute_median([R[b][i] for b in range(nboots)]) for i in range(T)]
+    dev = [[R[b][i] - bm[i] for b in range(nboots)] for i in range(T)]
+    assert list(med) == bm
+    for i in range(T):
+        for j in range(T):
+            lo, hi = min(i, j), max(i, j)
+            assert mad[i, j] == _brute_median([dev[lo][b] * dev[hi][b] for b in range(nboots)])
+
+
+@pytest.mark.parametrize("nboots", [1, 2, 999, 1000])
+def test_median_is_numpys_and_the_odd_diagonal_is_the_squared_mad(nboots):
+    from scipy.stats import median_abs_deviation
+    rng = np.random.default_rng(nboots + 5)
+    R = rng.standard_normal((nboots, 6)) * np.arange(1, 7) + 3.0
+    med, mad = ob.median_covariance(R)
+    assert np.array_equal(med, np.median(R, axis=0))
+    assert np.array_equal(mad, mad.T)
+    if nboots % 2:
+        np.testing.assert_allclose(np.diag(mad), median_abs_deviation(R, axis=0, scale=1.0) ** 2, rtol=1e-15, atol=0)
+    # the median covariance does not depend on the order of the realisations
+    perm = rng.permutation(nboots)
+    m2, c2 = ob.median_covariance(R[perm])
+    assert np.array_equal(m2, med) and np.array_equal(c2, mad)

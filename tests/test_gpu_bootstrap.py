@@ -158,3 +158,54 @@ def test_complete_bootstrap_matches_oracle(precision):
     for xb in X1:
         assert np.max(np.abs((xb - x) * d)) / np.max(np.abs(x * d)) < tol
     eng.close()
+
+
+def _median_engine():
+    w = synth.cpu_reference_config(nchunks=2)
+    return QMLEngine(w.binning, precision="fp64")
+
+
+@pytest.mark.parametrize("nboots,T,ties", [(1, 3, False), (2, 5, False), (2001, 37, False), (2000, 70, True),
+                                            (1000, 130, True)])
+def test_median_bootstrap_matches_oracle(nboots, T, ties):
+    """qml_bootstrap_median against the oracle bit for bit: the radix
+    select returns the sorted arrays' values, and the half-sums and products
+    are the same FP64 operations.  Rounded inputs give ties and zero
+    deviations (signed zeros among the products)."""
+    print(f"[case] median bootstrap nboots={nboots} T={T} ties={ties}", flush=True)
+    rng = np.random.default_rng(nboots + T)
+    R = rng.standard_normal((nboots, T)) * rng.uniform(0.1, 10.0, T) + rng.uniform(-1.0, 1.0, T)
+    if ties:
+        R = np.round(R, 1)
+    eng = _median_engine()
+    med, mad = eng.bootstrap_median(R)
+    eng.close()
+    omed, omad = ob.median_covariance(R)
+    assert np.array_equal(med, omed)
+    assert np.array_equal(mad, omad)
+
+
+def test_median_bootstrap_full_size():
+    """20000 realisations of 455 bands (the DESI binning): every median, and
+    3000 pairs of the median covariance against the oracle, bit for bit."""
+    import time
+    nboots, T = 20000, 455
+    print(f"[case] median bootstrap full size nboots={nboots} T={T}", flush=True)
+    rng = np.random.default_rng(455)
+    R = rng.standard_normal((nboots, T)) @ (np.eye(T) + 0.1 * rng.standard_normal((T, T)) / np.sqrt(T))
+    eng = _median_engine()
+    eng.bootstrap_median(R[:64, :16])                       # load the kernels
+    t0 = time.perf_counter()
+    med, mad = eng.bootstrap_median(R)
+    ms = (time.perf_counter() - t0) * 1e3
+    eng.close()
+    print(f"median bootstrap {nboots} x {T}: {ms:.1f} ms with transfers", flush=True)
+    assert np.array_equal(med, ob.median(R))
+    assert np.array_equal(mad, mad.T)
+    X = R.T - med[:, None]
+    ii = rng.integers(0, T, 3000)
+    jj = rng.integers(0, T, 3000)
+    for s in range(0, 3000, 500):
+        i, j = ii[s:s + 500], jj[s:s + 500]
+        want = ob._median_rows(X[i] * X[j])
+        assert np.array_equal(mad[i, j], want)

@@ -157,10 +157,14 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
         t0 = time.perf_counter()
         X, valid, _, _ = eng.bootstrap_complete(boots, qso, len(accepted), seed=0)
         t_full = time.perf_counter() - t0
+        R, _, _ = eng.bootstrap(boots, qso, len(accepted), seed=0, realizations=True)
+        t0 = time.perf_counter()
+        eng.bootstrap_median(R)
+        t_median = time.perf_counter() - t0
         boot = {"realisations": boots, "spectra": len(accepted), "fast_s": t_fast, "complete_s": t_full,
-                "complete_valid": int(valid.sum())}
+                "complete_valid": int(valid.sum()), "median_s": t_median}
         log(f"[{precision}] bootstrap of {boots}: fast {t_fast:.2f} s, complete {t_full:.2f} s "
-            f"({boot['complete_valid']} valid)")
+            f"({boot['complete_valid']} valid), median covariance of the fast realisations {t_median:.3f} s")
     eng.close()
     return dict(F=F, dbt=d, times=times, timings=tm, n=len(accepted), shapes=shapes, serial_ms=serial_ms,
                 schedule=schedule, chain=chain, boot=boot)

@@ -197,6 +197,10 @@ int Engine::bootstrapComplete(const std::vector<OneQSOEstimate *> &queue, int nb
     return nvalid;
 }
 
+void Engine::bootstrapMedian(int nboots, const double *realizations, double *median, double *mad_cov) {
+    check(qml_bootstrap_median(ctx_, nboots, bins_.totalKzBins(), realizations, median, mad_cov));
+}
+
 // ------------------------------------------------------------------- Chunk --
 Chunk::Chunk(Engine &eng, const QSOSpectrum &q, int i1, int i2, const ResolutionRows *rr) : eng_(&eng) {
     ResolutionRows own;

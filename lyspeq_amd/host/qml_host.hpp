@@ -131,6 +131,10 @@ public:
     // solutions fill the first rows of realizations (may be null).
     int bootstrapComplete(const std::vector<OneQSOEstimate *> &queue, int nboots, unsigned long long seed,
                           double *mean, double *cov, double *realizations = nullptr);
+    // PoissonBootstrapper::run() on saved realisations (_medianBootstrap,
+    // qml_bootstrap_median): realizations nboots x TOTAL_KZ_BINS, one per row;
+    // median TOTAL_KZ_BINS values, mad_cov TOTAL_KZ_BINS^2.  Needs no chunks.
+    void bootstrapMedian(int nboots, const double *realizations, double *median, double *mad_cov);
 
     int totalKzBins() const { return bins_.totalKzBins(); }
     const Binning &binning() const { return bins_; }

@@ -28,7 +28,9 @@
 //             no early stop; the output is then int32 T, int32 iterations run,
 //             f64 theta[iterations][T], f64 sigma[T] of the last iteration.
 //   boot      after the iterations, Engine::bootstrap over the spectra (500
-//             realisations, seed 7); out.bin.boot receives f64 mean[T], cov[T*T].
+//             realisations, seed 7) and Engine::bootstrapMedian of its
+//             realisations; out.bin.boot receives f64 mean[T], cov[T*T],
+//             median[T], mad_cov[T*T].
 #include "qml_driver.hpp"
 #include "qml_host.hpp"
 
@@ -165,12 +167,16 @@ int main(int argc, char **argv) {
         if (boot) {
             std::vector<OneQSOEstimate *> queue;
             for (auto &q : qsos) queue.push_back(q.get());
-            std::vector<double> mean(T), cov((size_t)T * T);
-            eng.bootstrap(queue, 500, 7, mean.data(), cov.data());
+            std::vector<double> mean(T), cov((size_t)T * T), reals((size_t)500 * T);
+            eng.bootstrap(queue, 500, 7, mean.data(), cov.data(), reals.data());
+            std::vector<double> median(T), mad((size_t)T * T);
+            eng.bootstrapMedian(500, reals.data(), median.data(), mad.data());
             FILE *bo = std::fopen((std::string(argv[2]) + ".boot").c_str(), "wb");
             if (!bo) throw std::runtime_error("cannot open bootstrap output");
             std::fwrite(mean.data(), 8, T, bo);
             std::fwrite(cov.data(), 8, cov.size(), bo);
+            std::fwrite(median.data(), 8, T, bo);
+            std::fwrite(mad.data(), 8, mad.size(), bo);
             std::fclose(bo);
         }
         FILE *o = std::fopen(argv[2], "wb");

@@ -318,7 +318,9 @@ def test_cpp_newton_raphson_loop_with_graph_replay_and_autotuned_schedule(tmp_pa
 def test_cpp_bootstrap_over_the_queue_matches_python_binding(tmp_path):
     """Engine::bootstrap numbers the spectra that hold chunks in queue order
     and maps each chunk to its spectrum; the same mapping through the Python
-    binding must give the same mean and covariance bit for bit."""
+    binding must give the same mean and covariance bit for bit, and
+    Engine::bootstrapMedian of those realisations the oracle's median form."""
+    from oracle import bootstrap as ob
     from lyspeq_amd.qml import QMLEngine, QMLError
     print("[case] C++ bootstrap over the queue", flush=True)
     w = synth.desi_config(nchunks=16)
@@ -330,7 +332,8 @@ def test_cpp_bootstrap_over_the_queue_matches_python_binding(tmp_path):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     buf = np.fromfile(str(out) + ".boot", "<f8")
-    mean_c, cov_c = buf[:T], buf[T:].reshape(T, T)
+    mean_c, cov_c = buf[:T], buf[T:T + T * T].reshape(T, T)
+    med_c, mad_c = buf[T + T * T:2 * T + T * T], buf[2 * T + T * T:].reshape(T, T)
     eng = QMLEngine(w.binning, precision="fp64")
     for i, (R, dv) in enumerate(w.pairs):
         eng.add_table(R, dv, w.deriv[i], w.signal[i], w.v_length)
@@ -348,7 +351,9 @@ def test_cpp_bootstrap_over_the_queue_matches_python_binding(tmp_path):
     eng.iterate(th, F, d)
     order = {s: k for k, s in enumerate(sorted(set(spec_of)))}
     qso = np.array([order[s] for s in spec_of], dtype=np.int32)
-    _, mean, cov = eng.bootstrap(500, qso, len(order), seed=7, realizations=False)
+    R, mean, cov = eng.bootstrap(500, qso, len(order), seed=7, realizations=True)
     eng.close()
     assert np.array_equal(mean, mean_c) and np.array_equal(cov, cov_c)
+    omed, omad = ob.median_covariance(R)
+    assert np.array_equal(med_c, omed) and np.array_equal(mad_c, omad)
     assert np.abs(cov).max() > 0

@@ -355,8 +355,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU sharding of chunks is described in "
-              "DESIGN.md but not built, so --gpus must be 1", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; sharding chunks over several GPUs (one engine per "
+              "rank, an all-reduce of the sums) is not built, so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     # the CPU baseline's worker processes start before anything touches the GPU
     pool = start_cpu_pool() if not args.no_cpu_baseline else None

@@ -295,7 +295,8 @@ void launch_bootstrap(hipStream_t st, const ChunkOut *outs, const double *dbt, c
 // 256-bin LDS histogram of the values still matching the selected prefix and
 // a block scan that picks the digit holding the rank; for even n one more
 // pass counts the values at or below the selected one and finds the next
-// larger value.  The selected values are the sorted array's, bit for bit;
+// larger value.  After the first two passes the keys of the selected bin
+// (up to 4096) are copied to LDS and the rest of the select reads only them.  The selected values are the sorted array's, bit for bit;
 // products are formed the same way in every pass (one FP64 multiply).
 // Realisations are transposed first so that a band's values are contiguous.
 
@@ -318,63 +319,56 @@ __device__ __forceinline__ unsigned long long median_term(const double *a, const
     return order_key(PROD ? a[i] * b[i] : a[i]);
 }
 
+constexpr int MED_CAP = 4096;   // keys of the median's 16-bit bin kept in LDS
+
 struct SelectLds {
     unsigned int hist[256];
     unsigned int wave_tot[4];
     unsigned long long wave_min[4];
-    unsigned int digit, rank, count_le;
+    unsigned int digit, rank, binsize, ncand;
+    unsigned long long cand[MED_CAP];
 };
 
-// The median of the n values a[i] (PROD false) or a[i] * b[i] (PROD true).
-// Called by all 256 threads of the block; every thread returns the median.
-template <bool PROD>
-__device__ double block_median(const double *a, const double *b, int n, SelectLds &s) {
+// One radix pass: the 8-bit digit at `shift` of the keys key(i), i < count,
+// that match prefix under mask; selects the digit whose bin holds rank r and
+// leaves r as the rank within that bin and s.binsize as the bin's count.
+template <class Key>
+__device__ void radix_pass(Key key, int count, int shift, unsigned long long &prefix, unsigned long long &mask,
+                           unsigned int &r, SelectLds &s) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const unsigned int r_lo = (n % 2 == 0) ? (unsigned)(n / 2 - 1) : (unsigned)(n / 2);
-    unsigned long long prefix = 0, mask = 0;
-    unsigned int r = r_lo;
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        s.hist[tid] = 0;
-        __syncthreads();
-        for (int i = tid; i < n; i += 256) {
-            const unsigned long long k = median_term<PROD>(a, b, i);
-            if ((k & mask) == prefix) atomicAdd(&s.hist[(k >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        // inclusive scan of the 256 bins: within the wave, then across waves
-        const unsigned int c = s.hist[tid];
-        unsigned int incl = c;
-        for (int d = 1; d < 64; d <<= 1) {
-            const unsigned int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
-        }
-        if (lane == 63) s.wave_tot[wave] = incl;
-        __syncthreads();
-        for (int w = 0; w < wave; ++w) incl += s.wave_tot[w];
-        const unsigned int excl = incl - c;
-        if (excl <= r && r < incl) {   // exactly one bin holds the rank
-            s.digit = (unsigned)tid;
-            s.rank = r - excl;
-        }
-        __syncthreads();
-        prefix |= (unsigned long long)s.digit << shift;
-        mask |= 0xffull << shift;
-        r = s.rank;
-        __syncthreads();
-    }
-    const double lo = key_value(prefix);
-    if (n % 2 != 0) return lo;
-    // even n: the value of rank n/2 is lo itself when more than n/2 values
-    // are at or below it, else the smallest value above it
-    if (tid == 0) s.count_le = 0;
+    s.hist[tid] = 0;
     __syncthreads();
-    unsigned int cnt = 0;
-    unsigned long long mn = ~0ull;
-    for (int i = tid; i < n; i += 256) {
-        const unsigned long long k = median_term<PROD>(a, b, i);
-        if (k <= prefix) ++cnt;
-        else mn = k < mn ? k : mn;
+    for (int i = tid; i < count; i += 256) {
+        const unsigned long long k = key(i);
+        if ((k & mask) == prefix) atomicAdd(&s.hist[(k >> shift) & 255u], 1u);
     }
+    __syncthreads();
+    // inclusive scan of the 256 bins: within the wave, then across waves
+    const unsigned int c = s.hist[tid];
+    unsigned int incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) s.wave_tot[wave] = incl;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) incl += s.wave_tot[w];
+    const unsigned int excl = incl - c;
+    if (excl <= r && r < incl) {   // exactly one bin holds the rank
+        s.digit = (unsigned)tid;
+        s.rank = r - excl;
+        s.binsize = c;
+    }
+    __syncthreads();
+    prefix |= (unsigned long long)s.digit << shift;
+    mask |= 0xffull << shift;
+    r = s.rank;
+    __syncthreads();
+}
+
+// Block-wide sum of cnt and minimum of mn; every thread gets both.
+__device__ void block_sum_min(unsigned int &cnt, unsigned long long &mn, SelectLds &s) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int d = 32; d >= 1; d >>= 1) {
         cnt += __shfl_xor(cnt, d, 64);
         const unsigned long long o = __shfl_xor(mn, d, 64);
@@ -385,17 +379,84 @@ __device__ double block_median(const double *a, const double *b, int n, SelectLd
         s.wave_min[wave] = mn;
     }
     __syncthreads();
-    unsigned int tot = 0;
-    unsigned long long gmin = ~0ull;
+    cnt = 0;
+    mn = ~0ull;
     for (int w = 0; w < 4; ++w) {
-        tot += s.wave_tot[w];
-        gmin = s.wave_min[w] < gmin ? s.wave_min[w] : gmin;
+        cnt += s.wave_tot[w];
+        mn = s.wave_min[w] < mn ? s.wave_min[w] : mn;
     }
     __syncthreads();
-    const double hi = (tot >= (unsigned)(n / 2 + 1)) ? lo : key_value(gmin);
-    return (lo + hi) / 2;
 }
 
+// Among key(i), i < count: how many are <= sel, and the smallest above it.
+template <class Key>
+__device__ void count_le_min_gt(Key key, int count, unsigned long long sel, unsigned int &tot,
+                                unsigned long long &gmin, SelectLds &s) {
+    unsigned int cnt = 0;
+    unsigned long long mn = ~0ull;
+    for (int i = threadIdx.x; i < count; i += 256) {
+        const unsigned long long k = key(i);
+        if (k <= sel) ++cnt;
+        else mn = k < mn ? k : mn;
+    }
+    block_sum_min(cnt, mn, s);
+    tot = cnt;
+    gmin = mn;
+}
+
+// The median of the n values a[i] (PROD false) or a[i] * b[i] (PROD true).
+// Called by all 256 threads of the block; every thread returns the median.
+// The first two passes read the values; when the selected 16-bit bin holds
+// at most MED_CAP keys they are copied to LDS once and the remaining six
+// passes (and the even-count step) read only those, else every pass reads
+// the values again.
+template <bool PROD>
+__device__ double block_median(const double *a, const double *b, int n, SelectLds &s) {
+    const unsigned int r_lo = (n % 2 == 0) ? (unsigned)(n / 2 - 1) : (unsigned)(n / 2);
+    auto gkey = [a, b](int i) { return median_term<PROD>(a, b, i); };
+    auto lkey = [&s](int i) { return s.cand[i]; };
+    unsigned long long prefix = 0, mask = 0;
+    unsigned int r = r_lo;
+    radix_pass(gkey, n, 56, prefix, mask, r, s);
+    radix_pass(gkey, n, 48, prefix, mask, r, s);
+    const unsigned int nbin = s.binsize, r_bin = r;
+    const bool in_lds = nbin <= (unsigned)MED_CAP;
+    unsigned long long above = ~0ull;   // smallest key above the bin
+    if (in_lds) {
+        if (threadIdx.x == 0) s.ncand = 0;
+        __syncthreads();
+        unsigned int unused = 0;
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const unsigned long long k = gkey(i);
+            if ((k & mask) == prefix) {
+                const unsigned int at = atomicAdd(&s.ncand, 1u);
+                if (at < (unsigned)MED_CAP) s.cand[at] = k;   // at < nbin always
+            } else if (k > prefix) {
+                above = k < above ? k : above;
+            }
+        }
+        block_sum_min(unused, above, s);
+        for (int shift = 40; shift >= 0; shift -= 8) radix_pass(lkey, (int)nbin, shift, prefix, mask, r, s);
+    } else {
+        for (int shift = 40; shift >= 0; shift -= 8) radix_pass(gkey, n, shift, prefix, mask, r, s);
+    }
+    const double lo = key_value(prefix);
+    if (n % 2 != 0) return lo;
+    // even n: the value of rank n/2 is lo itself when more than n/2 values
+    // are at or below it, else the smallest value above it
+    unsigned int tot;
+    unsigned long long gmin;
+    double hi;
+    if (in_lds) {
+        // (r_lo - r_bin) values lie below the bin
+        count_le_min_gt(lkey, (int)nbin, prefix, tot, gmin, s);
+        hi = (tot >= r_bin + 2) ? lo : key_value(gmin < above ? gmin : above);
+    } else {
+        count_le_min_gt(gkey, n, prefix, tot, gmin, s);
+        hi = (tot >= (unsigned)(n / 2 + 1)) ? lo : key_value(gmin);
+    }
+    return (lo + hi) / 2;
+}
 }  // namespace
 
 // R (nboots x T, one realisation per row) to X (T x nboots), 64 x 64 tiles.

@@ -396,10 +396,13 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     const int lr = tid / 4, lc = tid % 4;
     const auto arow = [&](int q) { const int r = I0 + lr + RP * q; return r < arows ? r : 0; };
     const auto brow = [&](int q) { const int r = J0 + lr + RP * q; return r < brows ? r : 0; };
-    // per-thread element offsets of the staged rows (below 2^31 for every
-    // operand); the operand bases are workgroup-uniform
-    const int oa0 = arow(0) * (int)lda + 8 * lc, oa1 = arow(PA - 1) * (int)lda + 8 * lc;
-    const int ob0 = brow(0) * (int)ldb + 8 * lc, ob1 = brow(PB - 1) * (int)ldb + 8 * lc;
+    // per-thread element offsets of the staged rows; the operand bases are
+    // workgroup-uniform.  The whitening products' offsets stay below Np^2
+    // (32 bits); the Gram's rows are whole members of 2 ntu TILE^2 elements,
+    // so m ntu >= 2^18 (e.g. N ~ 2900 with 250 stored Q) needs 64 bits.
+    using off_t = typename std::conditional<MODE == MODE_GRAM, long long, int>::type;
+    const off_t oa0 = (off_t)arow(0) * (off_t)lda + 8 * lc, oa1 = (off_t)arow(PA - 1) * (off_t)lda + 8 * lc;
+    const off_t ob0 = (off_t)brow(0) * (off_t)ldb + 8 * lc, ob1 = (off_t)brow(PB - 1) * (off_t)ldb + 8 * lc;
     const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
     const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
     // two staging register sets (P, Q): stage k+2 is in flight while k+1 waits

@@ -134,6 +134,7 @@ struct qml_ctx {
     int poison = 0;   // LYSPEQ_QML_POISON=1 at commit: NaN-filled scratch and LDS
     hipStream_t st = nullptr, st2 = nullptr;
     bool bins_set = false, committed = false;
+    bool plan_ok = false;   // a batch plan is in place (cleared by release_plan)
     std::vector<double> kedges, kcenters, zcenters;
     qml_bins bins{};
     BinsDev binsdev{};
@@ -586,6 +587,7 @@ void release_plan(qml_ctx *c) {
         if (b.evR) hipEventDestroy(b.evR);
     }
     c->batches.clear();
+    c->plan_ok = false;
     for (int k = 0; k < 2; ++k) {
         void *set[] = {c->d_mat[k], c->d_vec[k], c->d_pj[k], c->d_sj[k], c->d_lic[k]};
         for (void *p : set) if (p) hipFree(p);
@@ -775,6 +777,7 @@ int plan_batches(qml_ctx *c) {
         HIPCHK(hipEventCreate(&B.evR));
         c->batches.push_back(B);
     }
+    c->plan_ok = true;
     return QML_OK;
 }
 }  // namespace
@@ -1004,6 +1007,9 @@ extern "C" {
 
 int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt_sum) {
     if (!c || !c->committed) return fail(QML_ERR_ARG, "qml_iterate: commit first");
+    if (!c->plan_ok)
+        return fail(QML_ERR_MEMORY, "qml_iterate: no batch plan (a re-plan failed; call qml_autotune with a cap "
+                                    "that fits, or create a new context)");
     if (!theta || !fisher_sum || !dbt_sum) return fail(QML_ERR_ARG, "qml_iterate: null argument");
     HIPCHK(hipSetDevice(c->device));
     const int T = c->T;
@@ -1101,11 +1107,25 @@ int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, in
     std::vector<double> F((size_t)T * T), dbt((size_t)3 * T);
     int best = -1;
     double best_t = 0.0;
+    const int prev_cap = c->max_batch;
+    // a plan that cannot be made (e.g. a cap whose arenas do not fit) ends the
+    // search; the previous cap's plan is restored so the context stays usable
+    auto replan = [&](int cap) {
+        release_plan(c);
+        c->max_batch = cap;
+        const int rc = plan_batches(c);
+        if (rc != QML_OK) {
+            const std::string why = g_err;
+            release_plan(c);
+            c->max_batch = prev_cap;
+            if (plan_batches(c) != QML_OK) return fail(rc, why + " (and the previous plan could not be restored)");
+            return fail(rc, why);
+        }
+        return QML_OK;
+    };
     for (int i = 0; i < ncaps; ++i) {
         if (i == 0 || caps[i] != c->max_batch) {
-            release_plan(c);
-            c->max_batch = caps[i];
-            const int rc = plan_batches(c);
+            const int rc = replan(caps[i]);
             if (rc != QML_OK) return rc;
         }
         double t = 0.0;
@@ -1120,9 +1140,7 @@ int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, in
         }
     }
     if (c->max_batch != best) {
-        release_plan(c);
-        c->max_batch = best;
-        const int rc = plan_batches(c);
+        const int rc = replan(best);
         if (rc != QML_OK) return rc;
     }
     if (c->prec >= 2) {

@@ -148,6 +148,30 @@ class ResoMatrix:
             rows[i] = out * (1.0 / out.sum())
         return ResoMatrix("oversampled", rows, nel, osamp)
 
+    def sandwich(self, fine: np.ndarray, n: int) -> np.ndarray:
+        """R fine R^T for a symmetric fine matrix, upper triangle mirrored.
+        Oversampled storage follows OversampledMatrix::sandwich
+        (mathtools/matrix_helper.cpp:877-902) row by row: v = sum over the
+        row's nelem weights of the fine rows from i*os, columns from i*os
+        (dgemv, Trans), then B[i][i+j] = v[j*os : j*os+nelem] . R_{i+j}
+        (ddot) for j >= 0.  Diagonal storage uses the dense n x n matrix
+        (DiaMatrix::sandwich multiplies left, then right-transposed,
+        :815-821)."""
+        if self.kind != "oversampled":
+            r = self.dense(n)
+            out = r @ fine @ r.T
+        else:
+            vals, os_, nel = self.values, self.oversampling, self.nelem
+            out = np.zeros((n, n))
+            for i in range(n):
+                c0 = i * os_
+                v = vals[i] @ fine[c0:c0 + nel, c0:]
+                win = np.lib.stride_tricks.sliding_window_view(v, nel)[::os_][:n - i]
+                out[i, i:] = np.einsum("jt,jt->j", win, vals[i:n])
+        iu = np.triu_indices(n, 1)
+        out.T[iu] = out[iu]
+        return out
+
     def dense(self, n: int) -> np.ndarray:
         """R as an n x ncols array."""
         if self.kind == "dia":
@@ -303,24 +327,23 @@ class Chunk:
         return m
 
     def _fine_vz(self):
-        """v/z on the fine grid of the resolution matrix (upper, mirrored)."""
-        li = self.fine_wave[:, None]
-        lj = self.fine_wave[None, :]
-        n = self.fine_wave.size
-        iu = np.triu_indices(n)
-        v = np.zeros((n, n))
-        z = np.zeros((n, n))
-        v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
-        z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
-        return v, z, iu
+        """v/z on the fine grid of the resolution matrix (upper, mirrored),
+        computed once per chunk."""
+        if getattr(self, "_fine_cache", None) is None:
+            li = self.fine_wave[:, None]
+            lj = self.fine_wave[None, :]
+            n = self.fine_wave.size
+            iu = np.triu_indices(n)
+            v = np.zeros((n, n))
+            z = np.zeros((n, n))
+            v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
+            z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
+            self._fine_cache = (v, z, iu)
+        return self._fine_cache
 
     def _sandwich(self, fine):
-        """Resolution::sandwich, R fine R^T, upper triangle mirrored
-        (mathtools/matrix_helper.cpp:877-902)."""
-        out = self.R @ fine @ self.R.T
-        iu = np.triu_indices(self.N, 1)
-        out.T[iu] = out[iu]
-        return out
+        """Resolution::sandwich (mathtools/matrix_helper.cpp:1075-1084)."""
+        return self.rmat.sandwich(fine, self.N)
 
     def signal_matrix(self, v, z):
         """_setFiducialSignalMatrix, core/chunk_estimate.cpp:400-426."""

@@ -33,7 +33,7 @@ def inline_array(src, name):
 def main(ref):
     t = os.path.join(ref, "tests")
     for rel in ["input/desilite_seed332298_id0_0_z2.4_lognormal.dat",
-                "input/test.config", "input/slist.txt", "input/flist.txt",
+"input/slist.txt", "input/flist.txt",
                 "truth/signal_R70000_dv20.0.fits",
                 "truth/test_it1_fisher_matrix.dat",
                 "truth/test_it1_inversefisher_matrix.dat",

@@ -1,0 +1,68 @@
+"""Writes tests/golden/rmat_large.npz: the oracle's per-chunk F and d/b/t for
+three resolution-matrix chunks at N = 828, 981 and 1190 pixels (the upper
+half of the oversampled resolution-matrix workload's log-U[100, 1200] size
+range), in two forms:
+
+* "dia": the 11-diagonal matrix as read (mxhelp::DiaMatrix storage);
+* "os3": the same matrix deconvolved (m = 1) and oversampled x3 at read time
+  (core/one_qso_estimate.cpp:74-89), the oracle applying its own
+  ResoMatrix.deconvolve / .oversample.
+
+The oracle needs 10-30 s per chunk at these sizes, so its results are kept
+as data; the GPU parity test runs only the engine.  The chunk inputs are
+stored with them (wave in Angstrom, delta, ivar, R_fwhm, the diagonals), and
+theta.  Tables come from synth.rmat_config (table 0, deterministic).
+
+    python tests/golden/make_rmat_large.py
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+CHUNKS = (0, 2, 4)          # of synth.rmat_config(seed=3, n_range=(800, 1200), kind="dia")
+THETA_SEED = 53
+
+
+def workload(with_tables=True):
+    from lyspeq_amd import synth
+    w = synth.rmat_config(nchunks=12, n_range=(800, 1200), kind="dia", with_tables=with_tables)
+    w.chunks = [w.chunks[i] for i in CHUNKS]
+    return w
+
+
+def main():
+    import parity as P
+    w = workload()
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, THETA_SEED)
+    out = dict(theta=th, nchunks=len(w.chunks))
+    for i, c in enumerate(w.chunks):
+        out[f"wave{i}"] = c["wave"]
+        out[f"delta{i}"] = c["delta"]
+        out[f"ivar{i}"] = c["ivar"]
+        out[f"R{i}"] = c["R"]
+        out[f"dia{i}"] = np.asarray(c["rmat"]["values"])
+    for form in ("dia", "os3"):
+        if form == "os3":
+            for c in w.chunks:
+                ro = P.oracle_rmat(c).deconvolve(1.0).oversample(3)
+                c["rmat"] = dict(kind="oversampled", nelem=31, oversampling=3, values=ro.values)
+        for i in range(len(w.chunks)):
+            t0 = time.time()
+            res, _, _ = P.run_oracle(w, th, [i])
+            F, dbt, ok, ch = res[i]
+            assert ok
+            out[f"{form}_F{i}"] = F
+            out[f"{form}_dbt{i}"] = dbt
+            print(f"{form} chunk {i}: N={ch.N} n={len(ch.stored)} {time.time() - t0:.1f} s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "rmat_large.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

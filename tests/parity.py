@@ -84,3 +84,112 @@ def fisher_norm_err(Fa, Fb):
     iu = np.triu_indices(Fb.shape[0])
     m = den[iu] > 0
     return float(np.max(np.abs(Fa - Fb)[iu][m] / den[iu][m])) if m.any() else 0.0
+
+
+def vec_norm_err(va, vb, Fb):
+    """max |dv_k| / sqrt(F_kk) over the bands with F_kk > 0: a d/b/t error in
+    units of the band's own 1-sigma (theta = F^-1 (d - b - t))."""
+    d = np.sqrt(np.abs(np.diag(Fb)))
+    m = d > 0
+    return float(np.max(np.abs(np.asarray(va) - np.asarray(vb))[m] / d[m])) if m.any() else 0.0
+
+
+def elem_err(a, b, floor):
+    """max |a - b| / |b| over the entries with |b| >= floor * max |b| (upper
+    triangle for a square matrix)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if b.ndim == 2:
+        iu = np.triu_indices(b.shape[0])
+        a, b = a[iu], b[iu]
+    big = np.abs(b) >= floor * np.max(np.abs(b)) if b.size else np.zeros(0, bool)
+    big &= b != 0
+    return float(np.max(np.abs(a[big] - b[big]) / np.abs(b[big]))) if big.any() else 0.0
+
+
+# FP64 bars.  Normalised: F by sqrt(F_kk F_k'k'), d/b/t by sqrt(F_kk).
+# Elementwise: relative error of every entry at least ELEM_FLOOR times the
+# largest of its quantity.  The rounding floor of the two inversions (the
+# oracle's LU against its own Cholesky form, DESI chunks) is about 1e-14
+# normalised and 1e-12 elementwise at this floor.
+FP64_TOL = 1e-10
+ELEM_FLOOR = 1e-6
+ELEM_TOL = 1e-9
+
+
+def errors(Fa, dbta, Fb, dbtb, floor=ELEM_FLOOR):
+    """Normalised and elementwise errors of (Fa, dbta) against (Fb, dbtb)."""
+    e = dict(F_norm=fisher_norm_err(Fa, Fb), F_elem=elem_err(Fa, Fb, floor), F_max=rel_err(Fa, Fb))
+    for k, nm in enumerate("dbt"):
+        e[f"{nm}_norm"] = vec_norm_err(dbta[k], dbtb[k], Fb)
+        e[f"{nm}_elem"] = elem_err(dbta[k], dbtb[k], floor)
+    return e
+
+
+def fmt(e):
+    return " ".join(f"{k} {v:.1e}" for k, v in e.items())
+
+
+def assert_fp64(Fa, dbta, Fb, dbtb, what="", tol=FP64_TOL, elem_tol=ELEM_TOL):
+    """The FP64 parity bar: every normalised error below tol and every
+    elementwise error (entries above the floor) below elem_tol."""
+    e = errors(Fa, dbta, Fb, dbtb)
+    print(f"[fp64 parity] {what}: {fmt(e)}", flush=True)
+    bad = {k: v for k, v in e.items() if (k.endswith("_norm") and not v < tol)
+           or (k.endswith("_elem") and not v < elem_tol) or not np.isfinite(v)}
+    assert not bad, f"{what}: FP64 parity bar missed: {bad}"
+    return e
+
+
+def assert_chunks_fp64(out, oout, what="", **kw):
+    """assert_fp64 on every chunk both sides accepted; returns the worst of
+    each error."""
+    assert set(out) == set(oout), f"{what}: accepted chunks differ"
+    worst = {}
+    for i in sorted(out):
+        e = errors(out[i][0], out[i][1], oout[i][0], oout[i][1])
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    print(f"[fp64 parity] {what} worst per chunk over {len(out)}: {fmt(worst)}", flush=True)
+    tol = kw.get("tol", FP64_TOL)
+    elem_tol = kw.get("elem_tol", ELEM_TOL)
+    bad = {k: v for k, v in worst.items() if (k.endswith("_norm") and not v < tol)
+           or (k.endswith("_elem") and not v < elem_tol) or not np.isfinite(v)}
+    assert not bad, f"{what}: FP64 parity bar missed per chunk: {bad}"
+    return worst
+
+
+def p1d_deviation(Fa, dbta, Fb, dbtb):
+    """max |theta_a - theta_b| / sigma_b over the bands with sigma_b > 0, both
+    power estimates from the driver's arithmetic (0.5 scaling, the stable
+    inverse, theta = 0.5 F^-1 (d - b - t))."""
+    from oracle.driver import power_estimates
+    ra, rb = power_estimates(Fa.copy(), dbta), power_estimates(Fb.copy(), dbtb)
+    s = rb["sigma"]
+    m = np.isfinite(s) & (s > 0)
+    return float(np.max(np.abs(ra["theta"] - rb["theta"])[m] / s[m]))
+
+
+# Mixed-precision bars against the FP64 path: normalised F, d/b/t (d is
+# FP64 in every mode and must agree to rounding).
+MIXED_TOL = {
+    "fp32": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
+    "bf16x3": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
+    "bf16x4": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
+}
+# P1D bar: max |dP| / sigma
+P1D_TOL = {("desi", "bf16x3"): 1e-3, ("hires", "bf16x4"): 1e-3}
+
+
+def assert_mixed(Fm, dm, F64, d64, precision, what=""):
+    """Errors of a mixed-precision run against the FP64 run on the same
+    chunks: normalised F, d, b, t, and b/t relative to their largest entry."""
+    e = dict(F_norm=fisher_norm_err(Fm, F64),
+             d_norm=vec_norm_err(dm[0], d64[0], F64), d_max=rel_err(dm[0], d64[0]),
+             b_norm=vec_norm_err(dm[1], d64[1], F64), b_max=rel_err(dm[1], d64[1]),
+             t_norm=vec_norm_err(dm[2], d64[2], F64), t_max=rel_err(dm[2], d64[2]))
+    print(f"[mixed] {what}: {fmt(e)}", flush=True)
+    assert np.isfinite(Fm).all() and np.isfinite(dm).all(), f"{what}: non-finite results"
+    assert e["d_max"] < 1e-12, f"{what}: d_k is FP64 in every precision"
+    bad = {k: e[k] for k, tol in MIXED_TOL[precision].items() if not e[k] < tol}
+    assert not bad, f"{what}: mixed-precision bar missed: {bad}"
+    return e

@@ -11,7 +11,6 @@ from oracle.chunk import Chunk as OracleChunk, ChunkError
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-10
 
 
 def _print_case(name):
@@ -65,9 +64,7 @@ def test_constructor_rejections_match_the_oracle():
     oF, od = np.zeros((T, T)), np.zeros((3, T))
     assert och.one_iteration(th, oF, od)
     assert st == 0
-    assert P.rel_err(F, oF) < FP64_TOL
-    for k in range(3):
-        assert P.rel_err(d[k], od[k]) < FP64_TOL
+    P.assert_fp64(F, d, oF, od, "masked boundary cut")
 
 
 def test_call_order_and_argument_errors():

@@ -22,7 +22,6 @@ from oracle.chunk import Chunk as OracleChunk, ChunkError, decide_indices
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-10
 
 
 def _write_workload(path, w, theta, precision, iterations=1, prep=None):
@@ -168,9 +167,7 @@ def test_cpp_host_layer_matches_python_binding_and_oracle(tmp_path, precision):
     if precision == "fp64":
         Fo, do, n = _oracle(w, th)
         assert n == len(chunks)
-        assert P.rel_err(F, Fo) < FP64_TOL
-        for k in range(3):
-            assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+        P.assert_fp64(F, dbt, Fo, do, "C++ host layer")
 
 
 def test_cpp_one_qso_estimate_splits_spectra_like_decide_indices(tmp_path):
@@ -189,9 +186,7 @@ def test_cpp_one_qso_estimate_splits_spectra_like_decide_indices(tmp_path):
     assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
     Fo, do, n = _oracle(w, th)
     assert n == len(chunks)
-    assert P.rel_err(F, Fo) < FP64_TOL
-    for k in range(3):
-        assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+    P.assert_fp64(F, dbt, Fo, do, "C++ OneQSOEstimate split spectra")
 
 
 def test_cpp_resolution_matrix_spectra_preprocessed_like_the_reader(tmp_path):
@@ -246,10 +241,7 @@ def test_cpp_resolution_matrix_spectra_preprocessed_like_the_reader(tmp_path):
     assert np.array_equal(F, Fp) and np.array_equal(dbt, dp)
     for (s, fis, nq, st, fc, d), (fr, dr, sr) in zip(chunks, res):
         assert st == sr == 0 and np.array_equal(fc, fr) and np.array_equal(d, dr)
-    print(f"preprocessed matrices, C++ layer vs oracle: F {P.rel_err(F, Fo):.2e}", flush=True)
-    assert P.rel_err(F, Fo) < FP64_TOL
-    for k in range(3):
-        assert P.rel_err(dbt[k], do[k]) < FP64_TOL
+    P.assert_fp64(F, dbt, Fo, do, "preprocessed matrices, C++ layer")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])

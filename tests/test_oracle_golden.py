@@ -294,6 +294,20 @@ def test_rmat_storage_against_reference_matrix_products(golden_dir):
     np.testing.assert_allclose(Ro @ A, d["truth_oversample_left_multiplication"].reshape(7, 18), rtol=1e-5)
     np.testing.assert_allclose(Ro @ A @ Ro.T, d["truth_oversample_right_multiplication"].reshape(7, 7),
                                rtol=1e-5)
+    # the banded row-by-row sandwich the oracle's chunks use (tests/cblas_tests.cpp:417-418
+    # checks OversampledMatrix::sandwich against the same truth)
+    Rs = ResoMatrix("oversampled", d["oversamp_values"].reshape(7, 5), 5, 2)
+    np.testing.assert_allclose(Rs.sandwich(A, 7), d["truth_oversample_right_multiplication"].reshape(7, 7),
+                               rtol=1e-5)
+    # and the dense product on random symmetric fine matrices, oversampling 1-3
+    rng = np.random.default_rng(17)
+    for os_, nel, n in ((1, 5, 9), (2, 5, 7), (3, 31, 23)):
+        ncols = n * os_ + nel - 1
+        a = rng.normal(size=(ncols, ncols))
+        a = a + a.T
+        Rr = ResoMatrix("oversampled", rng.uniform(0, 1, (n, nel)), nel, os_)
+        want = Rr.dense(n) @ a @ Rr.dense(n).T
+        np.testing.assert_allclose(Rr.sandwich(a, n), want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
 def test_rmat_rows_with_mirrored_edges_against_reference_rows(golden_dir):

@@ -59,6 +59,7 @@ def build(verbose: bool = False) -> str:
 HOST_LIB = os.path.join(HERE, "libqmlhost.so")
 HOST_DRIVER = os.path.join(ROOT, "tests", "cpp", "host_driver")
 DRIVER_CHECK = os.path.join(ROOT, "tests", "cpp", "driver_check")
+DROPIN_CHECK = os.path.join(ROOT, "tests", "cpp", "dropin_check")
 
 
 def _run(cmd, what, verbose):
@@ -73,16 +74,21 @@ def build_host(verbose: bool = False) -> str:
     """The C++ host layer (Chunk / OneQSOEstimate / Engine over the C ABI)
     as libqmlhost.so, and the C++ test driver that exercises it."""
     rocm_lib = "/opt/rocm/lib"
-    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+           "-I", os.path.join(HERE, "host", "dropin")]
     link = ["-L", HERE, "-lqmlhip", "-L", rocm_lib, f"-Wl,-rpath-link,{rocm_lib}"]
     tmp = HOST_LIB + ".tmp"
+    # the drop-in Chunk / OneQSOEstimate, with this build's mirrors of the
+    # reference globals and reader (replaced by the reference's own inside its tree)
+    dropin = [os.path.join(HERE, "host", "dropin", f) for f in ("dropin.cpp", "mirror.cpp")]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
-          os.path.join(HERE, "host", "qml_host.cpp"), os.path.join(HERE, "host", "qml_driver.cpp"), *link,
+          os.path.join(HERE, "host", "qml_host.cpp"), os.path.join(HERE, "host", "qml_driver.cpp"), *dropin, *link,
           "-Wl,-rpath,$ORIGIN", "-o", tmp],
          "host library build", verbose)
     os.replace(tmp, HOST_LIB)
     for src, exe in ((os.path.join(ROOT, "tests", "cpp", "host_driver.cpp"), HOST_DRIVER),
-                     (os.path.join(ROOT, "tests", "cpp", "driver_check.cpp"), DRIVER_CHECK)):
+                     (os.path.join(ROOT, "tests", "cpp", "driver_check.cpp"), DRIVER_CHECK),
+                     (os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp"), DROPIN_CHECK)):
         if os.path.exists(src):
             _run(["g++", "-O2", "-std=c++17", "-Wall", *inc, src, "-L", HERE, "-lqmlhost", *link,
                   "-Wl,-rpath,$ORIGIN/../../lyspeq_amd", "-o", exe], f"{os.path.basename(exe)} build", verbose)

@@ -1,0 +1,356 @@
+// The reference globals and reader classes that the drop-in Chunk /
+// OneQSOEstimate compile against (core/global_numbers.hpp,
+// io/qso_file.hpp), for builds outside the reference tree.  Inside it the
+// reference's own core/global_numbers.cpp and io/qso_file.cpp replace this
+// file.  Each function follows the reference function named in its comment.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+
+#include "core/global_numbers.hpp"
+#include "io/qso_file.hpp"
+#include "lyspeq_dropin.hpp"
+
+namespace process {
+double MEMORY_ALLOC = 1500.0;
+void updateMemory(double deltamem) { MEMORY_ALLOC += deltamem; }
+}  // namespace process
+
+namespace specifics {
+bool TURN_OFF_SFID = false, USE_RESOLUTION_MATRIX = false, REDSHIFT_GROWTH_ON = false,
+     USE_PRECOMPUTED_FISHER = false, FAST_BOOTSTRAP = true;
+int CONT_LOGLAM_MARG_ORDER = 1, CONT_LAM_MARG_ORDER = -1, CONT_NVECS = 2, NUMBER_OF_CHUNKS = 1,
+    OVERSAMPLING_FACTOR = -1;
+double RESOMAT_DECONVOLUTION_M = -1, MIN_SNR_CUT = 0;
+qio::ifileformat INPUT_QSO_FILE = qio::Binary;
+double MAX_FOREST_LENGTH_V = 3.5e4, MIN_FOREST_LENGTH_V = 0.45 * 3.5e4, MAX_PIXEL_LENGTH_V = 120.0;
+}  // namespace specifics
+
+namespace bins {
+int NUMBER_OF_K_BANDS = 0, NUMBER_OF_Z_BINS = 0, FISHER_SIZE = 0, TOTAL_KZ_BINS = 0;
+std::vector<double> KBAND_EDGES, KBAND_CENTERS, ZBIN_CENTERS;
+double Z_BIN_WIDTH = 0, Z_LOWER_EDGE = 0, Z_UPPER_EDGE = 0;
+BinningMethod Z_BINNING_METHOD = TriangleBinningMethod;
+
+// bins::findRedshiftBin (core/global_numbers.cpp:344-355)
+int findRedshiftBin(double z) {
+    if (z < Z_LOWER_EDGE) return -1;
+    int r = (int)((z - Z_LOWER_EDGE) / Z_BIN_WIDTH);
+    if (r >= NUMBER_OF_Z_BINS) r = NUMBER_OF_Z_BINS;
+    return r;
+}
+}  // namespace bins
+
+namespace conv {
+static void noConversion(const double *, double *, double *, int) {}
+void (*convertFluxToDeltaF)(const double *, double *, double *, int) = &noConversion;
+}  // namespace conv
+
+namespace lyspeq_amd {
+namespace dropin {
+// readBins / readSpecifics / calcNvecs (core/global_numbers.cpp:78-130,209-300)
+// from a Binning.
+void setGlobals(const Binning &b, bool use_resolution_matrix, bool fast_bootstrap, double min_snr_cut,
+                double memory_alloc_mb) {
+    bins::NUMBER_OF_K_BANDS = b.nk();
+    bins::NUMBER_OF_Z_BINS = b.nz();
+    bins::TOTAL_KZ_BINS = b.totalKzBins();
+    bins::FISHER_SIZE = bins::TOTAL_KZ_BINS * bins::TOTAL_KZ_BINS;
+    bins::KBAND_EDGES = b.kband_edges;
+    bins::KBAND_CENTERS.resize(b.nk());
+    for (int k = 0; k < b.nk(); ++k) bins::KBAND_CENTERS[k] = (b.kband_edges[k] + b.kband_edges[k + 1]) / 2.;
+    bins::ZBIN_CENTERS = b.zbin_centers;
+    bins::Z_BIN_WIDTH = b.z_bin_width;
+    bins::Z_LOWER_EDGE = b.zbin_centers.front() - b.z_bin_width / 2.;
+    bins::Z_UPPER_EDGE = b.zbin_centers.back() + b.z_bin_width / 2.;
+    bins::Z_BINNING_METHOD = b.z_binning_method == 1 ? bins::TriangleBinningMethod : bins::TophatBinningMethod;
+    specifics::TURN_OFF_SFID = b.turn_off_sfid;
+    specifics::USE_RESOLUTION_MATRIX = use_resolution_matrix;
+    specifics::REDSHIFT_GROWTH_ON = b.redshift_growth_on;
+    specifics::FAST_BOOTSTRAP = fast_bootstrap;
+    specifics::CONT_LOGLAM_MARG_ORDER = b.cont_logl_order;
+    specifics::CONT_LAM_MARG_ORDER = b.cont_lam_order;
+    if (b.cont_logl_order >= 0 || b.cont_lam_order >= 0)
+        specifics::CONT_NVECS = 1 + std::max(0, b.cont_logl_order) + std::max(0, b.cont_lam_order);
+    else
+        specifics::CONT_NVECS = 0;
+    specifics::NUMBER_OF_CHUNKS = std::max(1, b.number_of_chunks);
+    specifics::OVERSAMPLING_FACTOR = b.oversampling_factor;
+    specifics::RESOMAT_DECONVOLUTION_M = b.resomat_deconvolution_m;
+    specifics::MIN_SNR_CUT = min_snr_cut;
+    specifics::MAX_FOREST_LENGTH_V = b.max_forest_length_v / specifics::NUMBER_OF_CHUNKS;
+    specifics::MIN_FOREST_LENGTH_V = 0.90 / 2 * specifics::MAX_FOREST_LENGTH_V;
+    specifics::MAX_PIXEL_LENGTH_V = b.max_pixel_length_v;
+    process::MEMORY_ALLOC = memory_alloc_mb;
+}
+}  // namespace dropin
+}  // namespace lyspeq_amd
+
+namespace {
+double median_of(std::vector<double> v) {   // stats::medianOfUnsortedVector
+    std::sort(v.begin(), v.end());
+    const size_t n = v.size();
+    if (n == 0) return 0;
+    return (n % 2 == 0) ? (v[n / 2 - 1] + v[n / 2]) / 2 : v[n / 2];
+}
+double median_sorted(const double *a, int n) {   // stats::medianOfSortedArray
+    return (n % 2 == 0) ? (a[n / 2 - 1] + a[n / 2]) / 2 : a[n / 2];
+}
+qio::QSOFile::Reader &reader_slot() {
+    static qio::QSOFile::Reader r;
+    return r;
+}
+}  // namespace
+
+namespace mxhelp {
+Resolution::Resolution(int nm, int ndia)
+    : is_dia_matrix(true), nrows(nm), ncols(nm), nelem(ndia), osamp(1), values((size_t)nm * ndia, 0.0) {}
+
+// Resolution(rmaster, i1, i2) (mathtools/matrix_helper.cpp:936-970): columns
+// [i1, i2) of each diagonal, or rows [i1, i2) of the oversampled rows
+Resolution::Resolution(const Resolution *r, int i1, int i2)
+    : is_dia_matrix(r->is_dia_matrix), nrows(i2 - i1), nelem(r->nelem), osamp(r->osamp) {
+    if (is_dia_matrix) {
+        ncols = nrows;
+        values.resize((size_t)nelem * nrows);
+        for (int d = 0; d < nelem; ++d)
+            std::copy_n(r->values.data() + (size_t)d * r->nrows + i1, nrows, values.data() + (size_t)d * nrows);
+    } else {
+        ncols = nrows * osamp + nelem - 1;
+        values.assign(r->values.begin() + (size_t)i1 * nelem, r->values.begin() + (size_t)i2 * nelem);
+    }
+}
+
+void Resolution::cutBoundary(int i1, int i2) {
+    Resolution sub(this, i1, i2);
+    *this = std::move(sub);
+}
+
+// DiaMatrix::deconvolve (mathtools/matrix_helper.cpp:653-682), by the engine's
+// read-time preprocessing
+void Resolution::deconvolve(double m) {
+    if (!is_dia_matrix) return;
+    if (qml_rmat_deconvolve(nrows, nelem, m, values.data()) != QML_OK) throw std::runtime_error(qml_last_error());
+}
+
+// Resolution::oversample (mathtools/matrix_helper.cpp:1029-1073); the row
+// spacing drops out of the spline and the normalisation
+void Resolution::oversample(int os, double) {
+    if (!is_dia_matrix || os < 1) return;
+    const int nel = 2 * (nelem / 2) * os + 1;
+    std::vector<double> rows((size_t)nrows * nel);
+    if (qml_rmat_oversample(nrows, nelem, values.data(), os, rows.data()) != QML_OK)
+        throw std::runtime_error(qml_last_error());
+    values.swap(rows);
+    is_dia_matrix = false;
+    nelem = nel;
+    osamp = os;
+    ncols = nrows * os + nel - 1;
+}
+}  // namespace mxhelp
+
+namespace qio {
+
+void QSOFile::setReader(Reader r) { reader_slot() = std::move(r); }
+
+QSOFile::QSOFile(const std::string &fname_qso, ifileformat p_or_b) : PB(p_or_b), fname(fname_qso) {}
+
+// QSOFile(qmaster, i1, i2) (io/qso_file.cpp:61-89): copy, cut the masked
+// boundary, the matrix's rows, and dv / dlambda again
+QSOFile::QSOFile(const QSOFile &q, int i1, int i2)
+    : PB(q.PB), fname(q.fname), z_qso(q.z_qso), snr(q.snr), R_kms(q.R_kms), ra(q.ra), dec(q.dec), id(q.id),
+      R_fwhm(q.R_fwhm), expid(q.expid), night(q.night), fiber(q.fiber), petal(q.petal) {
+    if (i1 < 0 || i2 > q.size() || i2 <= i1) throw std::out_of_range("QSOFile: pixel range outside the spectrum");
+    arr_size = _fullsize = i2 - i1;
+    wave_.assign(q.wave() + i1, q.wave() + i2);
+    delta_.assign(q.delta() + i1, q.delta() + i2);
+    ivar_.assign(q.ivar() + i1, q.ivar() + i2);
+    _cutMaskedBoundary();
+    if (q.Rmat) Rmat = std::make_unique<mxhelp::Resolution>(q.Rmat.get(), i1 + shift, i1 + shift + arr_size);
+    if (arr_size > 1) recalcDvDLam();
+}
+
+double QSOFile::getLengthV() const {
+    if (arr_size == 0) return 0;
+    return SPEED_OF_LIGHT * std::log(wave()[arr_size - 1] / wave()[0]);
+}
+
+double QSOFile::getMinMemUsage() const {
+    double mem = wave_.empty() ? 0.0 : process::getMemoryMB(_fullsize * 3);
+    if (Rmat) mem += Rmat->getMinMemUsage();
+    return mem;
+}
+
+void QSOFile::setData(std::vector<double> wave, std::vector<double> delta, std::vector<double> ivar) {
+    if (wave.size() != delta.size() || wave.size() != ivar.size())
+        throw std::invalid_argument("QSOFile::setData: arrays of different sizes");
+    wave_ = std::move(wave);
+    delta_ = std::move(delta);
+    ivar_ = std::move(ivar);
+    arr_size = _fullsize = (int)wave_.size();
+    shift = 0;
+    _countMaskedPixels();
+}
+
+// BQFile::readParameters / readData (io/qso_file.cpp:332-369): an 88-byte
+// header with natural alignment, then lambda, flux and sigma; sigma becomes
+// ivar = 1/sigma^2, values below 1e-15 set to 0
+namespace {
+struct BinaryHeader {
+    int data_size;
+    double redshift, declination, right_ascension;
+    int spectrograph_resolution_fwhm;
+    double signal_to_noise, pixel_width, lower_observed_lambda, upper_observed_lambda, lower_rest_lambda,
+        upper_rest_lambda;
+};
+static_assert(sizeof(BinaryHeader) == 88, "binary spectrum header is 88 bytes");
+}  // namespace
+
+void QSOFile::readParameters() {
+    if (PB == Picca || reader_slot()) {
+        if (!reader_slot()) throw std::runtime_error("QSOFile: no reader for this file format");
+        reader_slot()(fname, *this);
+        _fullsize = arr_size = (int)wave_.size();
+        if (R_kms == 0 && R_fwhm > 0) R_kms = SPEED_OF_LIGHT / R_fwhm / ONE_SIGMA_2_FWHM;
+        return;
+    }
+    FILE *f = std::fopen(fname.c_str(), "rb");
+    if (!f) throw std::runtime_error("Cannot open file: " + fname);
+    BinaryHeader h{};
+    const size_t got = std::fread(&h, sizeof h, 1, f);
+    std::fclose(f);
+    if (got != 1) throw std::runtime_error("fread error in header BQFile!");
+    arr_size = _fullsize = h.data_size;
+    z_qso = h.redshift;
+    dec = h.declination;
+    ra = h.right_ascension;
+    R_fwhm = h.spectrograph_resolution_fwhm;
+    snr = h.signal_to_noise;
+    dv_kms = h.pixel_width;
+    R_kms = SPEED_OF_LIGHT / R_fwhm / ONE_SIGMA_2_FWHM;
+}
+
+// QSOFile::readData (io/qso_file.cpp:107-127)
+void QSOFile::readData() {
+    if (arr_size <= 0) throw std::runtime_error("File is non-positive!");
+    if (!(PB == Picca || reader_slot())) {
+        FILE *f = std::fopen(fname.c_str(), "rb");
+        if (!f) throw std::runtime_error("Cannot open file: " + fname);
+        std::fseek(f, sizeof(BinaryHeader), SEEK_SET);
+        wave_.resize(arr_size);
+        delta_.resize(arr_size);
+        ivar_.resize(arr_size);
+        const size_t rl = std::fread(wave_.data(), sizeof(double), arr_size, f);
+        const size_t rf = std::fread(delta_.data(), sizeof(double), arr_size, f);
+        const size_t rn = std::fread(ivar_.data(), sizeof(double), arr_size, f);
+        std::fclose(f);
+        if ((int)rl != arr_size || (int)rf != arr_size || (int)rn != arr_size)
+            throw std::runtime_error("fread error in data BQFile!");
+        for (double &iv : ivar_) {
+            iv = 1.0 / (iv * iv);
+            if (iv < DOUBLE_EPSILON) iv = 0;
+        }
+    }
+    shift = 0;
+    if (dlambda < 0 || dv_kms < 0) {
+        std::vector<double> dl(arr_size - 1), dv(arr_size - 1);
+        for (int i = 0; i + 1 < arr_size; ++i) {
+            dl[i] = wave_[i + 1] - wave_[i];
+            dv[i] = SPEED_OF_LIGHT * std::log(wave_[i + 1] / wave_[i]);
+        }
+        if (dlambda < 0) dlambda = median_of(dl);
+        if (dv_kms < 0) dv_kms = median_of(dv);
+    }
+    _countMaskedPixels();
+}
+
+void QSOFile::_countMaskedPixels() {
+    num_masked_pixels = 0;
+    for (int i = 0; i < size(); ++i)
+        if (ivar()[i] == 0) ++num_masked_pixels;
+}
+
+// QSOFile::_cutMaskedBoundary (io/qso_file.cpp:139-157)
+void QSOFile::_cutMaskedBoundary() {
+    int ni1 = 0, ni2 = size();
+    while (ni1 < size() && ivar()[ni1] == 0) ++ni1;
+    while (ni2 > 0 && ivar()[ni2 - 1] == 0) --ni2;
+    shift += ni1;
+    arr_size = ni2 - ni1;
+    if (arr_size <= 0) throw std::runtime_error("Empty spectrum when masked pixels at boundary are removed!");
+    _countMaskedPixels();
+}
+
+// QSOFile::cutBoundary (io/qso_file.cpp:159-187)
+void QSOFile::cutBoundary(double z_lower_edge, double z_upper_edge) {
+    const double l1 = LYA_REST * (1 + z_lower_edge), l2 = LYA_REST * (1 + z_upper_edge);
+    const int org_size = arr_size, org_shift = shift;
+    const int wi1 = (int)(std::lower_bound(wave(), wave() + arr_size, l1) - wave());
+    const int wi2 = (int)(std::upper_bound(wave(), wave() + arr_size, l2) - wave());
+    const bool isempty = (wi1 == arr_size) || (wi2 == 0);
+    arr_size = wi2 - wi1;
+    if (isempty) throw std::runtime_error("Empty spectrum when redshift boundary is applied!");
+    shift += wi1;
+    _cutMaskedBoundary();
+    const bool nochange = (shift == org_shift) && (arr_size == org_size);
+    if (!nochange && Rmat) Rmat->cutBoundary(shift, shift + arr_size);
+}
+
+// QSOFile::maskOutliers (io/qso_file.cpp:189-230)
+int QSOFile::maskOutliers(double factor) {
+    auto meanFluxRecip = [](double l) { return std::exp(1.663e-03 * std::pow(l / LYA_REST, 4.107)); };
+    std::vector<double> tmp;
+    tmp.reserve(realSize());
+    double *iv = ivar(), *f = delta();
+    for (int i = 0; i < size(); ++i)
+        if (iv[i] != 0) tmp.push_back(f[i]);
+    const double median = median_of(tmp);
+    for (double &x : tmp) x = std::fabs(x - median);
+    double mad = 1.4826 * (1. + 1. / std::sqrt(realSize())) * median_of(tmp);
+    mad = std::max(1.0, mad);
+    int j = 0;
+    for (int i = 0; i < size(); ++i) {
+        if (iv[i] == 0) continue;
+        const double s = factor * std::max(1.0 / std::sqrt(iv[i]), mad);
+        const double fup = meanFluxRecip(wave()[i]);
+        if (f[i] < (-1.0 - s) || f[i] > (fup + s)) {
+            f[i] = 0;
+            iv[i] = 0;
+            ++j;
+        }
+    }
+    _countMaskedPixels();
+    return j;
+}
+
+// QSOFile::readMinMaxMedRedshift (io/qso_file.cpp:293-311)
+void QSOFile::readMinMaxMedRedshift(double &zmin, double &zmax, double &zmed) {
+    if (wave_.empty()) {
+        readData();
+        _cutMaskedBoundary();
+    }
+    zmin = wave()[0] / LYA_REST - 1;
+    zmax = wave()[size() - 1] / LYA_REST - 1;
+    zmed = median_sorted(wave(), size()) / LYA_REST - 1;
+}
+
+// QSOFile::readAllocResolutionMatrix (io/qso_file.cpp:313-321): a binary file
+// holds none; a reader may have supplied one
+void QSOFile::readAllocResolutionMatrix() {
+    if (!pending_rmat) throw std::runtime_error("Cannot read resolution matrix from Binary file!");
+    Rmat = std::move(pending_rmat);
+}
+
+// QSOFile::recalcDvDLam (io/qso_file.cpp:325-329)
+void QSOFile::recalcDvDLam() {
+    std::vector<double> dl(size() - 1), dv(size() - 1);
+    for (int i = 0; i + 1 < size(); ++i) {
+        dl[i] = wave()[i + 1] - wave()[i];
+        dv[i] = SPEED_OF_LIGHT * std::log(wave()[i + 1] / wave()[i]);
+    }
+    dlambda = median_of(dl);
+    dv_kms = median_of(dv);
+}
+
+}  // namespace qio

@@ -90,22 +90,39 @@ int Engine::addChunk(const QSOSpectrum &q, int i1, int i2, qml_chunk_info *info,
 }
 
 void Engine::enqueue(Chunk *c, const double *ps, std::vector<std::unique_ptr<double[]>> *dbt, double *fisher) {
-    if (!ps || !dbt || !fisher || dbt->size() < 3)
+    // dbt and fisher both null: the chunk's own results only
+    const bool sums = dbt || fisher;
+    if (!ps || (sums && (!dbt || !fisher || dbt->size() < 3)))
         throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: null estimate or sums");
     if (queue_.empty()) {
         ps_ = ps;
-        dbt_ = dbt;
-        fisher_ = fisher;
-    } else if (ps != ps_ || dbt != dbt_ || fisher != fisher_) {
-        throw QMLError(QML_ERR_ARG,
-                       "Chunk::oneQSOiteration: all chunks of one flush share the estimate and the sums");
+    } else if (ps != ps_) {
+        throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: all chunks of one flush share the estimate");
     }
-    if (std::find(queue_.begin(), queue_.end(), c) == queue_.end()) queue_.push_back(c);
+    if (sums) {
+        if (!dbt_) {
+            dbt_ = dbt;
+            fisher_ = fisher;
+        } else if (dbt != dbt_ || fisher != fisher_) {
+            throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: all chunks of one flush share the sums");
+        }
+    }
+    const auto it = std::find(queue_.begin(), queue_.end(), c);
+    if (it == queue_.end()) {
+        queue_.push_back(c);
+        queue_sums_.push_back(sums ? 1 : 0);
+    } else if (sums) {
+        queue_sums_[it - queue_.begin()] = 1;
+    }
 }
 
 void Engine::forget(Chunk *c) {
     if (c->id_ >= 0 && c->id_ < (int)chunks_.size()) chunks_[c->id_] = nullptr;
-    queue_.erase(std::remove(queue_.begin(), queue_.end(), c), queue_.end());
+    for (size_t k = queue_.size(); k-- > 0;)
+        if (queue_[k] == c) {
+            queue_.erase(queue_.begin() + k);
+            queue_sums_.erase(queue_sums_.begin() + k);
+        }
 }
 
 void Engine::commit() {
@@ -129,9 +146,9 @@ int Engine::autotune(const double *ps_estimate, const std::vector<int> &caps, in
 void Engine::flush() {
     if (queue_.empty()) return;
     commit();
-    // only enqueued chunks enter the sums
+    // only chunks enqueued with the sums enter them
     std::vector<int> want(chunks_.size(), 0);
-    for (Chunk *c : queue_) want[c->id_] = 1;
+    for (size_t k = 0; k < queue_.size(); ++k) want[queue_[k]->id_] = queue_sums_[k];
     for (size_t id = 0; id < chunks_.size(); ++id)
         if (want[id] != active_[id]) {
             check(qml_set_chunk_active(ctx_, (int)id, want[id]));
@@ -139,26 +156,36 @@ void Engine::flush() {
         }
     const int T = bins_.totalKzBins();
     std::vector<double> dbt((size_t)3 * T, 0.0);
-    check(qml_iterate(ctx_, ps_, fisher_, dbt.data()));
-    for (int k = 0; k < 3; ++k) {
-        double *dst = (*dbt_)[k].get();
-        for (int i = 0; i < T; ++i) dst[i] += dbt[(size_t)k * T + i];
+    std::vector<double> scratch_f;
+    double *fisher = fisher_;
+    if (!fisher) {   // results-only chunks: the sums go nowhere
+        scratch_f.assign((size_t)T * T, 0.0);
+        fisher = scratch_f.data();
     }
+    check(qml_iterate(ctx_, ps_, fisher, dbt.data()));
+    if (dbt_)
+        for (int k = 0; k < 3; ++k) {
+            double *dst = (*dbt_)[k].get();
+            for (int i = 0; i < T; ++i) dst[i] += dbt[(size_t)k * T + i];
+        }
     std::vector<double> flat;
     for (Chunk *c : queue_) {
         const int nq = c->N_Q_MATRICES;
         flat.assign((size_t)3 * nq, 0.0);
         int st = 0;
-        check(qml_get_chunk_result(ctx_, c->id_, c->fisher_matrix.get(), flat.data(), &st));
-        for (int k = 0; k < 3; ++k)
-            std::copy(flat.begin() + (size_t)k * nq, flat.begin() + (size_t)(k + 1) * nq,
-                      c->dbt_estimate_before_fisher_vector[k].get());
+        double *fm = c->result_fisher ? c->result_fisher : c->fisher_matrix.get();
+        check(qml_get_chunk_result(ctx_, c->id_, fm, flat.data(), &st));
+        for (int k = 0; k < 3; ++k) {
+            double *dst = c->result_dbt[k] ? c->result_dbt[k] : c->dbt_estimate_before_fisher_vector[k].get();
+            std::copy(flat.begin() + (size_t)k * nq, flat.begin() + (size_t)(k + 1) * nq, dst);
+        }
         c->status_ = st;
         if (st != QML_CHUNK_OK)
             std::fprintf(stderr, "Chunk::oneQSOiteration: covariance of chunk %d not positive definite; "
                                  "skipping its accumulation.\n", c->id_);
     }
     queue_.clear();
+    queue_sums_.clear();
     ps_ = nullptr;
     dbt_ = nullptr;
     fisher_ = nullptr;
@@ -232,7 +259,13 @@ Chunk::~Chunk() {
 void Chunk::oneQSOiteration(const double *ps_estimate, std::vector<std::unique_ptr<double[]>> &dbt_sum_vector,
                             double *fisher_sum) {
     if (!eng_) throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: the engine is gone");
+    if (!fisher_sum) throw QMLError(QML_ERR_ARG, "Chunk::oneQSOiteration: null Fisher sum");
     eng_->enqueue(this, ps_estimate, &dbt_sum_vector, fisher_sum);
+}
+
+void Chunk::enqueueResultsOnly(const double *ps_estimate) {
+    if (!eng_) throw QMLError(QML_ERR_ARG, "Chunk::computePSbeforeFvector: the engine is gone");
+    eng_->enqueue(this, ps_estimate, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------- OneQSOEstimate --

@@ -155,6 +155,7 @@ private:
     std::vector<Chunk *> chunks_;          // by engine chunk id; null once destroyed
     std::vector<int> active_;              // accumulation flag last sent to the device
     std::vector<Chunk *> queue_;
+    std::vector<int> queue_sums_;          // 1 if the queued chunk enters the sums
     const double *ps_ = nullptr;
     std::vector<std::unique_ptr<double[]>> *dbt_ = nullptr;
     double *fisher_ = nullptr;
@@ -188,6 +189,14 @@ public:
     // triangle) and 3 x TOTAL_KZ_BINS.
     void oneQSOiteration(const double *ps_estimate, std::vector<std::unique_ptr<double[]>> &dbt_sum_vector,
                          double *fisher_sum);
+    // Enqueues this chunk for its own results only (it enters no sums).
+    void enqueueResultsOnly(const double *ps_estimate);
+
+    // Where flush() writes this chunk's results: fisher_matrix and
+    // dbt_estimate_before_fisher_vector unless redirected here
+    // (N_Q_MATRICES^2 and 3 arrays of N_Q_MATRICES).
+    double *result_fisher = nullptr;
+    double *result_dbt[3] = {nullptr, nullptr, nullptr};
 
 private:
     friend class Engine;

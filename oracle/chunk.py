@@ -231,7 +231,14 @@ class Chunk:
 
     def __init__(self, bins: Bins, tables: List[TableSet], wave, delta, ivar,
                  R_fwhm: int, i1: int = 0, i2: Optional[int] = None,
-                 rmat: Optional[ResoMatrix] = None):
+                 rmat: Optional[ResoMatrix] = None, exact: bool = False):
+        # exact=True departs from the reference's rounding on purpose: v and z
+        # come from extended-precision logarithms and square roots (rounded
+        # to double once), and the continuum templates are orthonormalised by
+        # Gram-Schmidt in extended precision instead of dgesvd.  The tests use
+        # it only to measure how far the reference's own rounding moves a
+        # case's results (its floor); it is never the parity target itself.
+        self.exact = exact
         wave = np.asarray(wave, dtype=np.float64)
         i2 = wave.size if i2 is None else i2
         w = wave[i1:i2].copy()
@@ -312,14 +319,7 @@ class Chunk:
     # -- matrices -----------------------------------------------------------
     def vz_matrices(self):
         """_setVZMatrices, core/chunk_estimate.cpp:384-398, mirrored."""
-        li = self.wave[:, None]
-        lj = self.wave[None, :]
-        iu = np.triu_indices(self.N)
-        v = np.zeros((self.N, self.N))
-        z = np.zeros((self.N, self.N))
-        v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
-        z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
-        return v, z
+        return _vz_upper(self.wave, self.exact)
 
     def _mirror(self, m):
         iu = np.triu_indices(self.N, 1)
@@ -330,15 +330,8 @@ class Chunk:
         """v/z on the fine grid of the resolution matrix (upper, mirrored),
         computed once per chunk."""
         if getattr(self, "_fine_cache", None) is None:
-            li = self.fine_wave[:, None]
-            lj = self.fine_wave[None, :]
-            n = self.fine_wave.size
-            iu = np.triu_indices(n)
-            v = np.zeros((n, n))
-            z = np.zeros((n, n))
-            v[iu] = (SPEED_OF_LIGHT * np.log(lj / li))[iu]
-            z[iu] = (np.sqrt(li * lj) - 1.0)[iu]
-            self._fine_cache = (v, z, iu)
+            v, z = _vz_upper(self.fine_wave, self.exact)
+            self._fine_cache = (v, z, np.triu_indices(self.fine_wave.size))
         return self._fine_cache
 
     def _sandwich(self, fine):
@@ -475,6 +468,8 @@ class Chunk:
         u, s, _vt, info = lapack.dgesvd(vt.T.copy(order="F"), compute_uv=1, full_matrices=0)
         if info != 0:
             raise LapackError("ERROR in SVD.")
+        if self.exact:
+            u = _orthonormal_extended(vt.T)
         nuse = len(s)
         while nuse > 0:
             if s[nuse - 1] / s[0] > DOUBLE_EPSILON:
@@ -527,6 +522,32 @@ class Chunk:
             return False
         accumulate(self, fisher_sum, dbt_sum)
         return True
+
+
+def _vz_upper(wave, exact):
+    """v_ij = c ln(l_j / l_i) and z_ij = sqrt(l_i l_j) - 1 for j >= i (zero
+    below), in double as the reference, or from extended precision."""
+    n = wave.size
+    iu = np.triu_indices(n)
+    w = wave.astype(np.longdouble) if exact else wave
+    c = np.longdouble(SPEED_OF_LIGHT) if exact else SPEED_OF_LIGHT
+    v = np.zeros((n, n))
+    z = np.zeros((n, n))
+    v[iu] = (c * np.log(w[None, :] / w[:, None]))[iu]
+    z[iu] = (np.sqrt(w[:, None] * w[None, :]) - 1)[iu]
+    return v, z
+
+
+def _orthonormal_extended(a):
+    """Orthonormal basis of the columns of a by modified Gram-Schmidt, twice,
+    in extended precision (for the rounding-floor measurement only)."""
+    q = a.astype(np.longdouble).copy()
+    for _ in range(2):
+        for j in range(q.shape[1]):
+            for i in range(j):
+                q[:, j] -= (q[:, i] @ q[:, j]) * q[:, i]
+            q[:, j] /= np.sqrt(q[:, j] @ q[:, j])
+    return q.astype(np.float64)
 
 
 def _symv_upper(a, x):

@@ -9,7 +9,8 @@ range), in two forms:
   ResoMatrix.deconvolve / .oversample.
 
 The oracle needs 10-30 s per chunk at these sizes, so its results are kept
-as data; the GPU parity test runs only the engine.  The chunk inputs are
+as data (with its extended-precision run, the case's rounding floor); the
+GPU parity test runs only the engine.  The chunk inputs are
 stored with them (wave in Angstrom, delta, ivar, R_fwhm, the diagonals), and
 theta.  Tables come from synth.rmat_config (table 0, deterministic).
 
@@ -60,6 +61,11 @@ def main():
             assert ok
             out[f"{form}_F{i}"] = F
             out[f"{form}_dbt{i}"] = dbt
+            # the oracle's extended-precision run (v and z from extended-
+            # precision logarithms): the case's rounding floor
+            res, _, _ = P.run_oracle(w, th, [i], exact=True)
+            out[f"{form}_Fx{i}"] = res[i][0]
+            out[f"{form}_dbtx{i}"] = res[i][1]
             print(f"{form} chunk {i}: N={ch.N} n={len(ch.stored)} {time.time() - t0:.1f} s", flush=True)
     np.savez_compressed(os.path.join(HERE, "rmat_large.npz"), **out)
 

@@ -38,8 +38,10 @@ def theta_perturbed(b: Binning, pd13, scale, seed):
     return scale * pf * rng.standard_normal(pf.size)
 
 
-def run_oracle(w: synth.Workload, theta, idx):
-    """Per-chunk (F, dbt) of the oracle for workload chunks idx, plus the sums."""
+def run_oracle(w: synth.Workload, theta, idx, exact=False):
+    """Per-chunk (F, dbt) of the oracle for workload chunks idx, plus the sums
+    (exact=True: the oracle's extended-precision v/z and template basis, for
+    measuring a case's rounding floor)."""
     b = oracle_bins(w.binning)
     tabs = oracle_tables(w)
     T = b.total_kz_bins
@@ -48,7 +50,7 @@ def run_oracle(w: synth.Workload, theta, idx):
     for i in idx:
         c = w.chunks[i]
         try:
-            ch = OracleChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"], rmat=oracle_rmat(c))
+            ch = OracleChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"], rmat=oracle_rmat(c), exact=exact)
         except ChunkError:
             continue
         ok = ch.one_iteration(theta, Fs, ds)
@@ -129,32 +131,72 @@ def fmt(e):
     return " ".join(f"{k} {v:.1e}" for k, v in e.items())
 
 
-def assert_fp64(Fa, dbta, Fb, dbtb, what="", tol=FP64_TOL, elem_tol=ELEM_TOL):
+# Where the reference's own rounding moves a case's results by more than the
+# flat bar (resolution-matrix windows, whose phase k v carries the rounding of
+# v = c ln(l_j / l_i); template sets with s_min / s_0 far below 1, whose span
+# dgesvd fixes only to about eps s_0 / s_min), the bar is FLOOR_FACTOR times
+# that floor: the oracle's deviation from its own extended-precision run.
+FLOOR_FACTOR = 3.0
+
+
+def _bars(tol, elem_tol, floor):
+    def bar(k):
+        base = tol if k.endswith("_norm") else elem_tol
+        return max(base, FLOOR_FACTOR * floor[k]) if floor and k in floor else base
+    return bar
+
+
+def _check(e, what, tol, elem_tol, floor, label):
+    bar = _bars(tol, elem_tol, floor)
+    bad = {k: (v, bar(k)) for k, v in e.items()
+           if (k.endswith("_norm") or k.endswith("_elem")) and not v < bar(k)}
+    bad.update({k: (v, None) for k, v in e.items() if not np.isfinite(v)})
+    if floor:
+        print(f"[fp64 parity] {what} floor (oracle vs its extended-precision run): {fmt(floor)}", flush=True)
+    assert not bad, f"{what}: FP64 parity bar missed{label} (value, bar): {bad}"
+
+
+def assert_fp64(Fa, dbta, Fb, dbtb, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None):
     """The FP64 parity bar: every normalised error below tol and every
-    elementwise error (entries above the floor) below elem_tol."""
+    elementwise error (entries above the floor) below elem_tol, or below
+    FLOOR_FACTOR times the case's rounding floor where one is given."""
     e = errors(Fa, dbta, Fb, dbtb)
     print(f"[fp64 parity] {what}: {fmt(e)}", flush=True)
-    bad = {k: v for k, v in e.items() if (k.endswith("_norm") and not v < tol)
-           or (k.endswith("_elem") and not v < elem_tol) or not np.isfinite(v)}
-    assert not bad, f"{what}: FP64 parity bar missed: {bad}"
+    _check(e, what, tol, elem_tol, floor, "")
     return e
 
 
-def assert_chunks_fp64(out, oout, what="", **kw):
-    """assert_fp64 on every chunk both sides accepted; returns the worst of
-    each error."""
-    assert set(out) == set(oout), f"{what}: accepted chunks differ"
+def worst_errors(out, oout):
+    """The worst of each error over the chunks both sides accepted."""
+    assert set(out) == set(oout), "accepted chunks differ"
     worst = {}
     for i in sorted(out):
-        e = errors(out[i][0], out[i][1], oout[i][0], oout[i][1])
-        for k, v in e.items():
+        for k, v in errors(out[i][0], out[i][1], oout[i][0], oout[i][1]).items():
             worst[k] = max(worst.get(k, 0.0), v)
+    return worst
+
+
+def oracle_floor(w, theta, idx, oout=None, gout=None, what=""):
+    """The case's rounding floor: worst per-chunk deviation of the oracle from
+    its extended-precision run (oracle.chunk.Chunk exact=True).  With the
+    engine's per-chunk results gout, also prints their deviation from that
+    run."""
+    if oout is None:
+        oout = run_oracle(w, theta, idx)[0]
+    xout = run_oracle(w, theta, idx, exact=True)[0]
+    if gout is not None:
+        print(f"[fp64 parity] {what} engine vs the oracle's extended-precision run: "
+              f"{fmt(worst_errors(gout, xout))}", flush=True)
+    return worst_errors(oout, xout)
+
+
+def assert_chunks_fp64(out, oout, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None):
+    """assert_fp64 on every chunk both sides accepted (the worst of each
+    error over the chunks); returns the worst errors."""
+    assert set(out) == set(oout), f"{what}: accepted chunks differ"
+    worst = worst_errors(out, oout)
     print(f"[fp64 parity] {what} worst per chunk over {len(out)}: {fmt(worst)}", flush=True)
-    tol = kw.get("tol", FP64_TOL)
-    elem_tol = kw.get("elem_tol", ELEM_TOL)
-    bad = {k: v for k, v in worst.items() if (k.endswith("_norm") and not v < tol)
-           or (k.endswith("_elem") and not v < elem_tol) or not np.isfinite(v)}
-    assert not bad, f"{what}: FP64 parity bar missed per chunk: {bad}"
+    _check(worst, what, tol, elem_tol, floor, " per chunk")
     return worst
 
 
@@ -169,18 +211,32 @@ def p1d_deviation(Fa, dbta, Fb, dbtb):
     return float(np.max(np.abs(ra["theta"] - rb["theta"])[m] / s[m]))
 
 
-# Mixed-precision bars against the FP64 path: normalised F, d/b/t (d is
-# FP64 in every mode and must agree to rounding).
+# Mixed-precision bars against the FP64 path, per workload and precision:
+# about ten times the errors measured on MI355X for the same chunks (the
+# kernels are deterministic).  Normalised F by sqrt(F_kk F_k'k'), b and t by
+# sqrt(F_kk); d is FP64 in every mode and must agree to rounding.  Single
+# high-resolution chunks carry a noise bias b far above sqrt(F_kk), so their
+# b bar is the loosest.
 MIXED_TOL = {
-    "fp32": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
-    "bf16x3": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
-    "bf16x4": dict(F_norm=2e-3, b_norm=2e-3, t_norm=2e-3),
+    ("desi", "fp32"): dict(F_norm=4e-5, b_norm=2e-4, t_norm=3e-5),
+    ("desi", "bf16x3"): dict(F_norm=3e-4, b_norm=5e-4, t_norm=4e-4),
+    ("desi", "bf16x4"): dict(F_norm=2e-4, b_norm=3e-4, t_norm=2e-4),
+    ("hires", "fp32"): dict(F_norm=6e-5, b_norm=5e-3, t_norm=4e-4),
+    ("hires", "bf16x3"): dict(F_norm=1.2e-3, b_norm=1.2e-2, t_norm=4e-3),
+    ("hires", "bf16x4"): dict(F_norm=6e-4, b_norm=8e-3, t_norm=2e-3),
+    ("rmat", "fp32"): dict(F_norm=3e-5, b_norm=7e-5, t_norm=3e-5),
+    ("rmat", "bf16x3"): dict(F_norm=1.2e-4, b_norm=3e-4, t_norm=4e-4),
+    ("rmat", "bf16x4"): dict(F_norm=1e-4, b_norm=2e-4, t_norm=3e-4),
+    ("growth", "fp32"): dict(F_norm=7e-5, b_norm=1.5e-4, t_norm=3e-5),
+    ("growth", "bf16x3"): dict(F_norm=1.3e-4, b_norm=3e-4, t_norm=4e-4),
 }
-# P1D bar: max |dP| / sigma
-P1D_TOL = {("desi", "bf16x3"): 1e-3, ("hires", "bf16x4"): 1e-3}
+# P1D bar, max |dP| / sigma: 1e-3, or ten times the measured value where
+# that is lower (DESI sample: 4.9e-5; high-resolution sample in BF16x4:
+# 4.6e-4, so the 1e-3 bar itself)
+P1D_TOL = {("desi", "bf16x3"): 5e-4, ("hires", "bf16x4"): 1e-3}
 
 
-def assert_mixed(Fm, dm, F64, d64, precision, what=""):
+def assert_mixed(Fm, dm, F64, d64, precision, what="", workload="desi"):
     """Errors of a mixed-precision run against the FP64 run on the same
     chunks: normalised F, d, b, t, and b/t relative to their largest entry."""
     e = dict(F_norm=fisher_norm_err(Fm, F64),
@@ -190,6 +246,6 @@ def assert_mixed(Fm, dm, F64, d64, precision, what=""):
     print(f"[mixed] {what}: {fmt(e)}", flush=True)
     assert np.isfinite(Fm).all() and np.isfinite(dm).all(), f"{what}: non-finite results"
     assert e["d_max"] < 1e-12, f"{what}: d_k is FP64 in every precision"
-    bad = {k: e[k] for k, tol in MIXED_TOL[precision].items() if not e[k] < tol}
-    assert not bad, f"{what}: mixed-precision bar missed: {bad}"
+    bad = {k: (e[k], tol) for k, tol in MIXED_TOL[(workload, precision)].items() if not e[k] < tol}
+    assert not bad, f"{what}: mixed-precision bar missed (value, bar): {bad}"
     return e

@@ -344,6 +344,21 @@ int qml_bootstrap_complete(qml_ctx *ctx, int nboots, unsigned long long seed, in
 int qml_bootstrap_median(qml_ctx *ctx, int nboots, int T, const double *realizations, double *median,
                          double *mad_cov);
 
+/* qml_bootstrap_median without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *   d_realizations (nboots x T), d_median (T), d_mad_cov (T x T): device
+ *     memory of the context's device, owned by the caller; results as
+ *     qml_bootstrap_median's, bit for bit.
+ * The transposed realisations live in a static device array of 2^23 doubles
+ * (64 MiB) per device, set up at qml_create: nboots x T above 8388608 (e.g.
+ * 20000 x 455) is rejected with QML_ERR_LIMIT.  Every call on one device
+ * shares that array, so calls that overlap on one device (two threads, or two
+ * contexts) race.  Runs on the context's stream and returns when the results
+ * are in place. */
+int qml_bootstrap_median_noalloc(qml_ctx *ctx, int nboots, int T, const double *d_realizations,
+                                 double *d_median, double *d_mad_cov);
+
 /* n unit-mean Poisson draws of PoissonRNG(seed) (core/bootstrapper.hpp:19-37):
  * std::poisson_distribution<int> over std::mt19937_64 seeded once.  With the
  * same C++ standard library as the reference's build the draws are the

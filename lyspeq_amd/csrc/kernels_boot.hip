@@ -21,6 +21,8 @@
 #include "qml_common.hpp"
 #include "qml_kernels.hpp"
 
+#include <mutex>
+
 namespace qml {
 
 namespace {
@@ -503,6 +505,35 @@ void launch_boot_median(hipStream_t st, const double *R, int nboots, int T, doub
     k_boot_transpose<<<dim3((T + 63) / 64, (nboots + 63) / 64), 256, 0, st>>>(R, nboots, T, X);
     k_boot_median<<<T, 256, 0, st>>>(X, nboots, median);
     k_boot_mad<<<dim3(T, T), 256, 0, st>>>(X, nboots, T, mad);
+}
+
+// Scratch of the allocation-free median bootstrap: the realisations
+// transposed to band-major, BOOT_NOALLOC_MAX doubles (64 MiB) per device,
+// held from the library's load on a device until the process ends.  Its
+// address on each device comes from hipGetSymbolAddress at qml_create (with
+// that device current); host code never passes the array's own name.
+static __device__ __align__(256) double scratch[BOOT_NOALLOC_MAX];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[BOOT_NOALLOC_DEVICES] = {};
+}
+
+int boot_noalloc_setup(int device) {
+    if (device < 0 || device >= BOOT_NOALLOC_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return (int)hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[device] = (double *)ptr;
+    return (int)hipSuccess;
+}
+
+double *boot_noalloc_scratch(int device) {
+    if (device < 0 || device >= BOOT_NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace qml

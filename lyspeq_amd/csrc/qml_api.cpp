@@ -218,6 +218,12 @@ int qml_create(qml_ctx **out, const qml_opts *opts) {
         delete c;
         return fail(QML_ERR_HIP, "qml_create: stream/event creation failed");
     }
+    // per-device address of the allocation-free median bootstrap's scratch
+    if (const int e = boot_noalloc_setup(c->device); e != (int)hipSuccess) {
+        qml_destroy(c);
+        return fail(QML_ERR_HIP, std::string("qml_create: scratch symbol lookup: ") +
+                                     hipGetErrorString((hipError_t)e));
+    }
     *out = c;
     return QML_OK;
 }
@@ -1482,6 +1488,44 @@ int qml_bootstrap_median(qml_ctx *c, int nboots, int T, const double *realizatio
     void *ps[] = {d_R, d_X, d_med, d_mad};
     for (void *p : ps) if (p) hipFree(p);
     return rc;
+}
+
+int qml_bootstrap_median_noalloc(qml_ctx *c, int nboots, int T, const double *d_realizations, double *d_median,
+                                 double *d_mad_cov) {
+    if (!c || nboots < 1 || T < 1 || !d_realizations || !d_median || !d_mad_cov)
+        return fail(QML_ERR_ARG,
+                    "qml_bootstrap_median_noalloc: a context, nboots >= 1, T >= 1 and every array are required");
+    if (T > 65535) return fail(QML_ERR_LIMIT, "qml_bootstrap_median_noalloc: more than 65535 bands");
+    if ((long long)nboots * T > BOOT_NOALLOC_MAX) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "qml_bootstrap_median_noalloc: nboots x T = %lld exceeds the %lld-element scratch "
+                      "(use qml_bootstrap_median)", (long long)nboots * T, (long long)BOOT_NOALLOC_MAX);
+        return fail(QML_ERR_LIMIT, msg);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    // the arrays must be device memory of this context's device: a host
+    // pointer would fault in the kernels
+    const void *arrays[] = {d_realizations, d_median, d_mad_cov};
+    for (const void *a : arrays) {
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, a) != hipSuccess || at.device != c->device ||
+            (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)) {
+            (void)hipGetLastError();
+            return fail(QML_ERR_ARG, "qml_bootstrap_median_noalloc: arrays must be device memory of the "
+                                     "context's device");
+        }
+    }
+    double *X = boot_noalloc_scratch(c->device);
+    if (!X) return fail(QML_ERR_HIP, "qml_bootstrap_median_noalloc: scratch not set up on this device");
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    // the transpose writes every scratch element the selections read, so
+    // nothing needs clearing between calls
+    launch_boot_median(c->st, d_realizations, nboots, T, X, d_median, d_mad_cov);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return QML_OK;
 }
 
 int qml_num_chunks(qml_ctx *c) { return c ? (int)c->chunks.size() : 0; }

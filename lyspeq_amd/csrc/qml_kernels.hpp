@@ -91,6 +91,15 @@ void launch_boot_meancov(hipStream_t st, const double *R, int nboots, int T, dou
 // covariance (T x T); X is T x nboots scratch (deviations from the median).
 void launch_boot_median(hipStream_t st, const double *R, int nboots, int T, double *X, double *median,
                         double *mad);
+// Allocation-free median bootstrap: a static device array of
+// BOOT_NOALLOC_MAX doubles holds the transposed realisations.
+// boot_noalloc_setup looks its address up on the current device (once per
+// device; returns a hipError_t code); boot_noalloc_scratch returns it, or
+// null before the setup.
+constexpr long long BOOT_NOALLOC_MAX = 1LL << 23;
+constexpr int BOOT_NOALLOC_DEVICES = 64;
+int boot_noalloc_setup(int device);
+double *boot_noalloc_scratch(int device);
 // The complete form: per-spectrum collapsed Fisher upper blocks (ndim x ndim
 // at fq_off); per batch of nb realisations from b0, F_b and its right-hand
 // side in the matrix layout of the batched Cholesky (A, L, Li per

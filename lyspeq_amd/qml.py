@@ -115,7 +115,7 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_timings", "qml_set_pipeline", "qml_set_chunk_active", "qml_num_chunks",
            "qml_total_kz_bins", "qml_rmat_deconvolve", "qml_rmat_oversample", "qml_make_tables",
            "qml_set_graph", "qml_autotune", "qml_bootstrap", "qml_poisson_draws", "qml_set_kernel_stages",
-           "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median"]
+           "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median", "qml_bootstrap_median_noalloc"]
 
 
 class _Schedule(C.Structure):
@@ -163,6 +163,7 @@ def load_library(path: str = LIB_PATH):
     ip = C.POINTER(C.c_int)
     lib.qml_bootstrap_complete.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, ip, ip, dp, ip, ip, dp, dp]
     lib.qml_bootstrap_median.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
+    lib.qml_bootstrap_median_noalloc.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
     lib.qml_autotune.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     lib.qml_set_chunk_active.argtypes = [vp, C.c_int, C.c_int]
     lib.qml_num_chunks.argtypes = [vp]

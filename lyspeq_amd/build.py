@@ -80,9 +80,10 @@ def build_host(verbose: bool = False) -> str:
     tmp = HOST_LIB + ".tmp"
     # the drop-in Chunk / OneQSOEstimate, with this build's mirrors of the
     # reference globals and reader (replaced by the reference's own inside its tree)
-    dropin = [os.path.join(HERE, "host", "dropin", f) for f in ("dropin.cpp", "mirror.cpp")]
+    dropin = [os.path.join(HERE, "host", "dropin", f) for f in ("dropin.cpp", "mirror.cpp", "io/picca_file.cpp",
+                                                                 "io/bootstrap_file.cpp")]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
-          os.path.join(HERE, "host", "qml_host.cpp"), os.path.join(HERE, "host", "qml_driver.cpp"), *dropin, *link,
+          os.path.join(HERE, "host", "qml_host.cpp"), os.path.join(HERE, "host", "qml_driver.cpp"), *dropin, *link, "-lz",
           "-Wl,-rpath,$ORIGIN", "-o", tmp],
          "host library build", verbose)
     os.replace(tmp, HOST_LIB)

@@ -233,9 +233,13 @@ bool readSQTableFile(const std::string &path, TableData &out, std::string *err) 
     return true;
 }
 
-TableSource sqFileTables(const std::string &dir, const std::string &base) {
-    return [dir, base](int R, double dv, TableData &out) {
-        if (R < 0) return false;
+TableSource sqFileTables(const std::string &dir, const std::string &base, std::pair<int, double> first) {
+    return [dir, base, first](int R, double dv, TableData &out) {
+        if (R < 0) {   // resolution matrices: the first (R, dv) of the list
+            if (first.first < 0) return false;
+            R = first.first;
+            dv = first.second;
+        }
         char name[64];
         std::snprintf(name, sizeof name, "_R%d_dv%.1f.fits", R, dv);
         std::string err;

@@ -2,8 +2,9 @@
 // mathtools/matrix_helper.hpp:219-264) that the drop-in Chunk /
 // OneQSOEstimate use: the same names and meanings.  In the reference tree
 // the reference's own classes are used instead.  This version reads the
-// binary spectrum format (io/qso_file.cpp:332-369); other formats (picca
-// FITS) come in through qio::QSOFile::setReader.
+// binary spectrum format (io/qso_file.cpp:332-369) and picca's FITS delta
+// files (io/qso_file.cpp:376-677); other readers come in through
+// qio::QSOFile::setReader.
 #pragma once
 
 #include <functional>
@@ -87,5 +88,10 @@ public:
     // fills the arrays directly (a reader's or a test's data)
     void setData(std::vector<double> wave, std::vector<double> delta, std::vector<double> ivar);
 };
+
+// Fills q from HDU n + 1 of a picca delta file named "path[n]" (HDU 2 for a
+// bare path; gzip-compressed files too): the header keys, the wavelength /
+// delta / ivar columns and, if present, RESOMAT as pending_rmat.
+void readPiccaFile(const std::string &fname_qso, QSOFile &q);
 
 }  // namespace qio

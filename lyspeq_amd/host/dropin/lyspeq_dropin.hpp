@@ -5,6 +5,7 @@
 
 #include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "lyspeq_qml.h"
@@ -51,7 +52,11 @@ int ensureTable(int R, double dv);
 bool readSQTableFile(const std::string &path, TableData &out, std::string *err = nullptr);
 // A source reading <dir>/<base>_R<R>_dv<dv, one decimal>.fits
 // (sqhelper::SQTableFileNameConvention, io/sq_lookup_table_file.cpp:12-21).
-TableSource sqFileTables(const std::string &dir, const std::string &base);
+// With resolution matrices every chunk uses the first (R, dv) pair of the
+// reference's list (SQLookupTable's table 0, core/sq_table.cpp:165,189):
+// pass it as first.
+TableSource sqFileTables(const std::string &dir, const std::string &base,
+                         std::pair<int, double> first = {-1, 0.0});
 
 // Outside the reference tree: fills the bins:: and specifics:: globals of
 // core/global_numbers.hpp from a Binning, as readBins / readSpecifics would

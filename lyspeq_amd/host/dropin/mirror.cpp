@@ -209,8 +209,8 @@ static_assert(sizeof(BinaryHeader) == 88, "binary spectrum header is 88 bytes");
 
 void QSOFile::readParameters() {
     if (PB == Picca || reader_slot()) {
-        if (!reader_slot()) throw std::runtime_error("QSOFile: no reader for this file format");
-        reader_slot()(fname, *this);
+        if (reader_slot()) reader_slot()(fname, *this);
+        else readPiccaFile(fname, *this);
         _fullsize = arr_size = (int)wave_.size();
         if (R_kms == 0 && R_fwhm > 0) R_kms = SPEED_OF_LIGHT / R_fwhm / ONE_SIGMA_2_FWHM;
         return;

@@ -15,6 +15,10 @@
   values below 1e-15 set to 0 (io/qso_file.cpp:362-366).
 * Text matrices: "nrows ncols" then the values row by row
   (mathtools/matrix_helper.cpp:462-505).
+* picca delta files (written here for tests and for users feeding the
+  drop-in reader): one BINTABLE HDU per spectrum after an empty primary.
+* Bootstrap outputs (read here): the per-chunk results file and the
+  realizations file of io/bootstrap_file.cpp.
 
 The FITS reader handles exactly what the table writer produces (uncompressed
 BITPIX=-64 images in 2880-byte blocks), so no CFITSIO is needed.
@@ -128,6 +132,9 @@ def read_sq_table(path: str) -> SQTable:
 
 
 def _card(key: str, val) -> bytes:
+    if len(key) > 8:   # the HIERARCH convention for longer keywords
+        v = repr(val).upper() if isinstance(val, float) else (f"'{val}'" if isinstance(val, str) else int(val))
+        return f"HIERARCH {key} = {v}".ljust(80).encode("ascii")
     if isinstance(val, bool):
         v = "T" if val else "F"
         s = f"{key:<8}= {v:>20}"
@@ -171,6 +178,98 @@ def write_sq_table(path: str, t: SQTable) -> None:
         f.write(_image(t.deriv))
         f.write(_header(sig))
         f.write(_image(t.signal))
+
+
+_BINTABLE_FORMS = {"D": ">f8", "E": ">f4", "K": ">i8", "J": ">i4"}
+
+
+def _bintable(header, columns) -> bytes:
+    """One BINTABLE HDU: header (key, value) pairs after the structural keys;
+    columns (name, array, form) with form in D, E, K, J and arrays of shape
+    (N,) or (N, m) (a vector column of m values per row)."""
+    n = len(columns[0][1])
+    cols = []
+    for name, arr, form in columns:
+        a = np.asarray(arr, dtype=_BINTABLE_FORMS[form])
+        if a.shape[0] != n:
+            raise ValueError(f"column {name}: {a.shape[0]} rows, not {n}")
+        cols.append((name, a.reshape(n, -1), form))
+    row = np.zeros(n, dtype=[(f"c{i}", c[1].dtype, (c[1].shape[1],)) for i, c in enumerate(cols)])
+    for i, c in enumerate(cols):
+        row[f"c{i}"] = c[1]
+    cards = [("XTENSION", "BINTABLE"), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", row.dtype.itemsize),
+             ("NAXIS2", n), ("PCOUNT", 0), ("GCOUNT", 1), ("TFIELDS", len(cols))]
+    for i, (name, a, form) in enumerate(cols):
+        m = a.shape[1]
+        cards += [(f"TTYPE{i + 1}", name), (f"TFORM{i + 1}", f"{m}{form}" if m > 1 else form)]
+        if m > 1:
+            cards.append((f"TDIM{i + 1}", f"({m})"))
+    raw = row.tobytes()
+    return _header(cards + list(header)) + raw + b"\0" * ((-len(raw)) % _BLOCK)
+
+
+def write_picca_deltas(path: str, spectra, compress: bool = False) -> None:
+    """A picca delta file as io/qso_file.cpp:376-677 reads it: an empty
+    primary HDU, then one BINTABLE per spectrum.  Each spectrum is a
+    (header, columns) pair for ``_bintable``: header keys such as TARGETID or
+    THING_ID, Z, RA, DEC, MEANRESO, MEANSNR, DLL, DLAMBDA; columns LOGLAM or
+    LAMBDA, DELTA or DELTA_BLIND, IVAR and optionally RESOMAT (N x ndiags).
+    "path[k]" then names spectrum k (HDU k + 1).  compress=True gzips it."""
+    import gzip
+    data = _header([("SIMPLE", True), ("BITPIX", 8), ("NAXIS", 0), ("EXTEND", True)])
+    data += b"".join(_bintable(h, c) for h, c in spectra)
+    with (gzip.open(path, "wb") if compress else open(path, "wb")) as f:
+        f.write(data)
+
+
+@dataclass
+class ChunkResult:
+    """One HDU of a <base>-bootchunks-<pe>.fits file (written by
+    _saveChunkResults, core/quadratic_estimate.cpp:33-55, through
+    BootstrapChunksFile::writeChunk, io/bootstrap_file.cpp:134-185)."""
+    targetid: int
+    z_qso: float
+    ra: float
+    dec: float
+    istart: int
+    d: np.ndarray
+    b: np.ndarray
+    t: np.ndarray
+    fisher: np.ndarray   # ndim x ndim, symmetric, from the stored upper diagonals
+
+
+def read_boot_chunks(path: str):
+    """Every chunk of a bootchunks file: [d, b, t, diagonal 0, ..., diagonal
+    ndim - 1 of the upper triangle] per HDU; an empty primary means none."""
+    out = []
+    for cards, data in _read_hdus(path):
+        if data is None or "NQDIM" not in cards:
+            continue
+        n = int(cards["NQDIM"])
+        if data.size != 3 * n + n * (n + 1) // 2:
+            raise ValueError(f"chunk HDU of {data.size} values for NQDIM = {n}")
+        F = np.zeros((n, n))
+        pos = 3 * n
+        for d in range(n):
+            idx = np.arange(n - d)
+            F[idx, idx + d] = data[pos:pos + n - d]
+            F[idx + d, idx] = data[pos:pos + n - d]
+            pos += n - d
+        out.append(ChunkResult(int(cards["TARGETID"]), float(cards["ZQSO"]), float(cards["RA"]),
+                               float(cards["DEC"]), int(cards["ISTART"]), data[:n].copy(),
+                               data[n:2 * n].copy(), data[2 * n:3 * n].copy(), F))
+    return out
+
+
+def read_bootstrap_realizations(path: str):
+    """(powers (nboots, nk * nz), solver matrix (nk * nz, nk * nz), nboots,
+    nk, nz, fast) of a <base>-bootstrap-realizations.fits file
+    (io/bootstrap_file.cpp:17-120)."""
+    hdus = {c.get("EXTNAME"): (c, d) for c, d in _read_hdus(path)}
+    c, p = hdus["REALIZATIONS"]
+    _, inv = hdus["SOLV_INVF"]
+    nb, nk, nz = int(c["NBOOTS"]), int(c["NK"]), int(c["NZ"])
+    return (p.reshape(nb, nk * nz), inv.reshape(nk * nz, nk * nz), nb, nk, nz, bool(c["FASTBOOT"]))
 
 
 def sq_table_filename(directory: str, base: str, r: int, dv: float) -> str:

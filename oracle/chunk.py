@@ -31,6 +31,7 @@ brute force in extended precision for N <= 16.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Tuple
 
@@ -483,15 +484,18 @@ class Chunk:
             cinv += norm * np.outer(y, y)
         return cinv
 
-    def ps_before_fisher(self, cinv, sfid, qs):
+    def ps_before_fisher(self, cinv, sfid, qs, times=None):
         """computePSbeforeFvector, core/chunk_estimate.cpp:738-835."""
         N = self.N
         dk, nk_, tk = self.dbt
         y = _symv_upper(cinv, self.delta)
         ws = []
+        t0 = time.perf_counter()
         for i_kz, q in zip(self.stored, qs):
             dk[i_kz] = np.dot(y, _symv_upper(q, y))
             ws.append(cinv @ q)
+        if times is not None:
+            times["set_modqs"] = times.get("set_modqs", 0.0) + time.perf_counter() - t0
         iv = self.ivar
         with np.errstate(divide="ignore"):
             inv_iv = np.where(iv != 0, 1.0 / np.where(iv != 0, iv, 1.0), 0.0)
@@ -504,22 +508,43 @@ class Chunk:
                 tk[i_kz] = np.dot(wmat.ravel(), scinv.ravel())
         if self.bins.use_precomputed_fisher:
             return ws
+        t0 = time.perf_counter()
         for a, (i_kz, wa) in enumerate(zip(self.stored, ws)):
             wt = wa.T.ravel()
             for j_kz, wb in zip(self.stored[a:], ws[a:]):
                 self.fisher[i_kz, j_kz] = np.dot(wt, wb.ravel())
+        if times is not None:
+            times["set_fisher"] = times.get("set_fisher", 0.0) + time.perf_counter() - t0
         return ws
 
-    def one_iteration(self, theta, fisher_sum, dbt_sum, mode="lu"):
-        """Chunk::oneQSOiteration, core/chunk_estimate.cpp:837-914."""
+    def one_iteration(self, theta, fisher_sum, dbt_sum, mode="lu", times=None):
+        """Chunk::oneQSOiteration, core/chunk_estimate.cpp:837-914.  times: a
+        dict accumulating seconds per phase in the reference's mytime
+        categories (core/global_numbers.cpp:476-502): set_qs (v/z, Q_k and
+        their z weights), set_sfid, c_inv (the inverse with the continuum
+        marginalisation), set_modqs (d_k and W_k), set_fisher, and dbt_rest
+        (the b/t contractions)."""
+        clock = time.perf_counter
+        t0 = clock()
         v, z = self.vz_matrices()
         qs = self.q_matrices(v, z)
+        t1 = clock()
         sfid = None if self.bins.turn_off_sfid else self.signal_matrix(v, z)
+        t2 = clock()
         try:
             cinv = self.inverse_covariance(sfid, qs, theta, mode)
-            self.ps_before_fisher(cinv, sfid, qs)
+            t3 = clock()
+            inner = {} if times is not None else None
+            self.ps_before_fisher(cinv, sfid, qs, inner)
+            t4 = clock()
         except LapackError:
             return False
+        if times is not None:
+            for k, dt in (("set_qs", t1 - t0), ("set_sfid", t2 - t1), ("c_inv", t3 - t2)):
+                times[k] = times.get(k, 0.0) + dt
+            for k, dt in inner.items():
+                times[k] = times.get(k, 0.0) + dt
+            times["dbt_rest"] = times.get("dbt_rest", 0.0) + (t4 - t3) - sum(inner.values())
         accumulate(self, fisher_sum, dbt_sum)
         return True
 

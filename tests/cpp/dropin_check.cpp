@@ -9,11 +9,23 @@
 //        every file through OneQSOEstimate, oneQSOiteration at theta = 0,
 //        flush; then collapseBootstrap and addBootPowerOnly / addBoot with
 //        the weights of the config; results as text in <out>
+//   dropin_check bootio <base> <out>
+//        the bootstrap output files written from fixed arrays (a chunks file
+//        of three chunks, a realizations file); the arrays as text in <out>;
+//        exit 1 if readBootstrapRealizations does not return them
+//   dropin_check read binary|picca <file> <out>
+//        one file through qio::QSOFile (readParameters, readData and, if the
+//        file has one, readAllocResolutionMatrix); fields and arrays in <out>
 //
 // <config>: whitespace-separated "key value..." lines: kedges <n> <values>,
 // zcenters <n> <values>, zwidth, method, logl_order, lam_order, max_forest,
 // nchunks, max_pixel, fast_bootstrap, nboots, weights <nqso*nboots ints>,
-// file <path> (one per spectrum).
+// format binary|picca, rmat 0|1, deconv <m>, osamp <factor>,
+// first_table <R> <dv> (the table every chunk uses with rmat 1),
+// save_chunks <base> (_saveChunkResults' file after the flush), save_boot
+// <base> (the realizations file of the bootstrap powers, with the Fisher sum
+// standing in for the solver matrix), file <path> (one per spectrum).
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -24,6 +36,7 @@
 
 #include "core/chunk_estimate.hpp"
 #include "core/one_qso_estimate.hpp"
+#include "io/bootstrap_file.hpp"
 #include "lyspeq_dropin.hpp"
 
 using dbt_t = std::vector<std::unique_ptr<double[]>>;
@@ -87,7 +100,9 @@ static_assert(std::is_same<decltype(OneQSOEstimate::chunks), std::vector<std::un
 namespace {
 struct Config {
     lyspeq_amd::Binning b;
-    bool fast_bootstrap = true;
+    bool fast_bootstrap = true, picca = false, rmat = false;
+    std::pair<int, double> first_table{-1, 0.0};
+    std::string save_chunks, save_boot;
     int nboots = 0;
     std::vector<int> weights;
     std::vector<std::string> files;
@@ -135,6 +150,24 @@ Config read_config(const char *path) {
             in >> n;
             c.weights.resize(n);
             for (int &w : c.weights) in >> w;
+        } else if (key == "format") {
+            std::string v;
+            in >> v;
+            c.picca = v == "picca";
+        } else if (key == "rmat") {
+            int v;
+            in >> v;
+            c.rmat = v > 0;
+        } else if (key == "deconv") {
+            in >> c.b.resomat_deconvolution_m;
+        } else if (key == "osamp") {
+            in >> c.b.oversampling_factor;
+        } else if (key == "first_table") {
+            in >> c.first_table.first >> c.first_table.second;
+        } else if (key == "save_chunks") {
+            in >> c.save_chunks;
+        } else if (key == "save_boot") {
+            in >> c.save_boot;
         } else if (key == "file") {
             std::string f;
             in >> f;
@@ -176,7 +209,7 @@ int run(const Config &c, const char *dir, const char *base, const char *prec, co
     const std::string p(prec);
     o.precision = p == "fp64" ? QML_PREC_FP64 : p == "fp32" ? QML_PREC_FP32 : p == "bf16x3" ? QML_PREC_BF16X3
                                                                                               : QML_PREC_BF16X4;
-    lyspeq_amd::dropin::setup(o, lyspeq_amd::dropin::sqFileTables(dir, base));
+    lyspeq_amd::dropin::setup(o, lyspeq_amd::dropin::sqFileTables(dir, base, c.first_table));
     const int T = bins::TOTAL_KZ_BINS;
     std::vector<std::unique_ptr<OneQSOEstimate>> queue;
     for (const auto &f : c.files) queue.push_back(std::make_unique<OneQSOEstimate>(f));
@@ -199,6 +232,16 @@ int run(const Config &c, const char *dir, const char *base, const char *prec, co
             write_vec(out, "cF", ch->fisher_matrix.get(), (size_t)ch->N_Q_MATRICES * ch->N_Q_MATRICES);
             for (int k = 0; k < 3; ++k) write_vec(out, "cdbt", ch->dbt_estimate_before_fisher_vector[k].get(), ch->N_Q_MATRICES);
         }
+    if (!c.save_chunks.empty()) {   // _saveChunkResults (core/quadratic_estimate.cpp:33-55)
+        ioh::BootstrapChunksFile bfile(c.save_chunks, 0);
+        for (auto &q : queue)
+            for (auto &ch : q->chunks)
+                bfile.writeChunk(ch->dbt_estimate_before_fisher_vector[0].get(),
+                                 ch->dbt_estimate_before_fisher_vector[1].get(),
+                                 ch->dbt_estimate_before_fisher_vector[2].get(), ch->fisher_matrix.get(),
+                                 ch->N_Q_MATRICES, ch->fisher_index_start, ch->qFile->id, ch->qFile->z_qso,
+                                 ch->qFile->ra, ch->qFile->dec);
+    }
     // the bootstrapper's use (core/bootstrapper.hpp): collapse every spectrum,
     // then the fast form's rank-one updates or the complete form's sums
     std::vector<OneQSOEstimate *> live;
@@ -213,6 +256,10 @@ int run(const Config &c, const char *dir, const char *base, const char *prec, co
             live[s]->addBootPowerOnly(c.nboots, pcoeff.data(), power.data());
         }
         write_vec(out, "bootpower", power.data(), power.size());
+        if (!c.save_boot.empty())
+            ioh::saveBootstrapRealizations(c.save_boot, power.data(), fisher.data(), c.nboots,
+                                           bins::NUMBER_OF_K_BANDS, bins::NUMBER_OF_Z_BINS, specifics::FAST_BOOTSTRAP,
+                                           "drop-in check");
         if (!specifics::FAST_BOOTSTRAP) {
             // first realisation's complete-form sums
             std::vector<double> tp(T, 0.0), tf((size_t)T * T, 0.0);
@@ -229,13 +276,97 @@ int run(const Config &c, const char *dir, const char *base, const char *prec, co
 }
 }  // namespace
 
+// the bootstrap files from fixed arrays: no bins, no engine
+int bootio(const std::string &base, const char *out_path) {
+    FILE *out = std::fopen(out_path, "w");
+    if (!out) return 2;
+    {
+        ioh::BootstrapChunksFile bfile(base, 3);
+        const int dims[3] = {4, 1, 6};
+        for (int k = 0; k < 3; ++k) {
+            const int n = dims[k];
+            std::vector<double> p(n), nk(n), tk(n), F((size_t)n * n);
+            for (int i = 0; i < n; ++i) {
+                p[i] = std::sin(1.0 + i + 10 * k);
+                nk[i] = 1.0 / (3.0 + i + k);
+                tk[i] = -std::exp(-0.5 * i) * (k + 1);
+            }
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) F[(size_t)i * n + j] = std::cos(0.3 * i - 0.7 * j + k) * 1e-3;
+            const long id = 39627000000000000L + k;
+            bfile.writeChunk(p.data(), nk.data(), tk.data(), F.data(), n, 7 * k, id, 2.1 + 0.35 * k, 0.1 * k,
+                             -0.2 * k);
+            std::fprintf(out, "chunk %d %d %ld\n", n, 7 * k, id);
+            const double par[3] = {2.1 + 0.35 * k, 0.1 * k, -0.2 * k};
+            write_vec(out, "par", par, 3);
+            write_vec(out, "p", p.data(), n);
+            write_vec(out, "n", nk.data(), n);
+            write_vec(out, "t", tk.data(), n);
+            write_vec(out, "F", F.data(), F.size());
+        }
+    }
+    const unsigned nboots = 5;
+    const int nk = 3, nz = 2, T = nk * nz;
+    std::vector<double> pw((size_t)nboots * T), inv((size_t)T * T);
+    for (size_t i = 0; i < pw.size(); ++i) pw[i] = std::sqrt(2.0 + i) * (i % 2 ? -1 : 1);
+    for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / (1.0 + i);
+    const std::string name = ioh::saveBootstrapRealizations(base, pw.data(), inv.data(), nboots, nk, nz, true,
+                                                            "fixed arrays");
+    std::fprintf(out, "realizations %s\n", name.c_str() + 1);
+    write_vec(out, "powers", pw.data(), pw.size());
+    write_vec(out, "invf", inv.data(), inv.size());
+    std::fclose(out);
+    std::unique_ptr<double[]> rp, ri;
+    unsigned rn = 0;
+    int rk = 0, rz = 0;
+    bool rf = false;
+    ioh::readBootstrapRealizations(name.substr(1), rp, ri, rn, rk, rz, rf);
+    if (rn != nboots || rk != nk || rz != nz || !rf) return 1;
+    for (size_t i = 0; i < pw.size(); ++i)
+        if (rp[i] != pw[i]) return 1;
+    for (size_t i = 0; i < inv.size(); ++i)
+        if (ri[i] != inv[i]) return 1;
+    return 0;
+}
+
+// the reader alone: no bins, no engine
+int read_one(const std::string &format, const char *fname, const char *out_path) {
+    try {
+        qio::QSOFile q(fname, format == "picca" ? qio::Picca : qio::Binary);
+        q.readParameters();
+        q.readData();
+        FILE *out = std::fopen(out_path, "w");
+        if (!out) return 2;
+        std::fprintf(out, "id %ld\nR_fwhm %d\nexpid %d\nnight %d\nfiber %d\npetal %d\n", q.id, q.R_fwhm, q.expid,
+                     q.night, q.fiber, q.petal);
+        const double par[] = {q.z_qso, q.ra, q.dec, q.R_kms, q.snr, q.dv_kms, q.dlambda};
+        write_vec(out, "params", par, 7);
+        write_vec(out, "wave", q.wave(), q.size());
+        write_vec(out, "delta", q.delta(), q.size());
+        write_vec(out, "ivar", q.ivar(), q.size());
+        if (q.pending_rmat) {
+            q.readAllocResolutionMatrix();
+            std::fprintf(out, "rmat_nelem %d\n", q.Rmat->getNElemPerRow());
+            write_vec(out, "rmat", q.Rmat->matrix(), q.Rmat->getSize());
+        }
+        std::fclose(out);
+    } catch (std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: dropin_check cost|run <config> ...\n");
+        std::fprintf(stderr, "usage: dropin_check cost|run <config> ... | read binary|picca <file> <out>\n");
         return 2;
     }
+    if (std::string(argv[1]) == "read" && argc == 5) return read_one(argv[2], argv[3], argv[4]);
+    if (std::string(argv[1]) == "bootio" && argc == 4) return bootio(argv[2], argv[3]);
     const Config c = read_config(argv[2]);
-    lyspeq_amd::dropin::setGlobals(c.b, false, c.fast_bootstrap);
+    lyspeq_amd::dropin::setGlobals(c.b, c.rmat, c.fast_bootstrap);
+    if (c.picca) specifics::INPUT_QSO_FILE = qio::Picca;
     const std::string mode(argv[1]);
     if (mode == "cost") return cost(c);
     if (mode == "run" && argc == 7) return run(c, argv[3], argv[4], argv[5], argv[6]);

@@ -134,35 +134,38 @@ def fmt(e):
 # Where the reference's own rounding moves a case's results by more than the
 # flat bar (resolution-matrix windows, whose phase k v carries the rounding of
 # v = c ln(l_j / l_i); template sets with s_min / s_0 far below 1, whose span
-# dgesvd fixes only to about eps s_0 / s_min), the bar is FLOOR_FACTOR times
-# that floor: the oracle's deviation from its own extended-precision run.
+# dgesvd fixes only to about eps s_0 / s_min), the case's floor is the
+# oracle's deviation from its own extended-precision run.  The engine must
+# then lie no farther from that run than FLOOR_FACTOR floors, and hence no
+# farther from the oracle than FLOOR_FACTOR + 1.
 FLOOR_FACTOR = 3.0
 
 
-def _bars(tol, elem_tol, floor):
+def _check(e, what, tol, elem_tol, floor, factor, label):
     def bar(k):
         base = tol if k.endswith("_norm") else elem_tol
-        return max(base, FLOOR_FACTOR * floor[k]) if floor and k in floor else base
-    return bar
-
-
-def _check(e, what, tol, elem_tol, floor, label):
-    bar = _bars(tol, elem_tol, floor)
+        return max(base, factor * floor[k]) if floor and k in floor else base
     bad = {k: (v, bar(k)) for k, v in e.items()
            if (k.endswith("_norm") or k.endswith("_elem")) and not v < bar(k)}
     bad.update({k: (v, None) for k, v in e.items() if not np.isfinite(v)})
-    if floor:
-        print(f"[fp64 parity] {what} floor (oracle vs its extended-precision run): {fmt(floor)}", flush=True)
     assert not bad, f"{what}: FP64 parity bar missed{label} (value, bar): {bad}"
 
 
-def assert_fp64(Fa, dbta, Fb, dbtb, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None):
+def assert_fp64(Fa, dbta, Fb, dbtb, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None, exact=None):
     """The FP64 parity bar: every normalised error below tol and every
-    elementwise error (entries above the floor) below elem_tol, or below
-    FLOOR_FACTOR times the case's rounding floor where one is given."""
+    elementwise error (entries above the floor) below elem_tol.  With the
+    case's rounding floor and the oracle's extended-precision run (Fx, dbtx)
+    as exact, the bars widen to FLOOR_FACTOR + 1 floors against the oracle
+    and FLOOR_FACTOR floors against that run."""
     e = errors(Fa, dbta, Fb, dbtb)
     print(f"[fp64 parity] {what}: {fmt(e)}", flush=True)
-    _check(e, what, tol, elem_tol, floor, "")
+    if floor:
+        print(f"[fp64 parity] {what} floor (oracle vs its extended-precision run): {fmt(floor)}", flush=True)
+    _check(e, what, tol, elem_tol, floor, FLOOR_FACTOR + 1, " against the oracle")
+    if exact is not None:
+        ex = errors(Fa, dbta, exact[0], exact[1])
+        print(f"[fp64 parity] {what} engine vs the oracle's extended-precision run: {fmt(ex)}", flush=True)
+        _check(ex, what, tol, elem_tol, floor, FLOOR_FACTOR, " against the extended-precision run")
     return e
 
 
@@ -176,27 +179,30 @@ def worst_errors(out, oout):
     return worst
 
 
-def oracle_floor(w, theta, idx, oout=None, gout=None, what=""):
-    """The case's rounding floor: worst per-chunk deviation of the oracle from
-    its extended-precision run (oracle.chunk.Chunk exact=True).  With the
-    engine's per-chunk results gout, also prints their deviation from that
-    run."""
+def oracle_floor(w, theta, idx, oout=None):
+    """(floor, extended-precision per-chunk results, their sums): the case's
+    rounding floor is the worst per-chunk deviation of the oracle from its
+    extended-precision run (oracle.chunk.Chunk exact=True)."""
     if oout is None:
         oout = run_oracle(w, theta, idx)[0]
-    xout = run_oracle(w, theta, idx, exact=True)[0]
-    if gout is not None:
-        print(f"[fp64 parity] {what} engine vs the oracle's extended-precision run: "
-              f"{fmt(worst_errors(gout, xout))}", flush=True)
-    return worst_errors(oout, xout)
+    xout, Fx, dx = run_oracle(w, theta, idx, exact=True)
+    return worst_errors(oout, xout), xout, (Fx, dx)
 
 
-def assert_chunks_fp64(out, oout, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None):
+def assert_chunks_fp64(out, oout, what="", tol=FP64_TOL, elem_tol=ELEM_TOL, floor=None, exact=None):
     """assert_fp64 on every chunk both sides accepted (the worst of each
-    error over the chunks); returns the worst errors."""
+    error over the chunks); exact: the oracle's extended-precision per-chunk
+    results.  Returns the worst errors against the oracle."""
     assert set(out) == set(oout), f"{what}: accepted chunks differ"
     worst = worst_errors(out, oout)
     print(f"[fp64 parity] {what} worst per chunk over {len(out)}: {fmt(worst)}", flush=True)
-    _check(worst, what, tol, elem_tol, floor, " per chunk")
+    if floor:
+        print(f"[fp64 parity] {what} floor (oracle vs its extended-precision run): {fmt(floor)}", flush=True)
+    _check(worst, what, tol, elem_tol, floor, FLOOR_FACTOR + 1, " per chunk against the oracle")
+    if exact is not None:
+        wx = worst_errors(out, exact)
+        print(f"[fp64 parity] {what} engine vs the oracle's extended-precision run: {fmt(wx)}", flush=True)
+        _check(wx, what, tol, elem_tol, floor, FLOOR_FACTOR, " per chunk against the extended-precision run")
     return worst
 
 

@@ -226,3 +226,239 @@ def test_dropin_iteration_and_bootstrap_against_oracle(tmp_path, fast):
             tf[lo:lo + n, lo:lo + n] += weights[s_][0] * fishers[s_]
         np.testing.assert_allclose(rec["boot0power"], tp, rtol=1e-13, atol=0)
         np.testing.assert_allclose(rec["boot0fisher"].reshape(T, T), tf, rtol=1e-13, atol=0)
+
+
+def _read(tmp_path, fmt, path, name="read.txt"):
+    out = str(tmp_path / name)
+    r = subprocess.run([DROPIN_CHECK, "read", fmt, path, out], capture_output=True, text=True, timeout=60)
+    if r.returncode != 0:
+        return None, r.stderr
+    rec = {}
+    with open(out) as f:
+        for ln in f:
+            t = ln.split()
+            rec[t[0]] = int(t[1]) if len(t) == 2 else np.array(t[2:], float)
+    return rec, ""
+
+
+def test_picca_reader_matches_the_binary_file_of_the_same_spectrum(tmp_path):
+    """The reference's own test spectrum in both of its formats
+    (tests/input/desilite_seed332298_id0_0_z2.4_lognormal.{fits,dat}): the
+    picca reader (LAMBDA, DELTA, IVAR columns; TARGETID, MEANRESO, DLL keys)
+    and the binary reader must agree on the pixels, R_fwhm = 70000 and
+    dv = 20 km/s (DLL c ln 10 against the header's pixel width)."""
+    _need_exe()
+    g = os.path.join(os.path.dirname(__file__), "golden", "desilite_seed332298_id0_0_z2.4_lognormal")
+    pa, err = _read(tmp_path, "picca", g + ".fits", "a.txt")
+    assert pa is not None, err
+    bi, err = _read(tmp_path, "binary", g + ".dat", "b.txt")
+    assert bi is not None, err
+    assert pa["id"] == 1 and pa["R_fwhm"] == bi["R_fwhm"] == 70000
+    assert pa["expid"] == pa["night"] == pa["fiber"] == pa["petal"] == -1
+    assert pa["wave"].size == 488
+    assert np.array_equal(pa["wave"], bi["wave"]) and np.array_equal(pa["delta"], bi["delta"])
+    np.testing.assert_allclose(pa["ivar"], bi["ivar"], rtol=1e-14, atol=0)
+    # z, ra, dec, R_kms, snr, dv_kms, dlambda
+    np.testing.assert_allclose(pa["params"], bi["params"], rtol=1e-14, atol=0)
+    assert abs(pa["params"][5] - 20.0) < 1e-12
+
+
+def _picca_spectrum(rng, n, nd=0, form="lambda", ivar_form="D"):
+    wave = 3600.0 * np.exp(np.arange(n) * 20.0 / 299792.458)
+    cols = [("LOGLAM", np.log10(wave), "D")] if form == "loglam" else [("LAMBDA", wave, "D")]
+    cols += [("DELTA_BLIND" if form == "loglam" else "DELTA", rng.standard_normal(n), "D"),
+             ("IVAR", rng.uniform(0.5, 2.0, n), ivar_form)]
+    if nd:
+        cols.append(("RESOMAT", rng.uniform(0, 1, (n, nd)), "D"))
+    return cols
+
+
+def test_picca_reader_columns_keys_hdus_and_compression(tmp_path):
+    """Synthetic picca files (plain and gzip): "file[k]" selects spectrum k,
+    a bare name the first; LOGLAM becomes 10^x, DELTA_BLIND stands in for
+    DELTA, a float IVAR column is widened, THING_ID for TARGETID, the
+    optional keys default to -1, R_fwhm = int(c / MEANRESO / 2.35482004503 /
+    100 + 0.5) * 100, dv = DLL c 2.30258509299, and RESOMAT's N rows of
+    ndiags values come back as the diagonal storage ndiags x N
+    (io/qso_file.cpp:538-677).  Files missing a required key or column are
+    refused with the reference's messages."""
+    _need_exe()
+    rng = np.random.default_rng(5)
+    c = 299792.458
+    s0 = ([("TARGETID", 39627), ("Z", 2.75), ("RA", 0.5), ("DEC", -0.25), ("MEANRESO", 13.4),
+           ("MEANSNR", 3.5), ("DLL", 1e-4)], _picca_spectrum(rng, 150))
+    s1 = ([("THING_ID", 91), ("Z", 3.1), ("RA", 1.0), ("DEC", 0.1), ("MEANRESO", 60.0), ("MEANSNR", 1.5),
+           ("DLAMBDA", 0.8), ("EXPID", 12), ("NIGHT", 20210105), ("FIBER", 4), ("PETAL_LOC", 7)],
+          _picca_spectrum(rng, 211, nd=11, form="loglam", ivar_form="E"))
+    s2 = ([("TARGETID", 5), ("Z", 2.2), ("RA", 0.0), ("DEC", 0.0), ("MEANRESO", 20.0), ("MEANSNR", 1.0)],
+          [("FLUX", np.ones(120), "D"), ("DELTA", np.zeros(120), "D"), ("IVAR", np.ones(120), "D")])
+    s3 = ([("Z", 2.2), ("RA", 0.0), ("DEC", 0.0), ("MEANRESO", 20.0), ("MEANSNR", 1.0)], _picca_spectrum(rng, 120))
+    for compress in (False, True):
+        path = str(tmp_path / ("deltas.fits.gz" if compress else "deltas.fits"))
+        sqio.write_picca_deltas(path, [s0, s1, s2, s3], compress=compress)
+        r0, err = _read(tmp_path, "picca", path)
+        assert r0 is not None, err
+        r0b, _ = _read(tmp_path, "picca", path + "[1]", "r0b.txt")
+        assert np.array_equal(r0["wave"], r0b["wave"]) and r0b["id"] == 39627
+        cols0 = dict((k, v) for k, v, _ in s0[1])
+        assert r0["id"] == 39627 and r0["expid"] == r0["night"] == r0["fiber"] == r0["petal"] == -1
+        assert np.array_equal(r0["wave"], cols0["LAMBDA"]) and np.array_equal(r0["delta"], cols0["DELTA"])
+        assert np.array_equal(r0["ivar"], cols0["IVAR"]) and "rmat" not in r0
+        assert r0["R_fwhm"] == int(c / 13.4 / 2.35482004503 / 100 + 0.5) * 100
+        z, ra, dec, rk, snr, dv, dlam = r0["params"]
+        assert (z, ra, dec, rk, snr) == (2.75, 0.5, -0.25, 13.4, 3.5)
+        assert dv == 1e-4 * c * 2.30258509299
+        # dlambda absent: the median pixel step in wavelength
+        assert abs(dlam - np.median(np.diff(cols0["LAMBDA"]))) <= 1e-12 * dlam
+        r1, err = _read(tmp_path, "picca", path + "[2]", "r1.txt")
+        assert r1 is not None, err
+        cols1 = dict((k, v) for k, v, _ in s1[1])
+        assert r1["id"] == 91 and (r1["expid"], r1["night"], r1["fiber"], r1["petal"]) == (12, 20210105, 4, 7)
+        np.testing.assert_allclose(r1["wave"], 10.0 ** cols1["LOGLAM"], rtol=4e-16, atol=0)
+        assert np.array_equal(r1["delta"], cols1["DELTA_BLIND"])
+        assert np.array_equal(r1["ivar"], cols1["IVAR"].astype(np.float32).astype(float))
+        assert r1["params"][6] == 0.8
+        # dv absent: the median of c ln(lambda_i+1 / lambda_i)
+        w1 = r1["wave"]
+        assert abs(r1["params"][5] - np.median(c * np.log(w1[1:] / w1[:-1]))) < 1e-9
+        assert r1["rmat_nelem"] == 11
+        assert np.array_equal(r1["rmat"].reshape(11, 211), cols1["RESOMAT"].T)
+        bad, err = _read(tmp_path, "picca", path + "[3]", "r2.txt")
+        assert bad is None and "Wavelength must be in LOGLAM or LAMBDA!" in err
+        bad, err = _read(tmp_path, "picca", path + "[4]", "r3.txt")
+        assert bad is None and "TARGETID" in err and "THING_ID" in err
+        bad, err = _read(tmp_path, "picca", path + "[5]", "r4.txt")
+        assert bad is None and "not found" in err
+
+
+@pytest.mark.gpu
+def test_dropin_picca_files_with_resolution_matrices_against_oracle(tmp_path):
+    """Spectra in a picca delta file with an 11-diagonal RESOMAT column, read
+    with UseResoMatrix, ResoMatrixDeconvolutionM = 1 and OversampleRmat = 3
+    (core/one_qso_estimate.cpp:74-89): the drop-in reads, deconvolves and
+    oversamples each spectrum's matrix, cuts, chunks, iterates and flushes;
+    the oracle does the same to its own reading of the arrays.  FP64 bar,
+    with the floor of the reference's own rounding (the oracle against its
+    extended-precision run) for these matrices."""
+    from oracle.chunk import ResoMatrix
+    _need_exe()
+    print("[case] drop-in, picca files with RESOMAT, deconvolved and oversampled", flush=True)
+    w = synth.rmat_config(nchunks=8, n_range=(150, 320), kind="dia")
+    b = P.oracle_bins(w.binning)
+    T = b.total_kz_bins
+    c_kms = 299792.458
+    spectra = []
+    for i, c in enumerate(w.chunks):
+        hdr = [("TARGETID", 1000 + i), ("Z", 3.0), ("RA", 0.0), ("DEC", 0.0),
+               ("MEANRESO", c_kms / c["R"] / 2.35482004503), ("MEANSNR", 5.0)]
+        cols = [("LAMBDA", c["wave"], "D"), ("DELTA", c["delta"], "D"), ("IVAR", c["ivar"], "D"),
+                ("RESOMAT", np.asarray(c["rmat"]["values"]).T, "D")]
+        spectra.append((hdr, cols))
+    path = str(tmp_path / "deltas.fits.gz")
+    sqio.write_picca_deltas(path, spectra, compress=True)
+    files = [f"{path}[{i + 1}]" for i in range(len(spectra))]
+    tdir = tmp_path / "tables"
+    tdir.mkdir()
+    (R0, dv0), = w.pairs
+    t = sqio.SQTable(nvpoints=w.nv, nzpoints=w.nz_table, v_length=w.v_length, z1=0.0, z_length=0.0,
+                     spectrograph_resolution=R0, pixel_width=dv0, k1=0.0, dklin=0.0, dklog=0.0,
+                     nklin=w.binning.nk, nklog=0, deriv=w.deriv[0], signal=w.signal[0])
+    sqio.write_sq_table(str(tdir / f"sq_R{R0}_dv{dv0:.1f}.fits"), t)
+    tabs = P.oracle_tables(w)
+    Fo, do = np.zeros((T, T)), np.zeros((3, T))
+    Fx, dx = np.zeros((T, T)), np.zeros((3, T))
+    n_oracle = 0
+    for c in w.chunks:
+        wave = np.asarray(c["wave"], float)
+        delta, ivar, _ = oreader.mask_outliers(wave, c["delta"], c["ivar"])
+        rows = ResoMatrix("dia", np.asarray(c["rmat"]["values"]), 11).deconvolve(1.0).oversample(3)
+        cut = oreader.cut_boundary(b, wave, delta, ivar)
+        if cut is None:
+            continue
+        a, e = cut
+        if np.count_nonzero(ivar[a:e]) < oreader.MIN_PIXELS_IN_FILE:
+            continue
+        rm = rows.subset(a, e)
+        ix = decide_indices(wave[a:e], b)
+        for k in range(len(ix) - 1):
+            args = (b, tabs, wave[a:e], delta[a:e], ivar[a:e], c["R"], ix[k], ix[k + 1])
+            try:
+                ch = OracleChunk(*args, rmat=rm)
+            except ChunkError:
+                continue
+            assert ch.one_iteration(np.zeros(T), Fo, do)
+            OracleChunk(*args, rmat=rm, exact=True).one_iteration(np.zeros(T), Fx, dx)
+            n_oracle += 1
+    assert n_oracle >= 6
+    cfg = str(tmp_path / "run.cfg")
+    _write_config(cfg, w.binning, files)
+    with open(cfg, "a") as f:
+        f.write(f"format picca\nrmat 1\ndeconv 1.0\nosamp 3\nfirst_table {R0} {dv0!r}\n")
+    outp = str(tmp_path / "out.txt")
+    r = subprocess.run([DROPIN_CHECK, "run", cfg, str(tdir), "sq", "fp64", outp], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rec, nchunks = {}, 0
+    with open(outp) as f:
+        for ln in f:
+            t = ln.split()
+            if t[0] == "chunk":
+                nchunks += 1
+            elif t[0] not in ("cF", "cdbt", "collapsed"):
+                rec[t[0]] = np.array(t[2:], float)
+    assert nchunks == n_oracle
+    F = rec["fisher"].reshape(T, T)
+    dbt = np.array([rec["d"], rec["b"], rec["t"]])
+    floor = P.errors(Fo, do, Fx, dx)
+    P.assert_fp64(F, dbt, Fo, do, "drop-in, picca files with RESOMAT", floor=floor, exact=(Fx, dx))
+
+
+def _text_records(path):
+    recs = []
+    with open(path) as f:
+        for ln in f:
+            t = ln.split()
+            recs.append((t[0], t[1:]))
+    return recs
+
+
+def test_bootstrap_output_files_round_trip(tmp_path):
+    """BootstrapChunksFile::writeChunk and saveBootstrapRealizations
+    (io/bootstrap_file.cpp:17-185) from fixed arrays: three chunks of 4, 1 and
+    6 bands hold d, b, t and the upper diagonals of F with their TARGETID,
+    ZQSO, RA, DEC, NQDIM and ISTART; the realizations file holds the powers
+    (nboots x nk nz) and the solver matrix with NBOOTS, NK, NZ, FASTBOOT.  The
+    C++ reader (readBootstrapRealizations) and this package's readers must
+    return the written values bit for bit."""
+    _need_exe()
+    base = str(tmp_path / "run")
+    out = str(tmp_path / "bootio.txt")
+    r = subprocess.run([DROPIN_CHECK, "bootio", base, out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    recs = _text_records(out)
+    want, cur = [], None
+    for tag, v in recs:
+        if tag == "chunk":
+            cur = dict(dim=int(v[0]), istart=int(v[1]), id=int(v[2]))
+            want.append(cur)
+        elif tag in ("par", "p", "n", "t", "F", "powers", "invf"):
+            arr = np.array(v[1:], float)
+            if cur is not None and tag not in ("powers", "invf"):
+                cur[tag] = arr
+            else:
+                want.append((tag, arr))
+    chunks = [w_ for w_ in want if isinstance(w_, dict)]
+    got = sqio.read_boot_chunks(base + "-bootchunks-3.fits")
+    assert len(got) == len(chunks) == 3
+    for g, w_ in zip(got, chunks):
+        n = w_["dim"]
+        assert g.targetid == w_["id"] and g.istart == w_["istart"] and g.d.size == n
+        assert (g.z_qso, g.ra, g.dec) == tuple(w_["par"])
+        assert np.array_equal(g.d, w_["p"]) and np.array_equal(g.b, w_["n"]) and np.array_equal(g.t, w_["t"])
+        F = w_["F"].reshape(n, n)
+        iu = np.triu_indices(n)
+        assert np.array_equal(g.fisher[iu], F[iu]) and np.array_equal(g.fisher, g.fisher.T)
+    arrays = dict(x for x in want if not isinstance(x, dict))
+    p, inv, nb, nk, nz, fast = sqio.read_bootstrap_realizations(base + "-bootstrap-realizations.fits")
+    assert (nb, nk, nz, fast) == (5, 3, 2, True)
+    assert np.array_equal(p.reshape(-1), arrays["powers"]) and np.array_equal(inv.reshape(-1), arrays["invf"])

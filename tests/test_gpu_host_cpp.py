@@ -245,9 +245,7 @@ def test_cpp_resolution_matrix_spectra_preprocessed_like_the_reader(tmp_path):
     for (s, fis, nq, st, fc, d), (fr, dr, sr) in zip(chunks, res):
         assert st == sr == 0 and np.array_equal(fc, fr) and np.array_equal(d, dr)
     floor = P.errors(Fo, do, Fx, dx)
-    print(f"[fp64 parity] preprocessed matrices, C++ layer vs the oracle's extended-precision run: "
-          f"{P.fmt(P.errors(F, dbt, Fx, dx))}", flush=True)
-    P.assert_fp64(F, dbt, Fo, do, "preprocessed matrices, C++ layer", floor=floor)
+    P.assert_fp64(F, dbt, Fo, do, "preprocessed matrices, C++ layer", floor=floor, exact=(Fx, dx))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])

@@ -80,22 +80,25 @@ def fiducial_power(w):
 
 
 def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0,
-               nr_smooth=False, boots=0):
-    eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb)
+               nr_smooth=False, boots=0, stack_slab=0, stages=None):
+    eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb, stack_slab=stack_slab)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
     eng.commit()
     log(f"[{precision}] {len(accepted)} chunks registered and committed in {time.perf_counter() - t0:.1f} s")
     T = eng.T
     theta = np.zeros(T)
+    if stages and precision not in ("fp64", "fp32"):
+        eng.set_kernel_stages(*stages)
     schedule = dict(eng.schedule(), autotuned=False)
     if tune_caps:
         t0 = time.perf_counter()
         cap, tms = eng.autotune(theta, caps=tune_caps, reps=1)
         schedule = dict(eng.schedule(), autotuned=True, candidates=list(tune_caps), tuned_ms=tms,
                         tuning_s=time.perf_counter() - t0)
-        log(f"[{precision}] autotuned batch cap {cap}, LDS stages left {schedule['stages_left']} "
-            f"gram {schedule['stages_gram']} ({tms:.1f} ms) in {schedule['tuning_s']:.1f} s")
+        log(f"[{precision}] autotuned batch cap {cap}, operand staging left {schedule['stages_left']} "
+            f"right {schedule['stages_right']} gram {schedule['stages_gram']} ({tms:.1f} ms) "
+            f"in {schedule['tuning_s']:.1f} s")
     if graph:
         eng.set_graph(True)
     for i in range(warmup):
@@ -201,8 +204,8 @@ def measured_traffic(kernel: str, precision: str, workload: str):
 
 def measured_mfma_busy(kernel: str, precision: str):
     """MFMA-busy fraction of a dense product from the committed counter
-    summary (profiles/mfma_util_products_desi-256.json), if recorded."""
-    path = os.path.join(TRAFFIC_DIR, "mfma_util_products_desi-256.json")
+    summary (profiles/mfma_util_products_desi-4096.json), if recorded."""
+    path = os.path.join(TRAFFIC_DIR, "mfma_util_products_desi-4096.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -241,12 +244,27 @@ def _cpu_worker_job(args):
     return done
 
 
+# host cores a one-GPU run may use: the box allots 16 per GPU, whatever
+# nproc shows (its other cores belong to the other GPUs' jobs)
+CPU_SHARE = 16
+
+
+def _cores():
+    """(nproc, cores in this process's affinity mask, cores used)."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    return nproc, aff, max(1, min(CPU_SHARE, aff))
+
+
 def start_cpu_pool():
-    """Workers of the rank-emulation baseline (min(16, nproc) / 2 processes),
+    """Workers of the rank-emulation baseline (cores used / 2 processes),
     forked by a fork server that is started before the GPU is touched, so no
     worker comes from a process that has initialised the device."""
     import multiprocessing as mp
-    n = max(1, min(16, os.cpu_count() or 1) // 2)
+    n = max(1, _cores()[2] // 2)
     pool = mp.get_context("forkserver").Pool(n, initializer=_cpu_worker_init)
     pool.map(_noop, range(n))
     return pool, n
@@ -288,6 +306,7 @@ def cpu_baseline(w, budget_s=20.0, max_chunks=48, pool=None):
     th = np.zeros(T)
     done = 0
     used = []
+    phases = {}
     t0 = time.perf_counter()
     for c in w.chunks[:max_chunks]:
         used.append(c)
@@ -295,13 +314,20 @@ def cpu_baseline(w, budget_s=20.0, max_chunks=48, pool=None):
             ch = OChunk(b, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
         except ChunkError:
             continue
-        ch.one_iteration(th, F, d)
+        ch.one_iteration(th, F, d, times=phases)
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
+    # the driver's F inversion (T_Finv) once on the sample's sum
+    from oracle.driver import power_estimates
+    tf = time.perf_counter()
+    power_estimates(F.copy(), d)
+    phases["f_inv"] = time.perf_counter() - tf
     model, blas = _cpu_info()
-    modes = {"A_serial_threaded_blas": {"chunks_per_s": done / el, "threads": int(threads), "seconds": el}}
+    nproc, aff, _ = _cores()
+    modes = {"A_serial_threaded_blas": {"chunks_per_s": done / el, "threads": int(threads), "seconds": el,
+                                        "phase_seconds": {k: round(v, 3) for k, v in phases.items()}}}
     best, best_mode, cores = done / el, "A", int(threads)
     if pool is not None:
         p, n = pool
@@ -316,7 +342,10 @@ def cpu_baseline(w, budget_s=20.0, max_chunks=48, pool=None):
         if doneb / elb > best:
             best, best_mode, cores = doneb / elb, "B", 2 * n
     return dict(value=best, unit="chunks/s", cores=cores, kind="port", mode=best_mode, modes=modes,
-                cpu_model=model, blas=blas,
+                cpu_model=model, blas=blas, nproc=nproc, affinity_cores=aff, cpu_share=CPU_SHARE,
+                phases="mode A seconds per mytime category (core/global_numbers.cpp:476-502): set_qs = T_Q, "
+                       "set_sfid = T_Sfid, c_inv = T_Cinv, set_modqs = T_Qmod, set_fisher = T_F, "
+                       "f_inv = T_Finv; dbt_rest the b, t contractions",
                 sample=f"first {done} chunks of the same batch, FP64 LU oracle; mode A {el:.1f} s")
 
 
@@ -346,6 +375,13 @@ def main():
                          "(every chunk its own spectrum)")
     ap.add_argument("--save-sums", default="", metavar="NPZ",
                     help="write the FP64 and mixed Fisher and d, b, t sums to this .npz")
+    ap.add_argument("--stack-slab", type=int, default=-1, metavar="M",
+                    help="stream each chunk's stack in slabs of M members (0: whole stack; default 16 for "
+                         "the high-resolution workload, else 0)")
+    ap.add_argument("--stages", default="", metavar="L,R,G",
+                    help="operand staging of the split-BF16 left, right and Gram products (1, 2: through "
+                         "registers with one or two LDS stages; 3, 4: direct-to-LDS loads with two or three "
+                         "stages); default: the engine's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -368,8 +404,10 @@ def main():
         f"generated in {time.perf_counter() - t0:.1f} s")
 
     caps = tuple(int(x) for x in args.autotune.split(",") if x.strip())
+    slab = args.stack_slab if args.stack_slab >= 0 else (16 if args.config == "hires" else 0)
     opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor,
-               nr_smooth=args.nr_smooth, boots=args.bootstrap)
+               nr_smooth=args.nr_smooth, boots=args.bootstrap, stack_slab=slab,
+               stages=tuple(int(x) for x in args.stages.split(",")) if args.stages else None)
     res64 = None
     if not args.no_fp64:
         res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)

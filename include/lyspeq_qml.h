@@ -370,9 +370,9 @@ int qml_poisson_draws(unsigned long long seed, long long n, int *out);
  * size-bucketed batches (sorted by padded size, cut at the cap or at the
  * scratch budget of qml_opts), and reps + 1 iterations are run at theta, the
  * first as warm-up; the plan with the shortest iteration is kept.  In the
- * split-BF16 precisions the kernel shapes are tuned next, on that plan: the
- * LDS stages (1 or 2) of the left whitening product and of the Gram, every
- * combination timed the same way (qml_set_kernel_stages).  The chosen cap and
+ * split-BF16 precisions the staging of the three products' operands is tuned
+ * next, on that plan, one product at a time (left, right, Gram;
+ * qml_set_kernel_stages), each choice timed the same way.  The chosen cap and
  * the best time (ms) are returned; qml_get_schedule reports the whole choice.
  * The per-chunk results do not depend on any of these choices.  The
  * accumulated sums of these iterations are discarded.  No counterpart in the
@@ -380,20 +380,42 @@ int qml_poisson_draws(unsigned long long seed, long long n, int *out);
 int qml_autotune(qml_ctx *ctx, const double *theta, const int *caps, int ncaps, int reps, int *best_cap,
                  double *best_ms);
 
-/* Kernel shapes of the split-BF16 products: LDS stages of the left
- * whitening product and of the Gram, 1 (the next stage waits in registers,
- * three workgroups per CU) or 2 (two workgroups per CU).  Defaults 1 and 2.
- * Results are identical for every choice.  No counterpart in the reference. */
-int qml_set_kernel_stages(qml_ctx *ctx, int stages_left, int stages_gram);
+/* Staging of the split-BF16 products' operands into LDS, for the left and
+ * right whitening products and the Gram:
+ *   1  through registers, one LDS stage (the next stage waits in registers;
+ *      three workgroups per CU);
+ *   2  through registers, two LDS stages (two workgroups per CU); not for
+ *      the right product;
+ *   3  direct-to-LDS loads (global_load_lds), two stages (two per CU);
+ *   4  direct-to-LDS loads, three stages (one per CU).
+ * Defaults 1, 3 and 3.  Results are identical for every choice.  No
+ * counterpart in the reference. */
+int qml_set_kernel_stages(qml_ctx *ctx, int left, int right, int gram);
+
+/* Streams each chunk's stack of weighted matrices in slabs of `members`
+ * (0, the default: the whole stack at once).  The stack and its left
+ * products then take members x Np^2 elements each per chunk instead of
+ * (n + 2) x Np^2; each slab's whitened matrices go to a store of packed upper
+ * tiles, and one Gram runs over the store.  A chunk's device scratch is
+ * bounded by the slab instead of growing with the stored Q count (N ~ 2000
+ * with ~150 stored Q: about 4 GB in FP64 instead of 10.6).  The results are
+ * identical.  Set before registering chunks (their memory check uses it) or
+ * any time after qml_commit (the batches are planned again).  The reference's
+ * counterpart is its bounded number of stored Q matrices
+ * (Chunk::_setStoredMatrices, core/chunk_estimate.cpp:258-282). */
+int qml_set_stack_slab(qml_ctx *ctx, int members);
 
 /* The current launch schedule. */
 typedef struct {
     int max_batch_chunks;  /* cap on chunks per batch of the current plan */
     int n_batches;         /* batches of the current plan */
-    int stages_left;       /* LDS stages of the split-BF16 left product */
-    int stages_gram;       /* LDS stages of the split-BF16 Gram */
+    int stages_left;       /* operand staging of the split-BF16 left product (qml_set_kernel_stages) */
+    int stages_gram;       /* ... of the split-BF16 Gram */
     int pipeline;          /* two-stream stage overlap (qml_set_pipeline) */
     int graph;             /* graph replay (qml_set_graph) */
+    int stack_slab;        /* members per stack slab, 0 for the whole stack (qml_set_stack_slab) */
+    double max_chunk_gb;   /* the largest chunk's device scratch in the current plan, GiB */
+    int stages_right;      /* operand staging of the split-BF16 right product */
 } qml_schedule;
 int qml_get_schedule(qml_ctx *ctx, qml_schedule *schedule);
 

@@ -549,7 +549,8 @@ template <> struct XStore4<__bf16> {
 template <class OutT, class CT, int NUM, int OCC = 1>
 __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const double *in,
                                               const TableDev *tabs, BinsDev bins,
-                                              const double *mat, const double *aux, OutT *X) {
+                                              const double *mat, const double *aux, OutT *X,
+                                              int jlo, int jhi) {
     constexpr int EPT = (sizeof(CT) == 4) ? 2 : 1;               // row passes of 16
     constexpr int TR = 16 * EPT, TC = 64;
     constexpr int SL = (sizeof(CT) == 4) ? 2048 : 1024;        // slice buffer
@@ -688,16 +689,24 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
     // rows without guards (their factors U are zero for x >= nu)
     for (int f = tid; f < 2 * DP_ZB * MAX_NU * (TC + TR); f += 256) (&atv[0][0])[f] = (CT)0;
     __syncthreads();
+    // the bands with a member in the slab [jlo, jhi): those of one z bin's
+    // run of members, or every band when the slab spans two z bins
+    int knb = 0, kne = kbu;
     if (kbu > 0) {
-        fetch(0);
-        put(0, 0);
-        if (kbu > 1) fetch(1);
+        const int ma = max(jlo, 0) / kbu, mb = (min(jhi, c.n) - 1) / kbu;
+        if (jlo >= c.n || jhi <= 0) kne = 0;
+        else if (ma == mb) { knb = max(jlo, 0) - ma * kbu; kne = min(jhi, c.n) - ma * kbu; }
+    }
+    if (knb < kne) {
+        fetch(knb);
+        put(knb, 0);
+        if (knb + 1 < kne) fetch(knb + 1);
     }
     __syncthreads();
-    for (int kn = 0; kn < kbu; ++kn) {
-        const int cur = kn & 1;
-        if (kn + 1 < kbu) put(kn + 1, cur ^ 1);
-        if (kn + 2 < kbu) fetch(kn + 2);
+    for (int kn = knb; kn < kne; ++kn) {
+        const int cur = (kn - knb) & 1;
+        if (kn + 1 < kne) put(kn + 1, cur ^ 1);
+        if (kn + 2 < kne) fetch(kn + 2);
         // the band's values at the tile's elements; exact zeros outside the
         // chunk (a tile with no element inside stages no slice), so that
         // every member term below is finite and zero there
@@ -738,6 +747,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
 #pragma unroll
                 for (int u = 0; u < 4; ++u) ac[u][x] = ab[x * (TC + TR) + 4 * cg + u];
             const int j = m * kbu + kn;
+            if (j < jlo || j >= jhi) continue;
 #pragma unroll
             for (int e = 0; e < EPT; ++e) {
                 CT ar[NUM > 0 ? NUM : 1];
@@ -754,7 +764,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
                     // outside the chunk q, U and D are zero: an exact zero
                     val[u] = sc[e][u] * acc;
                 }
-                XStore4<OutT>::put(Xc, msz, j, (size_t)(i0 + rl + 16 * e) * Np + bb, val);
+                XStore4<OutT>::put(Xc, msz, j - jlo, (size_t)(i0 + rl + 16 * e) * Np + bb, val);
             }
         }
         __syncthreads();
@@ -771,6 +781,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int j = c.n + h;
+        if (j < jlo || j >= jhi) continue;
         const CT *ab = &atv[0][h * nu * (TC + TR)];
         CT ac[4][NUM > 0 ? NUM : 1];
 #pragma unroll
@@ -796,7 +807,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
                 for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
                 val[u] = sc[e][u] * acc;
             }
-            XStore4<OutT>::put(Xc, msz, j, (size_t)i * Np + bb, val);
+            XStore4<OutT>::put(Xc, msz, j - jlo, (size_t)i * Np + bb, val);
         }
     }
 }
@@ -953,11 +964,14 @@ static void launch_xgen_t(const LaunchCtx &L, OutT *X) {
     // (a few registers spill; measured faster than two waves without)
     constexpr int OCC = (sizeof(CT) == 4) ? 3 : 1;
     if (L.nu_max <= 0)
-        k_xgen<OutT, CT, 0, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+        k_xgen<OutT, CT, 0, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X,
+                                                             L.jlo, L.jhi);
     else if (L.nu_max <= 2)
-        k_xgen<OutT, CT, 2, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+        k_xgen<OutT, CT, 2, OCC><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X,
+                                                             L.jlo, L.jhi);
     else
-        k_xgen<OutT, CT, MAX_NU><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X);
+        k_xgen<OutT, CT, MAX_NU><<<grid, 256, 0, L.stream>>>(L.chunks, L.in, L.tables, L.bins, L.mat, L.pj, X,
+                                                             L.jlo, L.jhi);
 }
 void launch_xgen(const LaunchCtx &L, void *X, int prec) {
     if (prec == 0)

@@ -146,7 +146,7 @@ __device__ __forceinline__ void mainloop_k4(const Prob<T> &p, int I0, int J0,
 // ------------------------------------------------------------------ kernel --
 template <int PREC, int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *work, int nwork,
-                                              const void *Abase, const void *Bbase, void *Cbase) {
+                                              const void *Abase, const void *Bbase, void *Cbase, int jlo) {
     using T = typename std::conditional<PREC == 0, double, float>::type;
     using acc_t = typename std::conditional<PREC == 0, d4, f4>::type;
     const int li = xcd_logical(nwork);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     if (MODE == MODE_LEFT) {
         p.A = (const T *)Abase + c.lic_off;
         // X_j is symmetric, so its rows serve as the K-contiguous rows of B^T
-        p.B = (const T *)Bbase + c.stk_off + wk.j * msz;
+        p.B = (const T *)Bbase + c.stk_off + (wk.j - jlo) * msz;
         p.lda = p.ldb = Np;
         p.kbeg = 0;
         p.kend = (wk.I + 1) * TILE;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
         I0 = wk.I * TILE;
         J0 = wk.J * TILE;
     } else if (MODE == MODE_RIGHT) {
-        p.A = (const T *)Abase + c.stk_off + wk.j * msz;
+        p.A = (const T *)Abase + c.stk_off + (wk.j - jlo) * msz;
         p.B = (const T *)Bbase + c.lic_off;
         p.lda = p.ldb = Np;
         p.kbeg = 0;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
         J0 = wk.J * TILE;
     } else {
         const int ktot = c.ntu * TILE * TILE;
-        p.A = (const T *)Abase + c.stk_off;
+        p.A = (const T *)Abase + c.g_off;
         p.B = p.A;
         p.lda = p.ldb = ktot;
         p.kbeg = wk.j * TILE * TILE;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
     // holds rows u*4 + lane/16, the FP32-output MFMAs rows 4*(lane/16) + u
     auto brow = [&](int u) { return (PREC == 0) ? (4 * u + lane / 16) : (4 * (lane / 16) + u); };
     if (MODE == MODE_LEFT) {
-        T *C = (T *)Cbase + c.stk_off + wk.j * msz;
+        T *C = (T *)Cbase + c.stk_off + (wk.j - jlo) * msz;
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
                     C[(size_t)r * Np + q] = (T)acc[x][y][u];
                 }
     } else if (MODE == MODE_RIGHT) {
-        T *C = (T *)Cbase + c.stk_off + (size_t)wk.j * c.ntu * TILE * TILE
+        T *C = (T *)Cbase + c.g_off + (size_t)wk.j * c.ntu * TILE * TILE
              + (size_t)upper_tile_index(wk.I, wk.J, c.nt) * TILE * TILE;
 #pragma unroll
         for (int x = 0; x < 2; ++x)
@@ -248,34 +248,34 @@ __global__ __launch_bounds__(256) void k_gemm(const ChunkDev *cs, const Work *wo
 }
 
 void launch_bf3(hipStream_t st, int mode, int nprod, int nst, const ChunkDev *chunks, const Work *work, int nwork,
-                const void *A, const void *B, void *C);
+                const void *A, const void *B, void *C, int jlo);
 
 template <int MODE>
 static void launch_mode(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
-                        const void *A, const void *B, void *C) {
+                        const void *A, const void *B, void *C, int jlo) {
     if (nwork <= 0) return;
     if (prec >= 2) {   // split BF16: three products, or four with lo * lo
-        launch_bf3(st, MODE, prec == 3 ? 4 : 3, nst, chunks, work, nwork, A, B, C);
+        launch_bf3(st, MODE, prec == 3 ? 4 : 3, nst, chunks, work, nwork, A, B, C, jlo);
         return;
     }
     const int grid = 8 * ((nwork + 7) / 8);
     if (prec == 0)
-        k_gemm<0, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+        k_gemm<0, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C, jlo);
     else
-        k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C);
+        k_gemm<1, MODE><<<grid, 256, 0, st>>>(chunks, work, nwork, A, B, C, jlo);
 }
 
 void launch_gemm_whiten_left(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
-                             int nwork, const void *li, const void *X, void *Y) {
-    launch_mode<MODE_LEFT>(st, prec, nst, chunks, work, nwork, li, X, Y);
+                             int nwork, const void *li, const void *X, void *Y, int jlo) {
+    launch_mode<MODE_LEFT>(st, prec, nst, chunks, work, nwork, li, X, Y, jlo);
 }
-void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
-                              const void *li, const void *Y, void *G) {
-    launch_mode<MODE_RIGHT>(st, prec, 1, chunks, work, nwork, Y, li, G);
+void launch_gemm_whiten_right(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
+                              int nwork, const void *li, const void *Y, void *G, int jlo) {
+    launch_mode<MODE_RIGHT>(st, prec, nst, chunks, work, nwork, Y, li, G, jlo);
 }
 void launch_gemm_gram(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram) {
-    launch_mode<MODE_GRAM>(st, prec, nst, chunks, work, nwork, G, G, gram);
+    launch_mode<MODE_GRAM>(st, prec, nst, chunks, work, nwork, G, G, gram, 0);
 }
 
 }  // namespace qml
@@ -316,68 +316,182 @@ constexpr int BK = 32;
 // offset x * 16 * BK from block 0
 __device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ (((r >> 1) ^ ((r >> 3) & 1)) & 3)) << 3); }
 
-template <int MODE> struct Bf3Shape;
-template <> struct Bf3Shape<MODE_LEFT> { static constexpr int BM = 128, BN = 128; };
-template <> struct Bf3Shape<MODE_RIGHT> { static constexpr int BM = 128, BN = 128; };
-template <> struct Bf3Shape<MODE_GRAM> { static constexpr int BM = 128, BN = 128; };
+// 128x128 output tiles of four waves (2 x 2), each a 64x64 quarter
+constexpr int BF3_BM = 128, BF3_BN = 128;
+constexpr int BF3_PL = BF3_BM * BK;   // elements per LDS plane
+constexpr int BF3_STG = 4 * BF3_PL;   // elements per stage: A hi, A lo, B hi, B lo
+constexpr int BF3_EP = 72;            // row pitch of the staged epilogue block, in bf16
+
+// Operands and K range of one work item (tile I, J of member or split j).
+struct Bf3Item {
+    const __bf16 *Ahi, *Bhi;   // hi planes; the lo planes follow at alo / blo elements
+    long long lda, ldb, alo, blo;
+    int arows, brows, kbeg, kend, Mv, Nv;
+};
+
+template <int MODE>
+__device__ __forceinline__ Bf3Item bf3_item(const ChunkDev &c, const Work &wk, int jlo, const __bf16 *Abase,
+                                            const __bf16 *Bbase) {
+    Bf3Item t;
+    const int Np = c.Np;
+    const long long msz = (long long)Np * Np;
+    const int I0 = wk.I * BF3_BM, J0 = wk.J * BF3_BN;
+    t.kbeg = 0;
+    if (MODE == MODE_LEFT) {
+        // P = Li M, lower tiles (J <= I): Li[i,k] needs k <= i and the stored
+        // T = M^T (upper triangle of X_j, diagonal halved) needs k >= n
+        t.Ahi = Abase + c.lic_off;
+        t.Bhi = Bbase + c.stk_off + 2 * (wk.j - jlo) * msz;
+        t.lda = t.ldb = Np;
+        t.alo = t.blo = msz;
+        t.arows = t.brows = Np;
+        t.kbeg = J0;
+        t.kend = min(Np, I0 + BF3_BM);
+        t.Mv = t.Nv = Np;
+    } else if (MODE == MODE_RIGHT) {
+        // G = P Li^T + Li P^T, upper tiles (I <= J): both terms need
+        // k <= min(i, n) <= i, so k < I0 + BM; the second term swaps the
+        // operand bases (same row offsets)
+        t.Ahi = Abase + c.stk_off + 2 * (wk.j - jlo) * msz;
+        t.Bhi = Bbase + c.lic_off;
+        t.lda = t.ldb = Np;
+        t.alo = t.blo = msz;
+        t.arows = t.brows = Np;
+        t.kend = min(Np, I0 + BF3_BM);
+        t.Mv = t.Nv = Np;
+    } else {
+        const long long ktot = (long long)c.ntu * TILE * TILE;
+        t.Ahi = t.Bhi = Abase + c.g_off;
+        t.lda = t.ldb = 2 * ktot;
+        t.alo = t.blo = ktot;
+        t.arows = t.brows = c.m;
+        t.kbeg = wk.j * TILE * TILE;
+        t.kend = t.kbeg + TILE * TILE;
+        t.Mv = t.Nv = c.m;
+    }
+    return t;
+}
+
+// One BK step of a wave's 64x64 quarter out of LDS stage s0: fragment row
+// offsets aoff / boff (rows 16x + lane%16 of the wave's slab, chunk lane/16).
+template <int NPROD>
+__device__ __forceinline__ void bf3_mfma(const __bf16 *s0, int aoff, int boff, f4 (&acc)[4][4]) {
+    bf8 bh[4], bl[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        bh[y] = *(const bf8 *)(s0 + 2 * BF3_PL + boff + y * 16 * BK);
+        bl[y] = *(const bf8 *)(s0 + 3 * BF3_PL + boff + y * 16 * BK);
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        const bf8 ah = *(const bf8 *)(s0 + aoff + x * 16 * BK);
+        const bf8 al = *(const bf8 *)(s0 + BF3_PL + aoff + x * 16 * BK);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            if constexpr (NPROD == 4)
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], al, acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al, acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah, acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah, acc[x][y], 0, 0, 0);
+        }
+    }
+}
+
+// Output of a wave's 64x64 quarter at (r0, c0); ep is the wave's own LDS
+// region of 64 rows x BF3_EP (the stage buffers are free: the caller passed
+// a barrier after the last step).
+//   Gram: FP64 partials, x2 for off-diagonal G tiles, into the split's block.
+//   LEFT / RIGHT: the FP32 block split into bf16 hi and lo planes, written
+//   through LDS: the wave stages one plane (64 rows of 128 bytes at a row
+//   pitch of 144 bytes, against bank conflicts) in its region, then each
+//   store instruction writes eight full 128-byte rows, 16 bytes per lane.
+//   One wave's LDS operations complete in order, so no barrier is needed.
+template <int MODE>
+__device__ __forceinline__ void bf3_epilogue(f4 (&acc)[4][4], const ChunkDev &c, const Work &wk, int jlo, int r0,
+                                             int c0, void *Cbase, __bf16 *ep, int lane) {
+    // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
+    const int orow = lane % 16, ocol = 4 * (lane / 16);
+    if (MODE == MODE_GRAM) {
+        const int ntile = c.mt * (c.mt + 1) / 2;
+        const double wgt = packed_tile_is_diag(wk.j, c.nt) ? 1.0 : 2.0;
+        double *C = (double *)Cbase + c.gram_off
+                  + ((size_t)wk.j * ntile + upper_tile_index(r0 / TILE, c0 / TILE, c.mt)) * TILE * TILE;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                d4 v;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = wgt * (double)acc[x][y][u];
+                double *o = C + (16 * x + orow) * TILE + 16 * y + ocol;
+                *(double2 *)o = double2{v[0], v[1]};
+                *(double2 *)(o + 2) = double2{v[2], v[3]};
+            }
+        return;
+    }
+    const int Np = c.Np;
+    const long long msz = (long long)Np * Np;
+    __bf16 *C = (__bf16 *)Cbase;
+    long long base, ldc, lo;
+    if (MODE == MODE_LEFT) {
+        base = c.stk_off + 2 * (wk.j - jlo) * msz + (long long)r0 * Np + c0;
+        ldc = Np;
+        lo = msz;
+    } else {
+        const long long ktot = (long long)c.ntu * TILE * TILE;
+        base = c.g_off + 2 * wk.j * ktot + (long long)upper_tile_index(r0 / TILE, c0 / TILE, c.nt) * TILE * TILE;
+        ldc = TILE;
+        lo = ktot;
+    }
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    // hi plane first; acc then keeps the residual v - hi for the lo plane
+#pragma unroll 1
+    for (int plane = 0; plane < 2; ++plane) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                bf4 h;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    h[u] = (__bf16)acc[x][y][u];
+                    acc[x][y][u] -= (float)h[u];
+                }
+                *(bf4 *)(ep + (16 * x + orow) * BF3_EP + 16 * y + ocol) = h;
+            }
+        __bf16 *Cp = C + base + plane * lo;
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8) {
+            const int r = 8 * s8 + lane / 8, ch = lane % 8;
+            typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(*(const u4 *)(ep + r * BF3_EP + 8 * ch),
+                                        (u4 *)(Cp + (long long)r * ldc + 8 * ch));
+        }
+    }
+}
 
 template <int MODE, int NST, int NPROD>
-__global__ __launch_bounds__(64 * (Bf3Shape<MODE>::BM / 64) * (Bf3Shape<MODE>::BN / 64), NST == 1 ? 3 : 2)
+__global__ __launch_bounds__(256, NST == 1 ? 3 : 2)
 void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *Abase, const __bf16 *Bbase,
-                void *Cbase) {
-    constexpr int BM = Bf3Shape<MODE>::BM, BN = Bf3Shape<MODE>::BN;
-    constexpr int WN = BN / 64;                 // waves along N
-    constexpr int NTH = 64 * (BM / 64) * WN;    // threads
-    constexpr int RP = NTH / 4;                 // rows staged per pass
+                void *Cbase, int jlo) {
+    constexpr int BM = BF3_BM, BN = BF3_BN;
+    constexpr int RP = 256 / 4;                 // rows staged per pass
     constexpr int PA = BM / RP, PB = BN / RP;   // staging passes per plane
-    constexpr int PLA = BM * BK, PLB = BN * BK; // elements per LDS plane
-    constexpr int STG = 2 * (PLA + PLB);        // elements per stage
+    constexpr int PLA = BF3_PL, PLB = BF3_PL;   // elements per LDS plane
+    constexpr int STG = BF3_STG;                // elements per stage
     const int li = xcd_logical(nwork);
     if (li >= nwork) return;
     const Work wk = work[li];
     const ChunkDev c = cs[wk.c];
-    const int Np = c.Np;
-    const long long msz = (long long)Np * Np;
     const int I0 = wk.I * BM, J0 = wk.J * BN;
-    const __bf16 *Ahi, *Bhi;
-    long long lda, ldb, alo, blo;
-    int arows, brows, kbeg = 0, kend, Mv, Nv;
-    if (MODE == MODE_LEFT) {
-        // P = Li M, lower tiles (J <= I): Li[i,k] needs k <= i and the stored
-        // T = M^T (upper triangle of X_j, diagonal halved) needs k >= n
-        Ahi = Abase + c.lic_off;
-        Bhi = Bbase + c.stk_off + 2 * wk.j * msz;
-        lda = ldb = Np;
-        alo = blo = msz;
-        arows = brows = Np;
-        kbeg = J0;
-        kend = min(Np, I0 + BM);
-        Mv = Nv = Np;
-    } else if (MODE == MODE_RIGHT) {
-        // G = P Li^T + Li P^T, upper tiles (I <= J): both terms need
-        // k <= min(i, n) <= i, so k < I0 + BM; first term here, second below
-        Ahi = Abase + c.stk_off + 2 * wk.j * msz;
-        Bhi = Bbase + c.lic_off;
-        lda = ldb = Np;
-        alo = blo = msz;
-        arows = brows = Np;
-        kend = min(Np, I0 + BM);
-        Mv = Nv = Np;
-    } else {
-        const long long ktot = (long long)c.ntu * TILE * TILE;
-        Ahi = Bhi = Abase + c.stk_off;
-        lda = ldb = 2 * ktot;
-        alo = blo = ktot;
-        arows = brows = c.m;
-        kbeg = wk.j * TILE * TILE;
-        kend = kbeg + TILE * TILE;
-        Mv = Nv = c.m;
-    }
-    constexpr int EP = 72;   // row pitch of the staged epilogue block, in bf16
-    constexpr int SMEM = (NST * STG > 4 * 64 * EP) ? NST * STG : 4 * 64 * EP;
+    const Bf3Item it = bf3_item<MODE>(c, wk, jlo, Abase, Bbase);
+    const __bf16 *Ahi = it.Ahi, *Bhi = it.Bhi;
+    const long long lda = it.lda, ldb = it.ldb, alo = it.alo, blo = it.blo;
+    const int arows = it.arows, brows = it.brows, kbeg = it.kbeg, kend = it.kend, Mv = it.Mv, Nv = it.Nv;
+    constexpr int SMEM = (NST * STG > 4 * 64 * BF3_EP) ? NST * STG : 4 * 64 * BF3_EP;
     __shared__ __attribute__((aligned(16))) __bf16 smem[SMEM];
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int wm = w / WN, wn = w % WN;
+    const int wm = w / 2, wn = w % 2;
     const int r0 = I0 + 64 * wm, c0 = J0 + 64 * wn;
     bool active = (r0 < Mv) && (c0 < Nv);
     if (MODE != MODE_LEFT) active = active && (c0 >= r0);
@@ -442,26 +556,7 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
             *(uint4 *)(s0 + 2 * PLA + PLB + wb1) = R##rbl1;                      \
         }                                                                        \
     }
-#define QML_BF3_MFMA(st)                                                                              \
-    {                                                                                                 \
-        const __bf16 *s0 = smem + (st) * STG;                                                         \
-        bf8 bh[4], bl[4];                                                                             \
-        _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                               \
-            bh[y] = *(const bf8 *)(s0 + 2 * PLA + boff + y * 16 * BK);                                \
-            bl[y] = *(const bf8 *)(s0 + 2 * PLA + PLB + boff + y * 16 * BK);                          \
-        }                                                                                             \
-        _Pragma("unroll") for (int x = 0; x < 4; ++x) {                                               \
-            const bf8 ah = *(const bf8 *)(s0 + aoff + x * 16 * BK);                                   \
-            const bf8 al = *(const bf8 *)(s0 + PLA + aoff + x * 16 * BK);                             \
-            _Pragma("unroll") for (int y = 0; y < 4; ++y) {                                           \
-                if constexpr (NPROD == 4)                                                             \
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], al, acc[x][y], 0, 0, 0); \
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], al, acc[x][y], 0, 0, 0);   \
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[y], ah, acc[x][y], 0, 0, 0);   \
-                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[y], ah, acc[x][y], 0, 0, 0);   \
-            }                                                                                         \
-        }                                                                                             \
-    }
+#define QML_BF3_MFMA(st) bf3_mfma<NPROD>(smem + (st) * STG, aoff, boff, acc);
 // One K sweep over n = (kend - kbeg)/BK stages, two steps per pass: the
 // loads of stage s+2 are issued (into the register set just stored), stage
 // s is multiplied out of LDS, and stage s+1 (loaded a whole step earlier)
@@ -561,104 +656,153 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 #undef QML_BF3_SWEEP1_2T
 #undef QML_BF3_GLOADX
     if (!active) return;
-    // lane holds C[16x + lane%16][16y + 4(lane/16) + u], u = 0..3
-    const int orow = lane % 16, ocol = 4 * (lane / 16);
-    if (MODE == MODE_GRAM) {
-        const int ntile = c.mt * (c.mt + 1) / 2;
-        const double wgt = packed_tile_is_diag(wk.j, c.nt) ? 1.0 : 2.0;
-        double *C = (double *)Cbase + c.gram_off
-                  + ((size_t)wk.j * ntile + upper_tile_index(r0 / TILE, c0 / TILE, c.mt)) * TILE * TILE;
+    bf3_epilogue<MODE>(acc, c, wk, jlo, r0, c0, Cbase, smem + w * 64 * BF3_EP, lane);
+}
+
+// ------------------------------------------ split BF16, direct-to-LDS loads --
+// The same products, tiles, LDS layout and MFMA order as k_gemm_bf3 (so the
+// same results, bit for bit), with the operands staged by
+// global_load_lds_dwordx4: each load moves 16 bytes per lane from global
+// memory straight into LDS at (M0 base) + 16 x lane, with no register
+// round trip, so NS stages can be in flight at once without registers held
+// for them.  Wave w fills plane w of a stage (A hi, A lo, B hi, B lo), eight
+// 16-row blocks of 1 KB; lane i of a block loads row i/4 of the block at the
+// chunk that the swizzle places in slot i, (i%4) ^ (((r >> 1) ^ ((r >> 3) &
+// 1)) & 3) with r = i/4.  Rows past the operand read row 0, as in k_gemm_bf3.
+//
+// Step s waits for its own stage (s_waitcnt vmcnt counting the stages issued
+// after it), passes one barrier, issues stage s + NS - 1 into the buffer
+// that step s - 1 read (every wave is past that step), then multiplies
+// stage s.  The barrier is the bare s_barrier (no memory fence, which would
+// wait for every load in flight), with compiler-level memory clobbers so
+// that no LDS access moves across it.  NS = 2: two workgroups per CU (64 KB
+// of LDS each); NS = 3: one (96 KB).
+template <int NS>
+__device__ __forceinline__ void bf3_wait_stage(bool last) {
+    // s_waitcnt: vmcnt <= 8 (NS - 2) (the later stages' loads, eight per
+    // wave each), or 0 on the last step; lgkmcnt 0; expcnt not waited on.
+    // Encoding: vmcnt[3:0] and [15:14], expcnt[6:4], lgkmcnt[11:8].
+    constexpr int V = 8 * (NS - 2);
+    constexpr int ENC = (V & 15) | (7 << 4) | ((V >> 4) << 14);
+    if (NS == 2 || last) __builtin_amdgcn_s_waitcnt(7 << 4);
+    else __builtin_amdgcn_s_waitcnt(ENC);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int MODE, int NS, int NPROD>
+__global__ __launch_bounds__(256, NS == 2 ? 2 : 1)
+void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *Abase, const __bf16 *Bbase,
+                 void *Cbase, int jlo) {
+    static_assert(NS == 2 || NS == 3, "two or three stages");
+    const int li = xcd_logical(nwork);
+    if (li >= nwork) return;
+    const Work wk = work[li];
+    const ChunkDev c = cs[wk.c];
+    const int I0 = wk.I * BF3_BM, J0 = wk.J * BF3_BN;
+    const Bf3Item it = bf3_item<MODE>(c, wk, jlo, Abase, Bbase);
+    constexpr int SMEM = (NS * BF3_STG > 4 * 64 * BF3_EP) ? NS * BF3_STG : 4 * 64 * BF3_EP;
+    __shared__ __attribute__((aligned(1024))) __bf16 smem[SMEM];
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int wm = w / 2, wn = w % 2;
+    const int r0 = I0 + 64 * wm, c0 = J0 + 64 * wn;
+    bool active = (r0 < it.Mv) && (c0 < it.Nv);
+    if (MODE != MODE_LEFT) active = active && (c0 >= r0);
+    f4 acc[4][4];
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                d4 v;
+        for (int y = 0; y < 4; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+    // this wave's plane: A (w < 2) or B rows, hi (w even) or lo
+    const bool pa = w < 2;
+    const int prow0 = pa ? I0 : J0, prows = pa ? it.arows : it.brows;
+    const long long pld = pa ? it.lda : it.ldb, plo = (w & 1) ? (pa ? it.alo : it.blo) : 0;
+    using off_t = typename std::conditional<MODE == MODE_GRAM, long long, int>::type;
+    const int rl = lane / 4;
+    const int ch = (lane % 4) ^ (((rl >> 1) ^ ((rl >> 3) & 1)) & 3);
+    off_t roff[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = wgt * (double)acc[x][y][u];
-                double *o = C + (16 * x + orow) * TILE + 16 * y + ocol;
-                *(double2 *)o = double2{v[0], v[1]};
-                *(double2 *)(o + 2) = double2{v[2], v[3]};
-            }
-        return;
+    for (int b = 0; b < 8; ++b) {
+        const int r = prow0 + 16 * b + rl;
+        roff[b] = (off_t)(r < prows ? r : 0) * (off_t)pld + (off_t)plo + 8 * ch;
     }
-    // Split the 64x64 FP32 block into bf16 hi/lo planes and write them out
-    // through LDS: the wave stages one plane (64 rows of 128 bytes at a row
-    // pitch of 144 bytes, against bank conflicts) in its own LDS region, then
-    // each store instruction writes eight full 128-byte rows, 16 bytes per
-    // lane.  The main loop ended on a barrier, so the stage buffers are free;
-    // the region is private to the wave and one wave's LDS operations
-    // complete in order, so no further barrier is needed.
-    __bf16 *C = (__bf16 *)Cbase;
-    long long base, ldc, lo;
-    if (MODE == MODE_LEFT) {
-        base = c.stk_off + 2 * wk.j * msz + (long long)r0 * Np + c0;
-        ldc = Np;
-        lo = msz;
-    } else {
-        const long long ktot = (long long)c.ntu * TILE * TILE;
-        base = c.stk_off + 2 * wk.j * ktot + (long long)upper_tile_index(r0 / TILE, c0 / TILE, c.nt) * TILE * TILE;
-        ldc = TILE;
-        lo = ktot;
-    }
-    __bf16 *ep = smem + w * 64 * EP;
-    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-    // hi plane first; acc then keeps the residual v - hi for the lo plane
-#pragma unroll 1
-    for (int plane = 0; plane < 2; ++plane) {
+    const int kbeg = it.kbeg;
+    const int ns = (it.kend - kbeg) / BK;
+    const int total = (MODE == MODE_RIGHT) ? 2 * ns : ns;
+    // stage s: operands (the right product's second term swaps them) and k
+    const auto issue = [&](int s) {
+        const bool second = MODE == MODE_RIGHT && s >= ns;
+        const __bf16 *x = (pa != second) ? it.Ahi : it.Bhi;
+        const int k = kbeg + (second ? s - ns : s) * BK;
+        __bf16 *dst = smem + (s % NS) * BF3_STG + w * BF3_PL;
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int b = 0; b < 8; ++b)
+            __builtin_amdgcn_global_load_lds((const void *)(x + roff[b] + k),
+                                             (__attribute__((address_space(3))) void *)(dst + b * 16 * BK), 16, 0, 0);
+    };
+    const int rr = lane % 16, cq = lane / 16;
+    const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
+    if (total > 0) {
 #pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                bf4 h;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    h[u] = (__bf16)acc[x][y][u];
-                    acc[x][y][u] -= (float)h[u];
-                }
-                *(bf4 *)(ep + (16 * x + orow) * EP + 16 * y + ocol) = h;
-            }
-        __bf16 *Cp = C + base + plane * lo;
-#pragma unroll
-        for (int s8 = 0; s8 < 8; ++s8) {
-            const int r = 8 * s8 + lane / 8, ch = lane % 8;
-            typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-            __builtin_nontemporal_store(*(const u4 *)(ep + r * EP + 8 * ch), (u4 *)(Cp + (long long)r * ldc + 8 * ch));
+        for (int s = 0; s < NS - 1; ++s)
+            if (s < total) issue(s);
+        for (int s = 0; s < total; ++s) {
+            bf3_wait_stage<NS>(s + NS - 2 >= total);
+            if (s + NS - 1 < total) issue(s + NS - 1);
+            if (active) bf3_mfma<NPROD>(smem + (s % NS) * BF3_STG, aoff, boff, acc);
         }
+        // every wave is done with the stages before the epilogue reuses LDS
+        __builtin_amdgcn_s_waitcnt(7 << 4);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
     }
+    if (!active) return;
+    bf3_epilogue<MODE>(acc, c, wk, jlo, r0, c0, Cbase, smem + w * 64 * BF3_EP, lane);
 }
 
 template <int NPROD>
 static void launch_bf3_np(hipStream_t st, int mode, int nst, const ChunkDev *chunks, const Work *work, int nwork,
-                          const __bf16 *a, const __bf16 *b, void *C) {
+                          const __bf16 *a, const __bf16 *b, void *C, int jlo) {
     const int grid = 8 * ((nwork + 7) / 8);
-    // nst LDS stages for the left product and the Gram (one: three
-    // workgroups per CU, the next stage in registers; two: two workgroups per
-    // CU, a second register set two stages ahead); the right product's
-    // two-term sweep uses one.  Each output sums its K steps in the same
-    // order either way, so the results are identical.
+    // nst, the staging of the operands: 1 one LDS stage (three workgroups
+    // per CU, the next stage in registers); 2 two LDS stages (two workgroups
+    // per CU, a second register set two stages ahead; not for the right
+    // product, whose two-term sweep uses 1 instead); 3 and 4 direct-to-LDS
+    // loads with two (two workgroups per CU) and three (one) stages.  Each
+    // output sums its K steps in the same order either way, so the results
+    // are identical.
+#define QML_BF3_LAUNCH(M, N) k_gemm_bf3<M, N, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C, jlo)
+#define QML_BF3D_LAUNCH(M, N) k_gemm_bf3d<M, N, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C, jlo)
     if (mode == MODE_LEFT) {
-        if (nst == 2)
-            k_gemm_bf3<MODE_LEFT, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
-        else
-            k_gemm_bf3<MODE_LEFT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        if (nst == 4) QML_BF3D_LAUNCH(MODE_LEFT, 3);
+        else if (nst == 3) QML_BF3D_LAUNCH(MODE_LEFT, 2);
+        else if (nst == 2) QML_BF3_LAUNCH(MODE_LEFT, 2);
+        else QML_BF3_LAUNCH(MODE_LEFT, 1);
     } else if (mode == MODE_RIGHT) {
-        k_gemm_bf3<MODE_RIGHT, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        if (nst == 4) QML_BF3D_LAUNCH(MODE_RIGHT, 3);
+        else if (nst == 3) QML_BF3D_LAUNCH(MODE_RIGHT, 2);
+        else QML_BF3_LAUNCH(MODE_RIGHT, 1);
     } else {
-        if (nst == 1)
-            k_gemm_bf3<MODE_GRAM, 1, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
-        else
-            k_gemm_bf3<MODE_GRAM, 2, NPROD><<<grid, 256, 0, st>>>(chunks, work, nwork, a, b, C);
+        if (nst == 4) QML_BF3D_LAUNCH(MODE_GRAM, 3);
+        else if (nst == 3) QML_BF3D_LAUNCH(MODE_GRAM, 2);
+        else if (nst == 1) QML_BF3_LAUNCH(MODE_GRAM, 1);
+        else QML_BF3_LAUNCH(MODE_GRAM, 2);
     }
+#undef QML_BF3_LAUNCH
+#undef QML_BF3D_LAUNCH
 }
 
 void launch_bf3(hipStream_t st, int mode, int nprod, int nst, const ChunkDev *chunks, const Work *work, int nwork,
-                const void *A, const void *B, void *C) {
+                const void *A, const void *B, void *C, int jlo) {
     if (nwork <= 0) return;
     const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
     if (nprod == 4)
-        launch_bf3_np<4>(st, mode, nst, chunks, work, nwork, a, b, C);
+        launch_bf3_np<4>(st, mode, nst, chunks, work, nwork, a, b, C, jlo);
     else
-        launch_bf3_np<3>(st, mode, nst, chunks, work, nwork, a, b, C);
+        launch_bf3_np<3>(st, mode, nst, chunks, work, nwork, a, b, C, jlo);
 }
 
 }  // namespace qml

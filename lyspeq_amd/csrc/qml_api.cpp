@@ -60,6 +60,11 @@ struct Batch {
     ChunkDev *d_chunks = nullptr;
     Work *d_wl = nullptr, *d_wr = nullptr, *d_wg = nullptr;
     int nwl = 0, nwr = 0, nwg = 0;
+    // stack slabs: the left and right work lists of slab s are
+    // [sl_off[s], sl_off[s + 1]) of d_wl and [sr_off[s], sr_off[s + 1]) of d_wr
+    int nslab = 1;
+    std::vector<int> sl_off, sr_off;
+    std::vector<hipEvent_t> evS;   // 3 per slab after the first (xgen, left, right), timing only
     int np_max = 0, nvu_max = 0, nu_max = 0;
     int dp_rb = 1, dp_cb = 1;   // d-pass operand blocks of the batch
     int nq_max = 0;             // most N_Q_MATRICES of a chunk in the batch
@@ -130,7 +135,10 @@ struct qml_ctx {
     double scratch_gb = 32.0;
     int max_batch = 256;
     int pipeline = 1;
-    int nst_left = 1, nst_gram = 2;   // LDS stages of the split-BF16 left product and Gram
+    // staging of the split-BF16 products' operands (qml_set_kernel_stages):
+    // the defaults were the fastest on the 4096-chunk DESI batch
+    int nst_left = 1, nst_right = 3, nst_gram = 3;
+    int slab = 0;                     // stack members per slab (0: the whole stack at once)
     int poison = 0;   // LYSPEQ_QML_POISON=1 at commit: NaN-filled scratch and LDS
     hipStream_t st = nullptr, st2 = nullptr;
     bool bins_set = false, committed = false;
@@ -155,6 +163,8 @@ struct qml_ctx {
     void *d_lic[2] = {};
     double *d_gram = nullptr;
     void *d_X = nullptr, *d_Y = nullptr;
+    void *d_G = nullptr;   // the G store when the stack is slabbed (else G is written over X)
+    double max_chunk_bytes = 0;   // the largest chunk's scratch in the current plan
     double *d_theta = nullptr, *d_F = nullptr, *d_dbtsum = nullptr;
     double *d_rin = nullptr;   // resolution-matrix rows and fine grids
     std::vector<Batch> batches;
@@ -329,9 +339,11 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
 }  // extern "C"
 
 namespace {
-// Per-chunk scratch (elements of each arena) at padded size Np with m stack members.
-struct Need { size_t mat, vec, pj, sj, stk, lic, gram; };
-Need chunk_need(int prec, const ChunkHost &ch, size_t Np, size_t m) {
+// Per-chunk scratch (elements of each arena) at padded size Np with m stack
+// members.  With the stack slabbed (slab > 0 members) the X and Y arenas hold
+// one slab and the G store (packed upper tiles of every member) the rest.
+struct Need { size_t mat, vec, pj, sj, stk, lic, gram, g; };
+Need chunk_need(int prec, const ChunkHost &ch, size_t Np, size_t m, int slab) {
     Need n{};
     const size_t nt = Np / TILE, ntu = nt * (nt + 1) / 2;
     const size_t mt = (m + TILE - 1) / TILE;
@@ -340,7 +352,8 @@ Need chunk_need(int prec, const ChunkHost &ch, size_t Np, size_t m) {
     n.vec = (size_t)VEC_SLOTS * Np;
     n.pj = m * Np * (1 + MAX_NU) + m * MAX_NU * Np + (size_t)ch.nzq * ch.nvu;
     n.sj = m * MAX_NU * MAX_NU + ch.n;
-    n.stk = m * Np * Np;
+    n.stk = (slab > 0 ? std::min(m, (size_t)slab) : m) * Np * Np;
+    n.g = (slab > 0) ? m * ntu * TILE * TILE : 0;
     n.lic = (prec == 0) ? 0 : Np * Np;
     n.gram = ntu * (mt * (mt + 1) / 2) * TILE * TILE;
     return n;
@@ -348,7 +361,7 @@ Need chunk_need(int prec, const ChunkHost &ch, size_t Np, size_t m) {
 size_t need_bytes(int prec, const Need &n) {
     const size_t es = (prec == 0) ? 8 : 4;
     // stage-A arenas (mat, vec, aux, s, Li copy) exist twice for the two-stream pipeline
-    return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic);
+    return 8 * (2 * (n.mat + n.vec + n.pj + n.sj) + n.gram) + es * (2 * n.stk + 2 * n.lic + n.g);
 }
 }  // namespace
 
@@ -546,7 +559,7 @@ static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm
     if (info) *info = inf;
     {   // the matrices must fit: Chunk::_setStoredMatrices (core/chunk_estimate.cpp:258-282)
         // throws "Not all matrices are stored.", here against the device scratch budget
-        const size_t bt = need_bytes(c->prec, chunk_need(c->prec, ch, align_up(N, TILE), (size_t)ch.n + 2));
+        const size_t bt = need_bytes(c->prec, chunk_need(c->prec, ch, align_up(N, TILE), (size_t)ch.n + 2, c->slab));
         if (bt > (size_t)(c->scratch_gb * 1073741824.0)) {
             char msg[200];
             std::snprintf(msg, sizeof msg,
@@ -590,6 +603,7 @@ void release_plan(qml_ctx *c) {
         hipFree(b.d_chunks); hipFree(b.d_wl); hipFree(b.d_wr); hipFree(b.d_wg);
         for (auto &e : b.evA) if (e) hipEventDestroy(e);
         for (auto &e : b.evB) if (e) hipEventDestroy(e);
+        for (auto &e : b.evS) if (e) hipEventDestroy(e);
         if (b.evR) hipEventDestroy(b.evR);
     }
     c->batches.clear();
@@ -600,10 +614,10 @@ void release_plan(qml_ctx *c) {
         c->d_mat[k] = c->d_vec[k] = c->d_pj[k] = c->d_sj[k] = nullptr;
         c->d_lic[k] = nullptr;
     }
-    void *rest[] = {c->d_gram, c->d_X, c->d_Y};
+    void *rest[] = {c->d_gram, c->d_X, c->d_Y, c->d_G};
     for (void *p : rest) if (p) hipFree(p);
     c->d_gram = nullptr;
-    c->d_X = c->d_Y = nullptr;
+    c->d_X = c->d_Y = c->d_G = nullptr;
 }
 
 // Size-bucketed batch plan: chunks sorted by padded size (then stored Q
@@ -612,8 +626,9 @@ void release_plan(qml_ctx *c) {
 // batch and uploads every batch's chunk descriptors and work lists.
 int plan_batches(qml_ctx *c) {
     const int nch = (int)c->chunks.size();
+    c->max_chunk_bytes = 0;
     const size_t es = (c->prec == 0) ? 8 : 4;
-    auto need = [&](const ChunkHost &ch) { return chunk_need(c->prec, ch, ch.Np, ch.m); };
+    auto need = [&](const ChunkHost &ch) { return chunk_need(c->prec, ch, ch.Np, ch.m, c->slab); };
     auto bytes = [&](const Need &n) { return need_bytes(c->prec, n); };
     std::vector<int> order(nch);
     std::iota(order.begin(), order.end(), 0);
@@ -677,6 +692,7 @@ int plan_batches(qml_ctx *c) {
             // the split form stores two bf16 planes per element: offsets in bf16 units
             const long long su = (c->prec >= 2) ? 2 : 1;
             d.stk_off = (long long)acc.stk * su;
+            d.g_off = (c->slab > 0) ? (long long)acc.g * su : d.stk_off;
             d.gram_off = (long long)acc.gram;
             d.lic_off = (c->prec == 0) ? (long long)(acc.mat + 2 * (size_t)ch.Np * ch.Np) : (long long)acc.lic * su;
             d.rm = ch.rm;
@@ -687,13 +703,14 @@ int plan_batches(qml_ctx *c) {
             d.rin_off = ch.rin_off;
             d.qb_off = (long long)(acc.mat + 4 * (size_t)ch.Np * ch.Np);
             acc.mat += n.mat; acc.vec += n.vec; acc.pj += n.pj; acc.sj += n.sj;
-            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram;
+            acc.stk += n.stk; acc.lic += n.lic; acc.gram += n.gram; acc.g += n.g;
+            c->max_chunk_bytes = std::max(c->max_chunk_bytes, (double)bytes(n));
             cd.push_back(d);
         }
         mx.mat = std::max(mx.mat, acc.mat); mx.vec = std::max(mx.vec, acc.vec);
         mx.pj = std::max(mx.pj, acc.pj); mx.sj = std::max(mx.sj, acc.sj);
         mx.stk = std::max(mx.stk, acc.stk); mx.lic = std::max(mx.lic, acc.lic);
-        mx.gram = std::max(mx.gram, acc.gram);
+        mx.gram = std::max(mx.gram, acc.gram); mx.g = std::max(mx.g, acc.g);
         devs.push_back(cd);
     }
     for (int k = 0; k < 2; ++k) {
@@ -707,6 +724,7 @@ int plan_batches(qml_ctx *c) {
     HIPCHK(hipMalloc(&c->d_gram, std::max<size_t>(mx.gram, 1) * 8));
     HIPCHK(hipMalloc(&c->d_X, std::max<size_t>(mx.stk, 1) * es));
     HIPCHK(hipMalloc(&c->d_Y, std::max<size_t>(mx.stk, 1) * es));
+    if (c->slab > 0) HIPCHK(hipMalloc(&c->d_G, std::max<size_t>(mx.g, 1) * es));
     // LYSPEQ_QML_POISON=1 fills every scratch arena with all-ones bytes (NaN
     // in FP64, FP32 and bf16), so that a read of memory the iteration did
     // not write shows up in the results.  The vector arena keeps its zeros.
@@ -722,6 +740,7 @@ int plan_batches(qml_ctx *c) {
         HIPCHK(hipMemset(c->d_gram, 0xff, std::max<size_t>(mx.gram, 1) * 8));
         HIPCHK(hipMemset(c->d_X, 0xff, std::max<size_t>(mx.stk, 1) * es));
         HIPCHK(hipMemset(c->d_Y, 0xff, std::max<size_t>(mx.stk, 1) * es));
+        if (c->d_G) HIPCHK(hipMemset(c->d_G, 0xff, std::max<size_t>(mx.g, 1) * es));
     }
 
     for (size_t bi = 0; bi < groups.size(); ++bi) {
@@ -744,16 +763,8 @@ int plan_batches(qml_ctx *c) {
                 rb_max = std::max(rb_max, 31 * d.ros + d.rnb);
                 nb_max = std::max(nb_max, d.rnb);
             }
-            // Output tiles: 64x64 for FP64 and FP32, 128x128 for split BF16,
-            // whose left product P = Li M is lower triangular (J <= I only)
             const int g = (c->prec >= 2) ? 2 : 1;
-            const int nt = (d.nt + g - 1) / g, mtg = (d.mt + g - 1) / g;
-            for (int j = 0; j < d.m; ++j)
-                for (int I = 0; I < nt; ++I)
-                    for (int J = 0; J < ((c->prec >= 2) ? I + 1 : nt); ++J) wl.push_back({li, j, I, J});
-            for (int j = 0; j < d.m; ++j)
-                for (int I = 0; I < nt; ++I)
-                    for (int J = I; J < nt; ++J) wr.push_back({li, j, I, J});
+            const int mtg = (d.mt + g - 1) / g;
             for (int s = 0; s < d.ntu; ++s)
                 for (int I = 0; I < mtg; ++I)
                     for (int J = I; J < mtg; ++J) wg.push_back({li, s, I, J});
@@ -766,6 +777,34 @@ int plan_batches(qml_ctx *c) {
             if (B.rm_lay.bytes > 65536 || B.rm_lay.em != 10 || B.rm_lay.npass > 4)
                 B.rm_lay = rmat_layout(16, ra_max, ra_max, nb_max);
             if (!rmat_layout_ok(B.rm_lay)) return fail(QML_ERR_LIMIT, "qml_commit: no sandwich kernel for the resolution matrices");
+        }
+        // Output tiles of the whitening products: 64x64 for FP64 and FP32,
+        // 128x128 for split BF16, whose left product P = Li M is lower
+        // triangular (J <= I only).  The lists run slab by slab (one slab of
+        // every member without slabbing), chunk by chunk within a slab.
+        {
+            int mmax = 0;
+            for (const ChunkDev &d : cd) mmax = std::max(mmax, d.m);
+            const int S = (c->slab > 0) ? c->slab : std::max(mmax, 1);
+            B.nslab = (mmax + S - 1) / S;
+            B.sl_off.assign(1, 0);
+            B.sr_off.assign(1, 0);
+            const int g = (c->prec >= 2) ? 2 : 1;
+            for (int sb = 0; sb < B.nslab; ++sb) {
+                for (int li = 0; li < (int)cd.size(); ++li) {
+                    const ChunkDev &d = cd[li];
+                    const int nt = (d.nt + g - 1) / g;
+                    const int ja = sb * S, jb = std::min(d.m, (sb + 1) * S);
+                    for (int j = ja; j < jb; ++j) {
+                        for (int I = 0; I < nt; ++I)
+                            for (int J = 0; J < ((c->prec >= 2) ? I + 1 : nt); ++J) wl.push_back({li, j, I, J});
+                        for (int I = 0; I < nt; ++I)
+                            for (int J = I; J < nt; ++J) wr.push_back({li, j, I, J});
+                    }
+                }
+                B.sl_off.push_back((int)wl.size());
+                B.sr_off.push_back((int)wr.size());
+            }
         }
         B.nwl = (int)wl.size();
         B.nwr = (int)wr.size();
@@ -781,6 +820,8 @@ int plan_batches(qml_ctx *c) {
         for (auto &e : B.evA) HIPCHK(hipEventCreate(&e));
         for (auto &e : B.evB) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipEventCreate(&B.evR));
+        B.evS.assign(3 * (B.nslab - 1), nullptr);
+        for (auto &e : B.evS) HIPCHK(hipEventCreate(&e));
         c->batches.push_back(B);
     }
     c->plan_ok = true;
@@ -961,13 +1002,24 @@ int enqueue_iteration(qml_ctx *c, bool timing) {
         HIPCHK(hipStreamWaitEvent(sB, B.evA[6], 0));
         if (c->poison) launch_poison_lds(sB);
         HIPCHK(mark(B.evB[0], sB));
-        launch_xgen(L2, c->d_X, c->prec);
-        HIPCHK(mark(B.evB[1], sB));
-        launch_gemm_whiten_left(sB, c->prec, c->nst_left, B.d_chunks, B.d_wl, B.nwl, li, c->d_X, c->d_Y);
-        HIPCHK(mark(B.evB[2], sB));
-        launch_gemm_whiten_right(sB, c->prec, B.d_chunks, B.d_wr, B.nwr, li, c->d_Y, c->d_X);
-        HIPCHK(mark(B.evB[3], sB));
-        launch_gemm_gram(sB, c->prec, c->nst_gram, B.d_chunks, B.d_wg, B.nwg, c->d_X, c->d_gram);
+        // the stack slab by slab: X and Y hold members [jlo, jlo + S) of every
+        // chunk, and each slab's G goes to the G store (over X without slabs)
+        void *G = (c->slab > 0) ? c->d_G : c->d_X;
+        const int S = (c->slab > 0) ? c->slab : (1 << 30);
+        for (int sb = 0; sb < B.nslab; ++sb) {
+            L2.jlo = sb * (c->slab > 0 ? S : 0);
+            L2.jhi = (c->slab > 0) ? L2.jlo + S : (1 << 30);
+            const bool last = sb + 1 == B.nslab;
+            launch_xgen(L2, c->d_X, c->prec);
+            HIPCHK(mark(last ? B.evB[1] : B.evS[3 * sb], sB));
+            launch_gemm_whiten_left(sB, c->prec, c->nst_left, B.d_chunks, B.d_wl + B.sl_off[sb],
+                                    B.sl_off[sb + 1] - B.sl_off[sb], li, c->d_X, c->d_Y, L2.jlo);
+            HIPCHK(mark(last ? B.evB[2] : B.evS[3 * sb + 1], sB));
+            launch_gemm_whiten_right(sB, c->prec, c->nst_right, B.d_chunks, B.d_wr + B.sr_off[sb],
+                                     B.sr_off[sb + 1] - B.sr_off[sb], li, c->d_Y, G, L2.jlo);
+            HIPCHK(mark(last ? B.evB[3] : B.evS[3 * sb + 2], sB));
+        }
+        launch_gemm_gram(sB, c->prec, c->nst_gram, B.d_chunks, B.d_wg, B.nwg, G, c->d_gram);
         HIPCHK(mark(B.evB[4], sB));
         launch_gram_finalize(L2, c->d_outs, c->d_gram, c->d_status, c->d_fc, c->d_dbt);
         HIPCHK(hipEventRecord(B.evB[5], sB));
@@ -1053,7 +1105,20 @@ int qml_iterate(qml_ctx *c, const double *theta, double *fisher_sum, double *dbt
                 hipEventElapsedTime(&ms, B.evA[0], B.evR);
                 tm.kernel_ms[QML_K_RMAT] += ms;
             }
-            for (int k = 0; k < 5; ++k) {
+            // stage B: slab by slab, xgen / left / right from the event before
+            hipEvent_t prev = B.evB[0];
+            for (int sb = 0; sb < B.nslab; ++sb) {
+                const bool last = sb + 1 == B.nslab;
+                hipEvent_t e3[3] = {last ? B.evB[1] : B.evS[3 * sb], last ? B.evB[2] : B.evS[3 * sb + 1],
+                                    last ? B.evB[3] : B.evS[3 * sb + 2]};
+                for (int k = 0; k < 3; ++k) {
+                    float ms = 0;
+                    hipEventElapsedTime(&ms, prev, e3[k]);
+                    tm.kernel_ms[6 + k] += ms;
+                    prev = e3[k];
+                }
+            }
+            for (int k = 3; k < 5; ++k) {
                 float ms = 0;
                 hipEventElapsedTime(&ms, B.evB[k], B.evB[k + 1]);
                 tm.kernel_ms[6 + k] += ms;
@@ -1150,27 +1215,33 @@ int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, in
         if (rc != QML_OK) return rc;
     }
     if (c->prec >= 2) {
-        // kernel shapes of the split-BF16 products: LDS stages of the left
-        // product and of the Gram, timed on the chosen plan
-        static const int cand[4][2] = {{1, 2}, {2, 2}, {1, 1}, {2, 1}};
-        int bl = c->nst_left, bg = c->nst_gram;
+        // staging of the split-BF16 products' operands, timed on the chosen
+        // plan one product at a time (left, right, Gram), each keeping the
+        // best choice found for the products before it
+        int st[3] = {c->nst_left, c->nst_right, c->nst_gram};
         double bt = -1.0;
-        for (const auto &v : cand) {
-            const int rs = qml_set_kernel_stages(c, v[0], v[1]);
-            if (rs != QML_OK) return rs;
-            double t = 0.0;
-            for (int r = 0; r <= reps; ++r) {   // r = 0 warms the variant up
-                const int rc = qml_iterate(c, theta, F.data(), dbt.data());
-                if (rc != QML_OK) return rc;
-                if (r > 0 && (r == 1 || c->tm.total_ms < t)) t = c->tm.total_ms;
+        for (int p = 0; p < 3; ++p) {
+            int bp = st[p];
+            for (int v = 1; v <= 4; ++v) {
+                if (p == 1 && v == 2) continue;   // the right product has no two-register-set form
+                int trial[3] = {st[0], st[1], st[2]};
+                trial[p] = v;
+                const int rs = qml_set_kernel_stages(c, trial[0], trial[1], trial[2]);
+                if (rs != QML_OK) return rs;
+                double t = 0.0;
+                for (int r = 0; r <= reps; ++r) {   // r = 0 warms the variant up
+                    const int rc = qml_iterate(c, theta, F.data(), dbt.data());
+                    if (rc != QML_OK) return rc;
+                    if (r > 0 && (r == 1 || c->tm.total_ms < t)) t = c->tm.total_ms;
+                }
+                if (bt < 0 || t < bt) {
+                    bt = t;
+                    bp = v;
+                }
             }
-            if (bt < 0 || t < bt) {
-                bt = t;
-                bl = v[0];
-                bg = v[1];
-            }
+            st[p] = bp;
         }
-        const int rs = qml_set_kernel_stages(c, bl, bg);
+        const int rs = qml_set_kernel_stages(c, st[0], st[1], st[2]);
         if (rs != QML_OK) return rs;
         best_t = bt;
     }
@@ -1179,24 +1250,51 @@ int qml_autotune(qml_ctx *c, const double *theta, const int *caps, int ncaps, in
     return QML_OK;
 }
 
-int qml_set_kernel_stages(qml_ctx *c, int stages_left, int stages_gram) {
+int qml_set_kernel_stages(qml_ctx *c, int left, int right, int gram) {
     if (!c) return fail(QML_ERR_ARG, "qml_set_kernel_stages: null context");
-    if (stages_left < 1 || stages_left > 2 || stages_gram < 1 || stages_gram > 2)
-        return fail(QML_ERR_ARG, "qml_set_kernel_stages: stages must be 1 or 2");
-    if (stages_left != c->nst_left || stages_gram != c->nst_gram) {
+    if (left < 1 || left > 4 || right < 1 || right > 4 || right == 2 || gram < 1 || gram > 4)
+        return fail(QML_ERR_ARG, "qml_set_kernel_stages: staging must be 1 to 4 (right: 1, 3 or 4)");
+    if (left != c->nst_left || right != c->nst_right || gram != c->nst_gram) {
         HIPCHK(hipSetDevice(c->device));
         drop_graph(c);   // a captured sequence holds the launches of the old kernels
     }
-    c->nst_left = stages_left;
-    c->nst_gram = stages_gram;
+    c->nst_left = left;
+    c->nst_right = right;
+    c->nst_gram = gram;
+    return QML_OK;
+}
+
+int qml_set_stack_slab(qml_ctx *c, int members) {
+    if (!c) return fail(QML_ERR_ARG, "qml_set_stack_slab: null context");
+    if (members < 0) return fail(QML_ERR_ARG, "qml_set_stack_slab: members must be >= 0");
+    if (members == c->slab) return QML_OK;
+    const int prev = c->slab;
+    c->slab = members;
+    if (c->committed) {   // plan again under the new arenas
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipStreamSynchronize(c->st2));
+        release_plan(c);
+        const int rc = plan_batches(c);
+        if (rc != QML_OK) {
+            const std::string why = g_err;
+            release_plan(c);
+            c->slab = prev;
+            plan_batches(c);
+            return fail(rc, why);
+        }
+    }
     return QML_OK;
 }
 
 int qml_get_schedule(qml_ctx *c, qml_schedule *s) {
     if (!c || !s) return fail(QML_ERR_ARG, "qml_get_schedule: null argument");
+    s->stack_slab = c->slab;
+    s->max_chunk_gb = c->max_chunk_bytes / 1073741824.0;
     s->max_batch_chunks = c->max_batch;
     s->n_batches = (int)c->batches.size();
     s->stages_left = c->nst_left;
+    s->stages_right = c->nst_right;
     s->stages_gram = c->nst_gram;
     s->pipeline = c->pipeline;
     s->graph = c->use_graph;

@@ -56,7 +56,8 @@ struct ChunkDev {
     long long in_off;    // into inputs (doubles)
     long long mat_off;   // into the FP64 matrix arena: A, L, Li, S, each Np*Np
     long long vec_off;   // into the FP64 vector arena
-    long long stk_off;   // into the stack arenas (compute-type elements), m*Np*Np each
+    long long stk_off;   // into the stack arenas (compute-type elements), Np*Np per member of a slab
+    long long g_off;     // into the G store (compute-type elements): member j's packed upper tiles
     long long pj_off;    // into the aux arena (doubles): P_j, m x Np x (1+MAX_NU)
     long long at_off;    // into the aux arena: projected template weights, m x MAX_NU x Np
     long long th_off;    // into the aux arena: theta-contracted Q tables, nzq x nvu

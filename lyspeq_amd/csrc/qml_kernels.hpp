@@ -26,6 +26,7 @@ struct LaunchCtx {
     int nvu_max;              // largest theta-table length in the batch
     int nu_max;               // most continuum templates of a chunk in the batch
     int dp_rb, dp_cb;
+    int jlo = 0, jhi = 1 << 30;   // members [jlo, jhi) of the stack slab (all members: 0, large)
     int nq_max;               // most N_Q_MATRICES of a chunk in the batch         // d-pass operand blocks: most (KBIN_UPP + 1)/16 and (z bins + 1)(1 + nu)/16, rounded up
     const double *in;         // per-chunk input vectors
     const TableDev *tables;   // device table descriptors
@@ -65,14 +66,17 @@ void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, 
                        int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum);
 
 // kernels_gemm.hip.  prec: 0 FP64, 1 FP32, 2 BF16x3, 3 BF16x4.
-// nst: LDS stages of the split-BF16 left product and Gram (1 or 2; the
-// FP64/FP32 products and the right product ignore it).
-// Y_j = Li X_j for every member (lower-triangular Li).
+// nst: staging of the split-BF16 operands (1, 2 register-staged with one or
+// two LDS stages, 2 not for the right product; 3, 4 direct-to-LDS loads with
+// two or three stages); the FP64/FP32 products ignore it.
+// The stack arenas X and Y hold the members [jlo, jlo + slab) of each chunk
+// (member j at slot j - jlo); the work lists name global members.
+// Y_j = Li X_j for every member of the work list (lower-triangular Li).
 void launch_gemm_whiten_left(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
-                             int nwork, const void *li, const void *X, void *Y);
-// G_j = Y_j Li^T on upper tiles, written tile-packed.
-void launch_gemm_whiten_right(hipStream_t st, int prec, const ChunkDev *chunks, const Work *work, int nwork,
-                              const void *li, const void *Y, void *G);
+                             int nwork, const void *li, const void *X, void *Y, int jlo);
+// G_j = Y_j Li^T on upper tiles, written tile-packed into the G store.
+void launch_gemm_whiten_right(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work,
+                              int nwork, const void *li, const void *Y, void *G, int jlo);
 // Gram partials E_s = sum over G tile s of G_j G_j'.
 void launch_gemm_gram(hipStream_t st, int prec, int nst, const ChunkDev *chunks, const Work *work, int nwork,
                       const void *G, double *gram);

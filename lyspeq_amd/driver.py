@@ -63,6 +63,30 @@ def power_estimates(fisher_sum: np.ndarray, dbt_sum: np.ndarray) -> dict:
                 weighted=weighted, damp=damp, dof=dof)
 
 
+def _smoothing_spline(x, y, z, w, s):
+    """scipy's SmoothBivariateSpline(x, y, z, w=w, s=s) (bicubic, FITPACK
+    surfit), except for the workspace: scipy retries surfit once with the
+    lwrk2 that the first call asks for (ier > 10), and when the retry asks
+    for more it warns and keeps the unfinished fit.  Here the workspace grows
+    until surfit completes, so the fit is the one the routine defines."""
+    import warnings
+    from scipy.interpolate import BivariateSpline
+    from scipy.interpolate import _dfitpack as dfitpack
+    x, y, z, w = (np.ascontiguousarray(a, dtype=float) for a in (x, y, z, w))
+    lwrk2 = 1
+    for _ in range(16):
+        nx, tx, ny, ty, c, fp, _, ier = dfitpack.surfit_smth(x, y, z, w, kx=3, ky=3, s=float(s), eps=1e-16,
+                                                            lwrk2=lwrk2)
+        if ier <= 10:
+            break
+        lwrk2 = ier
+    if ier not in (0, -1, -2):
+        warnings.warn(f"surfit: ier={ier} (fp={fp:g}, s={s:g})", stacklevel=3)
+    spl = BivariateSpline._from_tck((tx[:nx], ty[:ny], c[:(nx - 4) * (ny - 4)], 3, 3))
+    spl.fp, spl.ier = fp, ier   # surfit's weighted residual sum and return code
+    return spl
+
+
 def smooth_power(theta: np.ndarray, sigma: np.ndarray, kcenters: np.ndarray, zcenters: np.ndarray,
                  pfid: np.ndarray, interp_log: bool = False) -> np.ndarray:
     """The fits the next iteration uses, as OneDQuadraticPowerEstimate::
@@ -74,9 +98,11 @@ def smooth_power(theta: np.ndarray, sigma: np.ndarray, kcenters: np.ndarray, zce
     P/e over the bands with P, e > 0, and evaluates it at every band;
     _readScriptOutput (:340-348) subtracts P_fid again.  Bands with e <= 0
     (no data), whose weight the script would make infinite, are left out of
-    the fit here.  kcenters (nk) and zcenters (nz) give the band grid; theta,
-    sigma and pfid have nz * nk values."""
-    from scipy.interpolate import SmoothBivariateSpline
+    the fit here.  Where scipy's one retry leaves surfit short of workspace
+    (ier > 10: an unfinished fit and a warning in the reference's run), the
+    workspace is grown until the fit completes (_smoothing_spline).
+    kcenters (nk) and zcenters (nz) give the band grid; theta, sigma and pfid
+    have nz * nk values."""
     nk, nz = len(kcenters), len(zcenters)
     assert theta.size == sigma.size == pfid.size == nk * nz
     zz = np.repeat(np.asarray(zcenters, float), nk)
@@ -88,10 +114,10 @@ def smooth_power(theta: np.ndarray, sigma: np.ndarray, kcenters: np.ndarray, zce
     e = np.array([float(f"{v:e}") for v in np.nan_to_num(sigma, nan=0.0)])
     if interp_log:
         m = (p > 0) & (e > 0)
-        spl = SmoothBivariateSpline(np.log(1 + z[m]), np.log(k[m]), np.log(p[m]), w=p[m] / e[m], s=int(m.sum()))
+        spl = _smoothing_spline(np.log(1 + z[m]), np.log(k[m]), np.log(p[m]), p[m] / e[m], int(m.sum()))
         smooth = np.exp(spl(np.log(1 + z), np.log(k), grid=False))
     else:
         m = e > 0
-        spl = SmoothBivariateSpline(z[m], k[m], p[m], w=1.0 / e[m], s=int(m.sum()))
+        spl = _smoothing_spline(z[m], k[m], p[m], 1.0 / e[m], int(m.sum()))
         smooth = spl(z, k, grid=False)
     return smooth - pfid

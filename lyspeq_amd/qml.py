@@ -118,9 +118,13 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median", "qml_bootstrap_median_noalloc"]
 
 
+EXPORTS += ["qml_set_stack_slab"]
+
+
 class _Schedule(C.Structure):
     _fields_ = [("max_batch_chunks", C.c_int), ("n_batches", C.c_int), ("stages_left", C.c_int),
-                ("stages_gram", C.c_int), ("pipeline", C.c_int), ("graph", C.c_int)]
+                ("stages_gram", C.c_int), ("pipeline", C.c_int), ("graph", C.c_int), ("stack_slab", C.c_int),
+                ("max_chunk_gb", C.c_double), ("stages_right", C.c_int)]
 
 
 class _TableSpec(C.Structure):
@@ -156,8 +160,9 @@ def load_library(path: str = LIB_PATH):
     lib.qml_get_timings.argtypes = [vp, C.POINTER(Timings)]
     lib.qml_set_pipeline.argtypes = [vp, C.c_int]
     lib.qml_set_graph.argtypes = [vp, C.c_int]
-    lib.qml_set_kernel_stages.argtypes = [vp, C.c_int, C.c_int]
+    lib.qml_set_kernel_stages.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.qml_get_schedule.argtypes = [vp, C.POINTER(_Schedule)]
+    lib.qml_set_stack_slab.argtypes = [vp, C.c_int]
     lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     ip = C.POINTER(C.c_int)
@@ -279,7 +284,7 @@ class QMLEngine:
     """One device context holding tables and chunks for repeated iterations."""
 
     def __init__(self, binning: Binning, precision="fp64", device: int = 0,
-                 scratch_gb: float = 0.0, max_batch_chunks: int = 0):
+                 scratch_gb: float = 0.0, max_batch_chunks: int = 0, stack_slab: int = 0):
         self._lib = load_library()
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         opts = _Opts(device, prec, scratch_gb, max_batch_chunks)
@@ -296,6 +301,8 @@ class QMLEngine:
                   binning.max_pixel_length_v, int(bool(binning.redshift_growth_on)),
                   float(binning.fiducial_redshift_power))
         _check(self._lib.qml_set_bins(self._ctx, C.byref(b)), self._lib)
+        if stack_slab:
+            self.set_stack_slab(stack_slab)
         self.T = len(self._edges) - 1
         self.T *= binning.nz
         self.tables = []
@@ -459,9 +466,15 @@ class QMLEngine:
                self._lib)
         return med, mad
 
-    def set_kernel_stages(self, left: int, gram: int):
-        """LDS stages (1 or 2) of the split-BF16 left product and Gram."""
-        _check(self._lib.qml_set_kernel_stages(self._ctx, int(left), int(gram)), self._lib)
+    def set_kernel_stages(self, left: int, right: int, gram: int):
+        """Operand staging of the split-BF16 left, right and Gram products:
+        1, 2 through registers with one or two LDS stages (2 not for the
+        right product), 3, 4 direct-to-LDS loads with two or three stages."""
+        _check(self._lib.qml_set_kernel_stages(self._ctx, int(left), int(right), int(gram)), self._lib)
+
+    def set_stack_slab(self, members: int):
+        """Streams each chunk's stack in slabs of `members` (0: all at once)."""
+        _check(self._lib.qml_set_stack_slab(self._ctx, int(members)), self._lib)
 
     def schedule(self) -> dict:
         s = _Schedule()

@@ -27,7 +27,7 @@ Modules
               (core/sq_table.cpp:201-323, core/fiducial_cosmology.cpp:273-331).
 
 How each function is pinned (golden vectors, invariants, closed forms) is
-listed in each module's docstring and exercised by ``tests/test_oracle_*.py``.
+listed in each module's docstring and exercised by ``tests/test_oracle_golden.py`` and ``tests/test_bootstrap_oracle.py``.
 Parity status: every function above is pinned, except where a module header
 says "parity unpinned".
 """

@@ -5,7 +5,7 @@ spectrum file (tests/truth/test_it1_quadratic_power_estimate_detailed.dat:
 columns z, k1, k2, kc); ``zbin_triangular_first`` by q0_matrix.txt;
 ``zbin_triangular_first`` and ``zbin_triangular_last`` by the golden Fisher
 matrix; the interior triangle and the top hat by their closed forms and by the
-partition of unity sum_m w_m(z) = 1 (tests/test_oracle_bins.py).
+partition of unity sum_m w_m(z) = 1 (tests/test_oracle_golden.py::test_redshift_weights_partition_of_unity).
 """
 from __future__ import annotations
 

@@ -21,7 +21,7 @@ mathtools/matrix_helper.cpp:184-204) and dgesvd with jobu='O'
 numpy.  Row-major CBLAS "Upper" calls are expressed as products with the
 symmetric matrix rebuilt from the upper triangle.
 
-Pins (tests/test_oracle_chunk.py): the golden 488x488 S_fid and Q_0 matrices
+Pins (tests/test_oracle_golden.py): the golden 488x488 S_fid and Q_0 matrices
 (all entries), the golden 10x10 Fisher matrix, the golden d/b/t sums and
 theta/sigma of tests/truth/test_it1_*; LU inverse against
 truth_inverse_matrix.txt; the marginalised inverse against the block-Woodbury

@@ -12,7 +12,7 @@ the reference's order.
 
 Pins: the golden signal_matrix.txt and q0_matrix.txt (all 488^2 entries) and
 closed forms (exact reproduction at grid nodes, linearity between nodes,
-extrapolation beyond the ends) in tests/test_oracle_interp.py.
+extrapolation beyond the ends) in tests/test_oracle_golden.py::test_interp_closed_forms.
 """
 from __future__ import annotations
 

@@ -13,7 +13,12 @@
   read, mask outliers, cut; None where the reference throws (low SNR, empty,
   fewer than 100 unmasked pixels).
 
-Test infrastructure only (see oracle/__init__.py).
+Test infrastructure only (see oracle/__init__.py).  Pins
+(tests/test_oracle_golden.py): the reference's test spectrum passes whole
+and its chunk reproduces the golden Fisher matrix; constructed outliers at
+either side of each threshold (MAD floor of 1, per-pixel sigma, the mean
+flux bound); the redshift and masked-boundary cuts against their closed
+forms.
 """
 from __future__ import annotations
 

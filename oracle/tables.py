@@ -15,7 +15,7 @@ core/sq_table.cpp:201-323).
 
 Pins: tests/truth/signal_R70000_dv20.0.fits (both tables) at the reference
 test's tolerance, rel 1e-5 / abs 1e-8 (tests/compareTestResults.py:8-9,
-80-100), in tests/test_oracle_tables.py.
+80-100), in tests/test_oracle_golden.py::test_tables_against_golden_fits.
 """
 from __future__ import annotations
 

@@ -167,6 +167,9 @@ def test_dropin_iteration_and_bootstrap_against_oracle(tmp_path, fast):
     weights = rng.poisson(1.0, (len(spectra), nboots))
     cfg = str(tmp_path / "run.cfg")
     _write_config(cfg, w.binning, files, fast_bootstrap=fast, nboots=nboots, weights=weights.reshape(-1))
+    base = str(tmp_path / "qmle")
+    with open(cfg, "a") as f:
+        f.write(f"save_chunks {base}\nsave_boot {base}\n")
     outp = str(tmp_path / "out.txt")
     r = subprocess.run([DROPIN_CHECK, "run", cfg, str(tdir), "sq", "fp64", outp], capture_output=True, text=True,
                        timeout=300)
@@ -219,6 +222,18 @@ def test_dropin_iteration_and_bootstrap_against_oracle(tmp_path, fast):
     for s_, (lo, n) in enumerate(spans):
         power[:, lo:lo + n] += weights[s_][:, None] * thetas[s_][None, :]
     np.testing.assert_allclose(rec["bootpower"].reshape(nboots, T), power, rtol=1e-13, atol=0)
+    # _saveChunkResults' file holds every chunk's d, b, t and Fisher upper
+    # triangle; the realizations file the bootstrap powers
+    saved = sqio.read_boot_chunks(base + "-bootchunks-0.fits")
+    assert len(saved) == len(oflat)
+    for i, sc in enumerate(saved):
+        Fc, dc = gout[i]
+        iu = np.triu_indices(Fc.shape[0])
+        assert sc.istart == chunks_out[i]["fis"] and sc.d.size == chunks_out[i]["nq"]
+        assert np.array_equal(np.array([sc.d, sc.b, sc.t]), dc) and np.array_equal(sc.fisher[iu], Fc[iu])
+    bp, binv, bn, bk, bz, bfast = sqio.read_bootstrap_realizations(base + "-bootstrap-realizations.fits")
+    assert (bn, bk * bz, bfast) == (nboots, T, fast)
+    assert np.array_equal(bp.reshape(-1), rec["bootpower"]) and np.array_equal(binv, F)
     if not fast:
         tp, tf = np.zeros(T), np.zeros((T, T))
         for s_, (lo, n) in enumerate(spans):

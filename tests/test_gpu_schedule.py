@@ -103,30 +103,41 @@ def test_autotuned_schedule_gives_identical_results(precision):
 
 @pytest.mark.parametrize("precision", ["bf16x3", "bf16x4"])
 def test_kernel_stage_variants_give_identical_results(precision):
-    """The split-BF16 left product and Gram in one or two LDS stages: every
-    combination, with a graph captured again after each change, must give
-    the default's results bit for bit."""
+    """The split-BF16 products' operand staging: through registers with one
+    or two LDS stages, and direct-to-LDS loads with two or three, for the
+    left product, the right product and the Gram.  Every choice, eager, in
+    stack slabs of 7 and replayed from a graph captured again after each
+    change, must give the default's results bit for bit."""
     _print_case(f"kernel stage variants {precision}")
     w = synth.desi_config(nchunks=24)
     th = P.theta_perturbed(w.binning, w.pd13, 0.05, 9)
     eng, acc = _engine(w, precision, max_batch_chunks=10)
     s = eng.schedule()
-    assert (s["stages_left"], s["stages_gram"]) == (1, 2) and s["n_batches"] >= 2
+    assert (s["stages_left"], s["stages_right"], s["stages_gram"]) == (1, 3, 3) and s["n_batches"] >= 2
     e0 = _run(eng, acc, th)
-    for left, gram in ((2, 2), (1, 1), (2, 1)):
-        eng.set_kernel_stages(left, gram)
+    for st in ((1, 1, 2), (2, 1, 1), (3, 3, 3), (4, 4, 4), (3, 4, 1), (4, 3, 2), (1, 3, 4)):
+        print(f"staging {st}", flush=True)
+        eng.set_kernel_stages(*st)
         _same(_run(eng, acc, th), e0)
+    eng.set_stack_slab(7)
+    for st in ((3, 3, 3), (4, 4, 4)):
+        eng.set_kernel_stages(*st)
+        _same(_run(eng, acc, th), e0)
+    eng.set_stack_slab(0)
     eng.set_graph(True)
-    for left, gram in ((1, 2), (2, 1)):
-        eng.set_kernel_stages(left, gram)
+    for st in ((1, 1, 2), (3, 3, 3), (4, 4, 4)):
+        eng.set_kernel_stages(*st)
         _same(_run(eng, acc, th), e0)
         _same(_run(eng, acc, th), e0)
-    with pytest.raises(Exception):
-        eng.set_kernel_stages(3, 1)
+    for bad in ((5, 1, 1), (1, 2, 1), (1, 1, 0)):
+        with pytest.raises(Exception):
+            eng.set_kernel_stages(*bad)
     best, ms = eng.autotune(th, caps=(10,), reps=1)
     s = eng.schedule()
-    print(f"autotuned stages {precision}: left {s['stages_left']} gram {s['stages_gram']}, {ms:.2f} ms", flush=True)
-    assert s["stages_left"] in (1, 2) and s["stages_gram"] in (1, 2) and s["max_batch_chunks"] == 10
+    print(f"autotuned staging {precision}: left {s['stages_left']} right {s['stages_right']} "
+          f"gram {s['stages_gram']}, {ms:.2f} ms", flush=True)
+    assert s["stages_left"] in (1, 2, 3, 4) and s["stages_right"] in (1, 3, 4)
+    assert s["stages_gram"] in (1, 2, 3, 4) and s["max_batch_chunks"] == 10
     _same(_run(eng, acc, th), e0)
     eng.close()
 
@@ -140,4 +151,28 @@ def test_autotune_rejects_bad_candidates():
         eng.autotune(np.zeros(T), caps=(0,))
     with pytest.raises(Exception):
         eng.autotune(np.zeros(T), caps=())
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16x4"])
+def test_stack_slabs_set_after_commit_with_graph_replay(precision):
+    """Slabs chosen on a committed engine (the batches planned again), with
+    both kernel-stage settings, eager and replayed, then back to the whole
+    stack: bit-identical to the first plan's results every time."""
+    _print_case(f"stack slabs after commit {precision}")
+    w = synth.desi_config(nchunks=24)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 8)
+    eng, acc = _engine(w, precision, max_batch_chunks=9)
+    ref = _run(eng, acc, th)
+    for stages in ((1, 1, 2), (2, 3, 1)):
+        eng.set_kernel_stages(*stages)
+        eng.set_stack_slab(7)
+        assert eng.schedule()["stack_slab"] == 7
+        _same(_run(eng, acc, th), ref)
+        eng.set_graph(True)
+        _same(_run(eng, acc, th), ref)
+        _same(_run(eng, acc, th), ref)
+        eng.set_graph(False)
+        eng.set_stack_slab(0)
+        _same(_run(eng, acc, th), ref)
     eng.close()

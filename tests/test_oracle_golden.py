@@ -444,3 +444,59 @@ def test_redshift_growth_is_a_per_pixel_congruence(power):
             assert scale > 0
             assert np.max(np.abs(q_on[s] - ref)) <= 1e-13 * scale
             assert np.array_equal(q_zero[s], q_off[s])
+
+
+def test_reader_preprocessing_on_the_golden_spectrum():
+    """The reader's masking and cuts leave the reference's test spectrum
+    whole (its golden run holds all 488 pixels), and the chunk built from
+    read_spectrum's output reproduces the golden Fisher matrix."""
+    from oracle import reader
+    b = Bins(**gc.BINS_KW)
+    sp = gc.spectrum()
+    out = reader.read_spectrum(b, sp.wave, sp.flux, sp.ivar, sp.snr)
+    assert out is not None
+    w, d, iv = out
+    assert w.size == 488 and np.array_equal(w, sp.wave) and np.array_equal(d, sp.flux)
+    t = gc.table()
+    ts = [TableSet(70000, 20.0, t.nvpoints, t.v_length, t.nzpoints, t.deriv, t.signal)]
+    ch = Chunk(b, ts, w, d, iv, sp.R_fwhm)
+    T = b.total_kz_bins
+    F, dbt = np.zeros((T, T)), np.zeros((3, T))
+    assert ch.one_iteration(np.zeros(T), F, dbt)
+    assert gc.all_close(power_estimates(F, dbt)["fisher"], gc.golden_fisher())
+
+
+def test_reader_outlier_mask_and_cuts_closed_forms():
+    """QSOFile::maskOutliers (io/qso_file.cpp:189-230) on constructed data:
+    with sigma = 0.1 and a MAD below 1 the threshold is 5 max(sigma, 1) = 5
+    about [-1, 1/F(lambda)], so -6.5 and F^-1 + 5.5 are masked, -5.5 and
+    F^-1 + 4.5 kept; with sigma = 2 (ivar 1/4) the threshold doubles.  The
+    median of an even count is the half-sum of the middle two.
+    cutBoundary (io/qso_file.cpp:159-187) keeps [lower_bound(l1),
+    upper_bound(l2)) and then trims masked pixels at both ends."""
+    from oracle import reader
+    from oracle.bins import LYA_REST
+    n = 40
+    wave = np.linspace(3500.0, 3600.0, n)
+    finv = np.exp(1.663e-3 * (wave / LYA_REST) ** 4.107)
+    delta = np.zeros(n)
+    ivar = np.full(n, 100.0)
+    delta[3], delta[5] = -6.5, -5.5
+    delta[7], delta[9] = finv[7] + 5.5, finv[9] + 4.5
+    ivar[11], delta[11] = 0.25, -1.0 - 9.5   # sigma 2: threshold 10, kept
+    ivar[13], delta[13] = 0.25, -1.0 - 10.5  # masked
+    d2, iv2, nm = reader.mask_outliers(wave, delta, ivar)
+    assert nm == 3
+    assert set(np.flatnonzero(iv2 == 0)) == {3, 7, 13}
+    assert np.all(d2[[3, 7, 13]] == 0) and np.array_equal(d2[[5, 9, 11]], delta[[5, 9, 11]])
+    assert reader._median(np.array([4.0, 1.0, 3.0, 2.0])) == 2.5 and reader._median(np.array([3.0, 1.0, 2.0])) == 2.0
+    b = Bins(**gc.BINS_KW)
+    l1, l2 = LYA_REST * (1 + b.z_lower_edge), LYA_REST * (1 + b.z_upper_edge)
+    wave = np.linspace(l1 - 20.0, l2 + 20.0, 200)
+    iv = np.ones(200)
+    a0, e0 = int(np.searchsorted(wave, l1, "left")), int(np.searchsorted(wave, l2, "right"))
+    iv[a0:a0 + 3] = 0.0
+    iv[e0 - 2:e0] = 0.0
+    assert reader.cut_boundary(b, wave, np.zeros(200), iv) == (a0 + 3, e0 - 2)
+    assert reader.cut_boundary(b, wave, np.zeros(200), np.zeros(200)) is None
+    assert reader.cut_boundary(b, wave + 5000.0, np.zeros(200), iv) is None

@@ -96,9 +96,11 @@ def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(
         cap, tms = eng.autotune(theta, caps=tune_caps, reps=1)
         schedule = dict(eng.schedule(), autotuned=True, candidates=list(tune_caps), tuned_ms=tms,
                         tuning_s=time.perf_counter() - t0)
-        log(f"[{precision}] autotuned batch cap {cap}, operand staging left {schedule['stages_left']} "
-            f"right {schedule['stages_right']} gram {schedule['stages_gram']} ({tms:.1f} ms) "
-            f"in {schedule['tuning_s']:.1f} s")
+        log(f"[{precision}] autotuned batch cap {cap}, Np buckets {schedule['np_buckets']}, operand staging "
+            f"left {schedule['stages_left']} right {schedule['stages_right']} gram {schedule['stages_gram']} "
+            f"({tms:.1f} ms) in {schedule['tuning_s']:.1f} s")
+    # the plan's batches: chunk count and padded-size range (Np bucket edges)
+    schedule["batches"] = [[b["n_chunks"], b["np_min"], b["np_max"]] for b in eng.batches()]
     if graph:
         eng.set_graph(True)
     for i in range(warmup):

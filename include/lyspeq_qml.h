@@ -368,8 +368,9 @@ int qml_poisson_draws(unsigned long long seed, long long n, int *out);
 /* Autotuned batch schedule.  For each candidate cap on chunks per batch
  * (caps[0..ncaps-1]), the committed chunks are planned again into
  * size-bucketed batches (sorted by padded size, cut at the cap or at the
- * scratch budget of qml_opts), and reps + 1 iterations are run at theta, the
- * first as warm-up; the plan with the shortest iteration is kept.  In the
+ * scratch budget of qml_opts, and with or without a break at every change of
+ * padded size), and reps + 1 iterations are run at theta, the first as
+ * warm-up; the plan with the shortest iteration is kept.  In the
  * split-BF16 precisions the staging of the three products' operands is tuned
  * next, on that plan, one product at a time (left, right, Gram;
  * qml_set_kernel_stages), each choice timed the same way.  The chosen cap and
@@ -416,8 +417,27 @@ typedef struct {
     int stack_slab;        /* members per stack slab, 0 for the whole stack (qml_set_stack_slab) */
     double max_chunk_gb;   /* the largest chunk's device scratch in the current plan, GiB */
     int stages_right;      /* operand staging of the split-BF16 right product */
+    int np_buckets;        /* 1 if a batch also ends where the padded size changes (qml_autotune) */
 } qml_schedule;
 int qml_get_schedule(qml_ctx *ctx, qml_schedule *schedule);
+
+/* Batch planning: 1 also ends a batch where the chunks' padded size changes
+ * (every batch then holds one size), 0 (the default) only at the cap on
+ * chunks per batch and at the scratch budget.  Set before qml_commit or any
+ * time after (the batches are planned again).  qml_autotune tries both.
+ * Results are identical.  No counterpart in the reference. */
+int qml_set_np_buckets(qml_ctx *ctx, int on);
+
+/* Batch `batch` (0 .. n_batches - 1, in launch order) of the current plan:
+ * its chunk count, the range of its chunks' padded sizes (the plan's
+ * Np bucket edges: chunks are planned in decreasing Np) and its largest
+ * stack member count. */
+typedef struct {
+    int n_chunks;
+    int np_min, np_max;
+    int members_max;
+} qml_batch_info;
+int qml_get_batch(qml_ctx *ctx, int batch, qml_batch_info *info);
 
 /* Number of chunks registered, and TOTAL_KZ_BINS. */
 int qml_num_chunks(qml_ctx *ctx);

@@ -112,12 +112,20 @@ def smooth_power(theta: np.ndarray, sigma: np.ndarray, kcenters: np.ndarray, zce
     k = np.array([float(f"{v:e}") for v in kk])
     p = np.array([float(f"{v:e}") for v in theta + pfid])
     e = np.array([float(f"{v:e}") for v in np.nan_to_num(sigma, nan=0.0)])
+    return smooth_values(z, k, p, e, interp_log) - pfid
+
+
+def smooth_values(z, k, p, e, interp_log: bool = False) -> np.ndarray:
+    """py/smbivspline.py's fit and evaluation on the values of its input
+    file (z, k, P, e per band): the weighted smoothing spline with s = the
+    number of points fitted, in (z, k, P) with weights 1/e, or with
+    interp_log in (ln(1+z), ln k, ln P) with weights P/e over the bands with
+    P, e > 0; bands with e <= 0 are left out of the fit."""
+    z, k, p, e = (np.asarray(a, float) for a in (z, k, p, e))
     if interp_log:
         m = (p > 0) & (e > 0)
         spl = _smoothing_spline(np.log(1 + z[m]), np.log(k[m]), np.log(p[m]), p[m] / e[m], int(m.sum()))
-        smooth = np.exp(spl(np.log(1 + z), np.log(k), grid=False))
-    else:
-        m = e > 0
-        spl = _smoothing_spline(z[m], k[m], p[m], 1.0 / e[m], int(m.sum()))
-        smooth = spl(z, k, grid=False)
-    return smooth - pfid
+        return np.exp(spl(np.log(1 + z), np.log(k), grid=False))
+    m = e > 0
+    spl = _smoothing_spline(z[m], k[m], p[m], 1.0 / e[m], int(m.sum()))
+    return spl(z, k, grid=False)

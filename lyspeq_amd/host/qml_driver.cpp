@@ -1,12 +1,15 @@
 // Newton-Raphson driver arithmetic on the host (see qml_driver.hpp).
 #include "qml_driver.hpp"
 
-#include "qml_host.hpp"
-
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <stdexcept>
+#include <string>
+
+#include "qml_host.hpp"
 
 namespace lyspeq_amd {
 
@@ -233,6 +236,41 @@ ConvergenceTest hasConverged(const double *current, const double *previous, cons
     c.chi2_full = std::sqrt(std::fabs(full) / dof);
     c.converged = c.chi2_diag < eps || c.chi2_full < eps;
     return c;
+}
+
+void ScriptSmoother::operator()(const std::vector<double> &theta, const std::vector<double> &sigma,
+                                std::vector<double> &fits) const {
+    const int nk = (int)kcenters.size(), nz = (int)zcenters.size(), T = nk * nz;
+    if ((int)theta.size() != T || (int)sigma.size() != T || (int)pfid.size() != T)
+        throw std::invalid_argument("ScriptSmoother: theta, sigma and pfid need nz * nk values");
+    const std::string fin = tmp_dir + "/tmp-power-" + tag + ".txt", fout = tmp_dir + "/tmp-smooth-" + tag + ".txt";
+    FILE *f = std::fopen(fin.c_str(), "w");
+    if (!f) throw std::runtime_error("ScriptSmoother: cannot write " + fin);
+    std::fprintf(f, "%d %d\n", nz, nk);
+    for (int z = 0; z < nz; ++z) std::fprintf(f, "%d ", z < (int)zcounts.size() ? zcounts[z] : 0);
+    std::fprintf(f, "\n");
+    for (int i = 0; i < T; ++i) {
+        const int kn = i % nk, zm = i / nk;
+        const double e = std::isnan(sigma[i]) ? 0.0 : sigma[i];
+        std::fprintf(f, "%.3lf %e %e %e\n", zcenters[zm], kcenters[kn], theta[i] + pfid[i], e);
+    }
+    std::fclose(f);
+    const std::string cmd = command + " " + fin + " " + fout + (interp_log ? " --interp_log" : "");
+    const int rc = std::system(cmd.c_str());
+    std::remove(fin.c_str());
+    if (rc != 0) throw std::runtime_error("smoothing error");
+    FILE *g = std::fopen(fout.c_str(), "r");
+    if (!g) throw std::runtime_error("smoothing error");
+    fits.assign(T, 0.0);
+    for (int i = 0; i < T; ++i) {
+        if (std::fscanf(g, "%le", &fits[i]) != 1) {
+            std::fclose(g);
+            throw std::runtime_error("Reading fit power values from tmp_fit_file!");
+        }
+        fits[i] -= pfid[i];
+    }
+    std::fclose(g);
+    std::remove(fout.c_str());
 }
 
 PowerEstimate::PowerEstimate(Engine &eng, std::vector<OneQSOEstimate *> qsos, int iterations, double eps,

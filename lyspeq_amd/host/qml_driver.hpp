@@ -30,6 +30,7 @@
 
 #include <functional>
 #include <memory>
+#include <string>
 #include <vector>
 
 namespace lyspeq_amd {
@@ -75,9 +76,34 @@ struct ConvergenceTest {
 ConvergenceTest hasConverged(const double *current, const double *previous, const double *inverse,
                              const double *fisher, int T, int dof, double eps);
 
+// The reference's smoothing step (OneDQuadraticPowerEstimate::
+// _smoothPowerSpectra, core/quadratic_estimate.cpp:353-396, with
+// writeSpectrumEstimates :624-650 and _readScriptOutput :320-348): writes
+// "nz nk", the z-bin counts, then z ("%.3lf"), k, P = theta + P_fid and
+// e = sigma ("%e") per band, z-major, to <tmp_dir>/tmp-power-<tag>.txt; runs
+// `<command> <in> <out>` (with " --interp_log" for the log mode) as a child
+// process through the shell, as the reference runs smbivspline.py; reads one
+// smoothed P per band from <tmp_dir>/tmp-smooth-<tag>.txt and subtracts P_fid.
+// command defaults to this build's script (python3 -m lyspeq_amd.smooth,
+// with the package on PYTHONPATH); the reference's "smbivspline.py" reads the
+// same files.  Throws std::runtime_error("smoothing error") if the command
+// fails, as the reference does.
+struct ScriptSmoother {
+    std::vector<double> kcenters;   // nk band centres
+    std::vector<double> zcenters;   // nz bin centres
+    std::vector<double> pfid;       // nz * nk fiducial power, z-major
+    std::vector<int> zcounts;       // nz chunk counts per z bin (the file's second line)
+    std::string command = "python3 -m lyspeq_amd.smooth";
+    std::string tmp_dir = "/tmp";
+    std::string tag = "0";          // the reference uses the MPI rank
+    bool interp_log = false;        // SmoothLogkLogP
+    void operator()(const std::vector<double> &theta, const std::vector<double> &sigma,
+                    std::vector<double> &fits) const;
+};
+
 // The iterate() loop.  smooth maps (theta, sigma) to the power fits used by
-// the next iteration (the reference runs py/smbivspline.py there); without
-// one the next iteration uses theta itself.
+// the next iteration (the reference runs py/smbivspline.py there: see
+// ScriptSmoother); without one the next iteration uses theta itself.
 class PowerEstimate {
 public:
     using Smoother = std::function<void(const std::vector<double> &theta, const std::vector<double> &sigma,

@@ -118,13 +118,17 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median", "qml_bootstrap_median_noalloc"]
 
 
-EXPORTS += ["qml_set_stack_slab"]
+EXPORTS += ["qml_set_stack_slab", "qml_get_batch", "qml_set_np_buckets"]
 
 
 class _Schedule(C.Structure):
     _fields_ = [("max_batch_chunks", C.c_int), ("n_batches", C.c_int), ("stages_left", C.c_int),
                 ("stages_gram", C.c_int), ("pipeline", C.c_int), ("graph", C.c_int), ("stack_slab", C.c_int),
-                ("max_chunk_gb", C.c_double), ("stages_right", C.c_int)]
+                ("max_chunk_gb", C.c_double), ("stages_right", C.c_int), ("np_buckets", C.c_int)]
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [("n_chunks", C.c_int), ("np_min", C.c_int), ("np_max", C.c_int), ("members_max", C.c_int)]
 
 
 class _TableSpec(C.Structure):
@@ -163,6 +167,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_set_kernel_stages.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.qml_get_schedule.argtypes = [vp, C.POINTER(_Schedule)]
     lib.qml_set_stack_slab.argtypes = [vp, C.c_int]
+    lib.qml_get_batch.argtypes = [vp, C.c_int, C.POINTER(_BatchInfo)]
+    lib.qml_set_np_buckets.argtypes = [vp, C.c_int]
     lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     ip = C.POINTER(C.c_int)
@@ -480,6 +486,20 @@ class QMLEngine:
         s = _Schedule()
         _check(self._lib.qml_get_schedule(self._ctx, C.byref(s)), self._lib)
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_np_buckets(self, on: bool):
+        """Also end a batch where the padded size changes (re-plans if committed)."""
+        _check(self._lib.qml_set_np_buckets(self._ctx, int(bool(on))), self._lib)
+
+    def batches(self) -> list:
+        """The current plan's batches in launch order: chunk count, padded
+        size range (the Np bucket edges) and largest member count."""
+        out = []
+        for b in range(self.schedule()["n_batches"]):
+            info = _BatchInfo()
+            _check(self._lib.qml_get_batch(self._ctx, b, C.byref(info)), self._lib)
+            out.append({k: getattr(info, k) for k, _ in info._fields_})
+        return out
 
     def set_chunk_active(self, cid: int, active: bool):
         """Leaves a chunk out of (or back in) the accumulated sums."""

@@ -8,6 +8,10 @@
 //                                                                  weighted, damp, dof
 //   driver_check converge in out   in: T, dof, eps, current, previous, inverse, fisher
 //                                  out: abs_mean, abs_max, chi2_diag, chi2_full, converged
+//   driver_check smooth in out command
+//                                  in: nz, nk, interp_log, zcenters, kcenters, theta, sigma,
+//                                      pfid (z-major)        out: the fits of ScriptSmoother
+//                                  running `command` (temporary files in $TMPDIR or /tmp)
 //
 // Files: int32 counts and little-endian doubles.  Exit status 3 and the
 // exception's message on stderr when the arithmetic throws.
@@ -15,6 +19,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -45,8 +50,9 @@ void put(FILE *o, double x) { std::fwrite(&x, 8, 1, o); }
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s invert|svals|power|converge in.bin out.bin\n", argv[0]);
+    if (argc != 4 && !(argc == 5 && std::string(argv[1]) == "smooth")) {
+        std::fprintf(stderr, "usage: %s invert|svals|power|converge in.bin out.bin | smooth in out command\n",
+                     argv[0]);
         return 2;
     }
     const std::string cmd = argv[1];
@@ -90,6 +96,20 @@ int main(int argc, char **argv) {
             put(o, c.chi2_diag);
             put(o, c.chi2_full);
             put(o, c.converged ? 1.0 : 0.0);
+        } else if (cmd == "smooth") {
+            const int nz = in.i(), nk = in.i(), lg = in.i();
+            ScriptSmoother sm;
+            sm.zcenters = in.v(nz);
+            sm.kcenters = in.v(nk);
+            const std::vector<double> theta = in.v((size_t)nz * nk), sigma = in.v((size_t)nz * nk);
+            sm.pfid = in.v((size_t)nz * nk);
+            sm.interp_log = lg != 0;
+            sm.command = argv[4];
+            const char *td = std::getenv("TMPDIR");
+            if (td && *td) sm.tmp_dir = td;
+            sm(theta, sigma, out);
+            o = std::fopen(argv[3], "wb");
+            put(o, out);
         } else {
             std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
             return 2;

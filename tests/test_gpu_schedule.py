@@ -176,3 +176,35 @@ def test_stack_slabs_set_after_commit_with_graph_replay(precision):
         eng.set_stack_slab(0)
         _same(_run(eng, acc, th), ref)
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_batches_report_np_edges_and_bucketed_plans_give_identical_results(precision):
+    """qml_get_batch: the plan's batches cover every chunk once, in
+    decreasing padded size, at most the cap each.  autotune tries each cap
+    with and without a batch break at every change of padded size; with it,
+    every batch holds one padded size.  Results are bit-identical under both
+    plans."""
+    _print_case(f"batch edges and Np buckets {precision}")
+    w = synth.desi_config(nchunks=40)
+    th = P.theta_perturbed(w.binning, w.pd13, 0.05, 12)
+    eng, acc = _engine(w, precision, max_batch_chunks=16)
+    e0 = _run(eng, acc, th)
+    bs = eng.batches()
+    print("batches:", bs, flush=True)
+    assert sum(b["n_chunks"] for b in bs) == len(acc) and all(b["n_chunks"] <= 16 for b in bs)
+    assert all(b["np_min"] <= b["np_max"] for b in bs)
+    assert all(bs[i]["np_min"] >= bs[i + 1]["np_max"] for i in range(len(bs) - 1))
+    assert eng.schedule()["np_buckets"] == 0
+    best, ms = eng.autotune(th, caps=(16, 64), reps=1)
+    s = eng.schedule()
+    print(f"autotuned cap {best}, np_buckets {s['np_buckets']}, {ms:.2f} ms", flush=True)
+    _same(_run(eng, acc, th), e0)
+    for on in (1, 0, 1):
+        eng.set_np_buckets(on)
+        bb = eng.batches()
+        assert eng.schedule()["np_buckets"] == on and sum(b["n_chunks"] for b in bb) == len(acc)
+        if on:
+            assert all(b["np_min"] == b["np_max"] for b in bb)
+        _same(_run(eng, acc, th), e0)
+    eng.close()

@@ -166,3 +166,73 @@ def test_convergence_test_closed_forms(tmp_path):
     prev2 = cur + np.sqrt(e2)
     r, o = _call(tmp_path, "converge", [T, T], [[1.0 + 1e-12], cur, prev2, np.diag(e2), np.zeros((T, T))])
     assert abs(o[2] - 1.0) < 1e-14 and o[3] == 0.0 and o[4] == 1.0
+
+
+def test_script_smoother_runs_the_smoothing_script_like_the_reference(tmp_path):
+    """ScriptSmoother (the reference's _smoothPowerSpectra: the spectrum file
+    of writeSpectrumEstimates, the script run as a child process, one value
+    per band read back, P_fid subtracted) running python3 -m
+    lyspeq_amd.smooth must give the Python driver's smooth_power fits bit
+    for bit, in both modes; a failing command raises "smoothing error"."""
+    import sys
+    from lyspeq_amd.driver import smooth_power
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_driver_smoother import _grid, _surface
+    kc, zc = _grid()
+    truth = _surface(kc, zc)
+    pfid = 0.9 * truth
+    sigma = 0.05 * truth
+    rng = np.random.default_rng(4)
+    theta = truth + sigma * rng.standard_normal(truth.size) - pfid
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cmd = f"PYTHONPATH={root} {sys.executable} -m lyspeq_amd.smooth"
+    env = dict(os.environ, TMPDIR=str(tmp_path))
+    for lg in (0, 1):
+        inp, out = tmp_path / f"s{lg}.in", tmp_path / f"s{lg}.out"
+        with open(inp, "wb") as f:
+            f.write(np.array([zc.size, kc.size, lg], dtype="<i4").tobytes())
+            for a in (zc, kc, theta, sigma, pfid):
+                f.write(np.asarray(a, dtype="<f8").tobytes())
+        r = subprocess.run([B.DRIVER_CHECK, "smooth", str(inp), str(out), cmd], capture_output=True, text=True,
+                           timeout=120, env=env)
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(out, dtype="<f8")
+        want = smooth_power(theta, sigma, kc, zc, pfid, interp_log=bool(lg))
+        assert np.array_equal(got, want)
+    assert not any(p.name.startswith("tmp-") for p in tmp_path.iterdir())   # the files are removed
+    r = subprocess.run([B.DRIVER_CHECK, "smooth", str(inp), str(out), "false"], capture_output=True, text=True,
+                       timeout=60, env=env)
+    assert r.returncode == 3 and "smoothing error" in r.stderr
+
+
+def test_power_estimates_at_the_desi_band_count_against_oracle(tmp_path):
+    """T = 455 (35 k bands x 13 z bins) as the DESI-like workload has it:
+    the Fisher sum of chunks that each cover up to 2 z bins and a band range
+    that ends at their KBIN_UPP, so high-k bands of some z bins are empty and
+    the matrix is banded by z bin.  The C++ arithmetic (normalise, stable
+    inversion with the empty-diagonal rule and the singular-value test,
+    theta, sigma, the weighted d/b/t) against the oracle's restatement."""
+    rng = np.random.default_rng(455)
+    nk, nz = 35, 13
+    T = nk * nz
+    F = np.zeros((T, T))
+    dbt = np.zeros((3, T))
+    for _ in range(600):
+        z0 = int(rng.integers(0, nz - 1))
+        kup = int(rng.integers(10, nk + 1)) if z0 < nz - 3 else int(rng.integers(5, 12))
+        idx = np.concatenate([np.arange(z0 * nk, z0 * nk + kup), np.arange((z0 + 1) * nk, (z0 + 1) * nk + kup)])
+        A = rng.standard_normal((idx.size, 3 * idx.size)) / idx.size
+        blk = A @ A.T
+        F[np.ix_(idx, idx)] += blk
+        dbt[:, idx] += rng.standard_normal((3, idx.size)) * np.sqrt(np.diag(blk))
+    F = 2.0 * np.triu(F)   # the unhalved upper triangle the chunks accumulate
+    empty = int(np.sum(np.diag(F) == 0))
+    assert empty > 0
+    p = _power(tmp_path, F, dbt)
+    o = power_estimates(F, dbt)
+    assert p["dof"] == o["dof"] == T - empty
+    assert abs(p["damp"] - o["damp"]) <= 1e-12 * max(1.0, abs(o["damp"]))
+    for key in ("fisher", "solver", "inverse", "theta", "weighted"):
+        assert _rel(p[key], o[key]) < 1e-10, key
+    ok = o["sigma"] > 0
+    assert np.all(np.abs(p["sigma"][ok] - o["sigma"][ok]) <= 1e-10 * o["sigma"][ok])

@@ -167,3 +167,41 @@ def test_chunk_beyond_the_scratch_budget_is_refused_at_registration():
     eng.iterate(np.zeros(T), F, d)
     assert np.abs(F).max() > 0 and eng.chunk_result(cid)[2] == 0 and cid == 0
     eng.close()
+
+
+def test_redshift_bin_limit_of_a_chunk():
+    """A chunk may span at most 8 redshift bins in this build (the kernels
+    pack a chunk's z-bin weights for 8 bins).  With bins of width 0.03 the
+    CPU configuration's chunk spans exactly 8 (triangular binning) and must
+    meet the FP64 bar against the oracle; with width 0.015 it spans 14 and
+    registration refuses it with QML_ERR_LIMIT, loudly: the reference would
+    process it, so the chunk is not skipped in silence."""
+    import dataclasses
+    _print_case("z-bin limit")
+    w = synth.cpu_reference_config(nchunks=2)
+    c = w.chunks[0]
+    tabs = P.oracle_tables(w)
+    for width, nz, spans in ((0.03, 9, 8), (0.015, 17, 14)):
+        b = dataclasses.replace(w.binning, z0=2.30, z_bin_width=width, nz=nz)
+        ob = P.oracle_bins(b)
+        och = OracleChunk(ob, tabs, c["wave"], c["delta"], c["ivar"], c["R"])
+        assert och.n_q == spans * ob.nk   # N_Q_MATRICES: the z bins spanned times the k bands
+        eng = QMLEngine(b, precision="fp64")
+        for i, (R, dv) in enumerate(w.pairs):
+            eng.add_table(R, dv, w.deriv[i], w.signal[i], w.v_length)
+        if spans > 8:
+            with pytest.raises(QMLError) as e:
+                eng.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
+            assert e.value.code == -7 and "more than 8 redshift bins" in str(e.value)
+            eng.close()
+            continue
+        cid, info = eng.add_chunk(c["wave"], c["delta"], c["ivar"], c["R"])
+        eng.commit()
+        T = eng.T
+        th = np.zeros(T)
+        F, d = np.zeros((T, T)), np.zeros((3, T))
+        eng.iterate(th, F, d)
+        eng.close()
+        Fo, do = np.zeros((T, T)), np.zeros((3, T))
+        assert och.one_iteration(th, Fo, do)
+        P.assert_fp64(F, d, Fo, do, f"chunk over {spans} z bins")

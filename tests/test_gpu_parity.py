@@ -314,3 +314,26 @@ def test_redshift_growth_mixed_precision_against_fp64():
     for precision in ("fp32", "bf16x3"):
         _, _, Fm, dm = P.run_engine(w, np.zeros(T), idx, precision=precision)
         P.assert_mixed(Fm, dm, F64, d64, precision, f"redshift growth {precision}", "growth")
+
+
+def test_gram_offsets_beyond_two_to_the_31_elements():
+    """A high-resolution chunk of N = 3783 pixels (Np = 3840, 60 tiles of 64,
+    1830 packed upper tiles) with 200 stored Q (m = 202 members): the
+    split-BF16 Gram reads member rows of 2 x 1830 x 4096 bf16, so its row
+    offsets reach (m - 1) x 1830 x 8192 ~ 3.0e9 elements, beyond 32 bits.
+    BF16x3 against the FP64 path (64-bit offsets throughout) on the same
+    chunk, at the high-resolution workload's mixed-precision bar; wrapped
+    offsets would read other members' tiles."""
+    print("[case] Gram offsets beyond 2^31 elements (N = 3783, m = 202)", flush=True)
+    w = synth.hires_config(nchunks=6, n_range=(3760, 3800))
+    assert len(w.chunks[2]["wave"]) == 3783
+    T = w.binning.total_kz_bins
+    th = np.zeros(T)
+    e64, o64, F64, d64 = P.run_engine(w, th, [2], precision="fp64", scratch_gb=96)
+    e64.close()
+    em, om, Fm, dm = P.run_engine(w, th, [2], precision="bf16x3", scratch_gb=96)
+    info = em.schedule()
+    em.close()
+    assert list(o64) == [2] and list(om) == [2] and o64[2][2] and om[2][2]
+    print(f"largest chunk {info['max_chunk_gb']:.1f} GB", flush=True)
+    P.assert_mixed(Fm, dm, F64, d64, "bf16x3", "N = 3783, m = 202", workload="hires")

@@ -262,6 +262,45 @@ int qml_get_chunk_result(qml_ctx *ctx, int chunk_id, double *fisher, double *dbt
 
 int qml_get_timings(qml_ctx *ctx, qml_timings *t);
 
+/* The matrices of Chunk's protected builders for one committed chunk,
+ * computed on the device from the chunk's registered pixels and tables and
+ * written to out (row-major n x n, both triangles; the reference fills the
+ * upper one and mirrors the matrices it reads in full).  which:
+ *   QML_CMAT_V        v_ij = c ln(l_j / l_i) for j >= i   Chunk::_setVZMatrices
+ *   QML_CMAT_Z        z_ij = sqrt(l_i l_j) - 1              (core/chunk_estimate.cpp:384-398)
+ *                     on the matrix grid: the oversampled grid of an
+ *                     oversampled resolution matrix (n = its column count,
+ *                     core/chunk_estimate.cpp:929-944), else the pixels
+ *   QML_CMAT_SFID     the fiducial signal matrix, R S R^T with a resolution
+ *                     matrix          Chunk::_setFiducialSignalMatrix (:400-426)
+ *   QML_CMAT_Q        Q of chunk-local index arg = i_kz before the redshift
+ *                     weights, R Q R^T with a resolution matrix
+ *                                     Chunk::_setQiMatrix (:429-480)
+ *   QML_CMAT_ZWEIGHT  the weights w_zm(z_ij) of absolute z bin arg = zm that
+ *                     Chunk::_applyRedshiftInterp multiplies in (:483-536)
+ *   QML_CMAT_GROWTH   ((1 + z_ij) / (1 + z_zm))^B of z bin arg = zm, B the
+ *                     bins' growth exponent
+ *                     Chunk::_scaleDerivativesWithRedshiftGrowth (:539-573)
+ * The pixel-grid matrices have n = the chunk's size.  With out NULL only *n
+ * is set.  QML_ERR_ARG for a chunk without signal table (SFID), an i_kz
+ * outside the chunk or of a band at or above KBIN_UPP (Q), or a z bin
+ * outside the bins.  Not for use while an iteration is running on ctx. */
+#define QML_CMAT_V 0
+#define QML_CMAT_Z 1
+#define QML_CMAT_SFID 2
+#define QML_CMAT_Q 3
+#define QML_CMAT_ZWEIGHT 4
+#define QML_CMAT_GROWTH 5
+int qml_chunk_matrix(qml_ctx *ctx, int chunk_id, int which, int arg, double *out, int *n);
+
+/* Chunk::_addMarginalizations (core/chunk_estimate.cpp:596-665) on the
+ * device: cinv (the chunk's size squared, row-major; the upper triangle is
+ * read) loses each of the chunk's orthonormal continuum templates by a
+ * Sherman-Morrison update, cinv <- cinv - y y^T / (u^T y) with y = cinv u,
+ * in place.  The templates are those the chunk was registered with (the
+ * SVD-orthonormalised set, small singular values dropped). */
+int qml_chunk_marginalize(qml_ctx *ctx, int chunk_id, double *cinv);
+
 /* Leaves chunk chunk_id out of (active = 0) or back in (1) the sums that
  * qml_iterate accumulates; its own results are still computed.  The
  * counterpart of the reference dropping a Chunk object from

@@ -917,7 +917,85 @@ __global__ __launch_bounds__(256) void k_accumulate(const ChunkOut *outs, const 
     }
 }
 
+// ------------------------------------------------ the builders, one matrix --
+// One n x n matrix of Chunk's protected builders for chunk *cp, row-major
+// into out: v or z on the matrix grid (fine = the oversampled grid), S_fid,
+// the unweighted Q of band kn, or the z bin zm weights or growth factors on
+// the pixel grid.  With a resolution matrix S and Q are read from the
+// chunk's materialised R Shat R^T, R Qhat R^T (qb_off, row stride Np).
+__global__ __launch_bounds__(256) void k_chunk_matrix(const ChunkDev *cp, const double *in, const double *rin,
+                                                      const TableDev *tabs, BinsDev bins, const double *mat,
+                                                      int which, int fine, int kn, int zm, double *out) {
+    const ChunkDev c = *cp;
+    const int n = fine ? c.rM : c.N;
+    const double *lam = fine ? rin + c.rin_off + (size_t)c.N * c.rnb : in + c.in_off + IN_WAVE * (size_t)c.Np;
+    const size_t total = (size_t)n * n, np2 = (size_t)c.Np * c.Np;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        double v, z, r = 0.0;
+        vz_pair(lam[i], lam[j], i, j, v, z);
+        if (which == 0) {
+            r = v;
+        } else if (which == 1) {
+            r = z;
+        } else if (which == 2 || which == 3) {
+            const size_t ij = (size_t)i * c.Np + j;
+            if (c.rm) {
+                r = mat[c.qb_off + (size_t)(which == 2 ? c.kbu : kn) * np2 + ij];
+            } else if (which == 2) {
+                r = signal_interp(tabs[c.table], z, v);
+            } else {
+                // DiscreteInterpolation1D::evaluate (mathtools/discrete_interpolation.cpp:61-72)
+                const TableDev t = tabs[c.table];
+                const double xx = (v - t.v1) / t.dv;
+                const int iv = interp_index(xx, t.nv);
+                const double dn = xx - iv, y1 = t.qt[(size_t)kn * t.nv + iv], y2 = t.qt[(size_t)kn * t.nv + iv + 1];
+                r = y1 * (1 - dn) + y2 * dn;
+            }
+        } else if (which == 4) {
+            r = zbin_weight(bins, z, zm);
+        } else {
+            r = growth_factor(z, bins.zc[zm], bins.growth_b);
+        }
+        out[e] = r;
+    }
+}
+
+// Sherman-Morrison removal of template u from the n x n row-major cinv
+// (_remShermanMorrison, core/chunk_estimate.cpp:596-603): y = cinv u from
+// the upper triangle (one block per row), then cinv += -y y^T / (u . y).
+__global__ __launch_bounds__(256) void k_marg_symv(const double *cinv, const double *u, int n, double *y) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256)
+        s += ((j >= i) ? cinv[(size_t)i * n + j] : cinv[(size_t)j * n + i]) * u[j];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) y[i] = s;
+}
+__global__ __launch_bounds__(256) void k_marg_update(double *cinv, const double *u, const double *y, int n) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) s += u[j] * y[j];
+    const double norm = -1.0 / block_sum(s, red);
+    const int i = blockIdx.x;
+    const double yi = norm * y[i];
+    for (int j = threadIdx.x; j < n; j += 256) cinv[(size_t)i * n + j] += yi * y[j];
+}
+
 // ------------------------------------------------------------------ launch --
+void launch_chunk_matrix(hipStream_t st, const ChunkDev *chunk, const double *in, const double *rin,
+                         const TableDev *tabs, const BinsDev &bins, const double *mat, int which, int fine, int kn,
+                         int zm, int n, double *out) {
+    const size_t total = (size_t)n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
+    k_chunk_matrix<<<blocks, 256, 0, st>>>(chunk, in, rin, tabs, bins, mat, which, fine, kn, zm, out);
+}
+void launch_marginalize(hipStream_t st, double *cinv, const double *u, int n, double *y) {
+    k_marg_symv<<<n, 256, 0, st>>>(cinv, u, n, y);
+    k_marg_update<<<n, 256, 0, st>>>(cinv, u, y, n);
+}
+
 void launch_assemble(const LaunchCtx &L, const double *theta) {
     if (L.nvu_max > 0) {   // chunks with a resolution matrix have no table rows
         dim3 gt((L.nvu_max + 255) / 256, L.nchunks);

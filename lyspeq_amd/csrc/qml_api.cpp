@@ -51,6 +51,7 @@ struct ChunkHost {
     long long in_off = 0;
     // resolution matrix: rows of R (N x rnb) and the fine grid (rM values)
     int rm = 0, rnb = 0, ros = 1, rcol0 = 0, rM = 0;
+    int rdia = 0;   // the matrix came in diagonal storage (its grid is the pixels')
     std::vector<double> rband, fine;
     long long rin_off = 0;
 };
@@ -447,6 +448,7 @@ static int add_chunk_impl(qml_ctx *c, const qml_chunk_in *in, const qml_rmat *rm
         ch.rm = 1;
         ch.rnb = nel;
         ch.rband.assign((size_t)N * nel, 0.0);
+        ch.rdia = rm->kind == QML_RMAT_DIA;
         if (rm->kind == QML_RMAT_DIA) {
             // row i, slot t <-> column i - nel/2 + t on diagonal nel-1-t; the
             // grid gets nel/2 points beyond each end so that every slot lies
@@ -951,6 +953,33 @@ namespace {
 // c->st (what a stream capture needs).  With timing every stage boundary
 // records its event; otherwise only the events that order the two streams are
 // recorded.
+// The launch parameters of batch bi on stream st, with its arena set bi % 2.
+LaunchCtx batch_launch(const qml_ctx *c, size_t bi) {
+    const Batch &B = c->batches[bi];
+    const int set = (int)(bi % 2);
+    LaunchCtx L;
+    L.stream = c->st;
+    L.chunks = B.d_chunks;
+    L.nchunks = (int)B.ids.size();
+    L.np_max = B.np_max;
+    L.nvu_max = B.nvu_max;
+    L.nu_max = B.nu_max;
+    L.dp_rb = B.dp_rb;
+    L.nq_max = B.nq_max;
+    L.dp_cb = B.dp_cb;
+    L.rin = c->d_rin;
+    L.rm_kbu = B.rm_kbu;
+    L.rm_lay = B.rm_lay;
+    L.in = c->d_in;
+    L.tables = c->d_tables;
+    L.bins = c->binsdev;
+    L.mat = c->d_mat[set];
+    L.vec = c->d_vec[set];
+    L.pj = c->d_pj[set];
+    L.sj = c->d_sj[set];
+    return L;
+}
+
 int enqueue_iteration(qml_ctx *c, bool timing) {
     const int T = c->T;
     hipStream_t sB = c->pipeline ? c->st2 : c->st;   // stream of the dense-product stage
@@ -963,26 +992,7 @@ int enqueue_iteration(qml_ctx *c, bool timing) {
     for (size_t bi = 0; bi < c->batches.size(); ++bi) {
         Batch &B = c->batches[bi];
         const int set = (int)(bi % 2);
-        LaunchCtx L;
-        L.stream = c->st;
-        L.chunks = B.d_chunks;
-        L.nchunks = (int)B.ids.size();
-        L.np_max = B.np_max;
-        L.nvu_max = B.nvu_max;
-        L.nu_max = B.nu_max;
-        L.dp_rb = B.dp_rb;
-        L.nq_max = B.nq_max;
-        L.dp_cb = B.dp_cb;
-        L.rin = c->d_rin;
-        L.rm_kbu = B.rm_kbu;
-        L.rm_lay = B.rm_lay;
-        L.in = c->d_in;
-        L.tables = c->d_tables;
-        L.bins = c->binsdev;
-        L.mat = c->d_mat[set];
-        L.vec = c->d_vec[set];
-        L.pj = c->d_pj[set];
-        L.sj = c->d_sj[set];
+        const LaunchCtx L = batch_launch(c, bi);
         // stage A on st: this arena set was last read by stage B of batch bi - 2
         if (bi >= 2) HIPCHK(hipStreamWaitEvent(c->st, c->batches[bi - 2].evB[5], 0));
         if (c->poison) launch_poison_lds(c->st);
@@ -1159,6 +1169,94 @@ int qml_get_chunk_result(qml_ctx *c, int id, double *fisher, double *dbt, int *s
         HIPCHK(hipMemcpy(fisher, c->d_fc + o.fc_off, (size_t)o.nq * o.nq * sizeof(double), hipMemcpyDeviceToHost));
     if (dbt) HIPCHK(hipMemcpy(dbt, c->d_dbt + o.dbt_off, (size_t)3 * o.nq * sizeof(double), hipMemcpyDeviceToHost));
     if (status) HIPCHK(hipMemcpy(status, c->d_status + id, sizeof(int), hipMemcpyDeviceToHost));
+    return QML_OK;
+}
+
+int qml_chunk_matrix(qml_ctx *c, int id, int which, int arg, double *out, int *n) {
+    if (!c || !c->committed || id < 0 || id >= (int)c->chunks.size())
+        return fail(QML_ERR_ARG, "qml_chunk_matrix: bad chunk id or not committed");
+    const ChunkHost &ch = c->chunks[id];
+    const int nk = c->binsdev.nk;
+    // v and z live on the matrix grid: the oversampled one of an oversampled
+    // resolution matrix (a diagonal matrix keeps the pixels' grid)
+    const bool fine = (which == QML_CMAT_V || which == QML_CMAT_Z) && ch.rm && !ch.rdia;
+    int kn = 0, zm = 0;
+    switch (which) {
+        case QML_CMAT_V:
+        case QML_CMAT_Z: break;
+        case QML_CMAT_SFID:
+            if (c->bins.turn_off_sfid || (!ch.rm && !c->tables[ch.info.res_index].d_s))
+                return fail(QML_ERR_ARG, "qml_chunk_matrix: the fiducial signal is off or the table has none");
+            break;
+        case QML_CMAT_Q:
+            if (arg < 0 || arg >= ch.nq) return fail(QML_ERR_ARG, "qml_chunk_matrix: i_kz outside the chunk");
+            kn = (arg + ch.fis) % nk;
+            if (kn >= ch.kbu) return fail(QML_ERR_ARG, "qml_chunk_matrix: the band of i_kz is not stored (KBIN_UPP)");
+            break;
+        case QML_CMAT_ZWEIGHT:
+        case QML_CMAT_GROWTH:
+            if (arg < 0 || arg >= c->binsdev.nz) return fail(QML_ERR_ARG, "qml_chunk_matrix: z bin outside the bins");
+            zm = arg;
+            break;
+        default: return fail(QML_ERR_ARG, "qml_chunk_matrix: unknown matrix");
+    }
+    const int nn = fine ? ch.rM : ch.N;
+    if (n) *n = nn;
+    if (!out) return QML_OK;
+    if (!c->plan_ok) return fail(QML_ERR_MEMORY, "qml_chunk_matrix: no batch plan");
+    size_t bi = 0;
+    int li = -1;
+    for (size_t b = 0; b < c->batches.size() && li < 0; ++b) {
+        const std::vector<int> &ids = c->batches[b].ids;
+        const auto it = std::find(ids.begin(), ids.end(), id);
+        if (it != ids.end()) {
+            bi = b;
+            li = (int)(it - ids.begin());
+        }
+    }
+    if (li < 0) return fail(QML_ERR_ARG, "qml_chunk_matrix: the chunk is in no batch");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->st2) HIPCHK(hipStreamSynchronize(c->st2));
+    const LaunchCtx L = batch_launch(c, bi);
+    // R Shat R^T and the R Qhat_k R^T of the batch's resolution-matrix chunks
+    // into their arena matrices, as the iteration's first step makes them
+    if (ch.rm && (which == QML_CMAT_SFID || which == QML_CMAT_Q) && !launch_rmat(L))
+        return fail(QML_ERR_LIMIT, "qml_chunk_matrix: no sandwich kernel for the resolution matrices");
+    double *d_out = nullptr;
+    const size_t bytes = (size_t)nn * nn * sizeof(double);
+    HIPCHK(hipMalloc(&d_out, bytes));
+    launch_chunk_matrix(c->st, c->batches[bi].d_chunks + li, c->d_in, c->d_rin, c->d_tables, c->binsdev, L.mat, which,
+                        fine ? 1 : 0, kn, zm, nn, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    hipFree(d_out);
+    if (e != hipSuccess) return fail(QML_ERR_HIP, std::string("qml_chunk_matrix: ") + hipGetErrorString(e));
+    return QML_OK;
+}
+
+int qml_chunk_marginalize(qml_ctx *c, int id, double *cinv) {
+    if (!c || !c->committed || id < 0 || id >= (int)c->chunks.size() || !cinv)
+        return fail(QML_ERR_ARG, "qml_chunk_marginalize: bad chunk id, null matrix or not committed");
+    const ChunkHost &ch = c->chunks[id];
+    if (ch.nu == 0) return QML_OK;
+    const int N = ch.N;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->st2) HIPCHK(hipStreamSynchronize(c->st2));
+    double *d = nullptr;
+    const size_t bytes = ((size_t)N * N + N) * sizeof(double);
+    HIPCHK(hipMalloc(&d, bytes));
+    hipError_t e = hipMemcpyAsync(d, cinv, (size_t)N * N * sizeof(double), hipMemcpyHostToDevice, c->st);
+    for (int t = 0; t < ch.nu && e == hipSuccess; ++t) {
+        launch_marginalize(c->st, d, c->d_in + ch.in_off + (size_t)(IN_U0 + t) * ch.Np, N, d + (size_t)N * N);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(cinv, d, (size_t)N * N * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    hipFree(d);
+    if (e != hipSuccess) return fail(QML_ERR_HIP, std::string("qml_chunk_marginalize: ") + hipGetErrorString(e));
     return QML_OK;
 }
 

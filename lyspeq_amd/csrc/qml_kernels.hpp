@@ -61,6 +61,14 @@ void launch_gram_finalize(const LaunchCtx &L, const ChunkOut *outs, const double
 // Debug aid: fills the LDS of every CU with all-ones bytes (NaN in FP64,
 // FP32 and bf16), so that a kernel reading LDS it did not write shows it.
 void launch_poison_lds(hipStream_t st);
+// One matrix of Chunk's protected builders for the chunk at *chunk (n x n,
+// row-major into out; which as QML_CMAT_*, fine: v/z on the oversampled grid).
+void launch_chunk_matrix(hipStream_t st, const ChunkDev *chunk, const double *in, const double *rin,
+                         const TableDev *tabs, const BinsDev &bins, const double *mat, int which, int fine, int kn,
+                         int zm, int n, double *out);
+// cinv (n x n, row-major, upper triangle read) minus its Sherman-Morrison
+// projection on template u; y is n doubles of scratch.
+void launch_marginalize(hipStream_t st, double *cinv, const double *u, int n, double *y);
 // Adds the results of chunks with status 0 and skip 0, in chunk-id order.
 void launch_accumulate(hipStream_t st, const ChunkOut *outs, const int *status, const int *skip, int nchunks,
                        int T, const double *fc, const double *dbt, double *Fsum, double *dbt_sum);

@@ -12,11 +12,16 @@
 //
 // The protected matrix builders of the reference (_initMatrices,
 // _setVZMatrices, _setFiducialSignalMatrix, _setQiMatrix,
-// _applyRedshiftInterp, _addMarginalizations and the matrix pointers) have no
-// counterpart: the matrices are built and kept on the device.  The
-// bookkeeping the constructor does (_copyQSOFile, _findRedshiftBin,
-// _setNQandFisherIndex, _setStoredMatrices, stored_ikz_qi with its indices)
-// is here.
+// _applyRedshiftInterp, _scaleDerivativesWithRedshiftGrowth,
+// _addMarginalizations) keep their names and the host buffers they fill
+// (covariance_matrix, temp_matrix, stored_ikz_qi, _vmatrix, _zmatrix, from
+// glmemory); the matrices are computed on the device from the chunk's
+// registered pixels and tables (qml_chunk_matrix, qml_chunk_marginalize) and
+// copied into those buffers.  The first such call commits the engine, after
+// which no chunk can be constructed.  oneQSOiteration does not use them: the
+// iteration keeps its matrices on the device.  The bookkeeping the
+// constructor does (_copyQSOFile, _findRedshiftBin, _setNQandFisherIndex,
+// _setStoredMatrices, stored_ikz_qi with its indices) is here too.
 #pragma once
 
 #include <memory>
@@ -27,21 +32,68 @@
 
 namespace lyspeq_amd {
 class Chunk;
+struct QSOSpectrum;
+struct ResolutionRows;
 }
+
+// Host buffers of the protected builders (core/chunk_estimate.hpp:29-43):
+// setMaxSizes records the largest chunk (every Chunk constructor calls it),
+// allocMemory allocates covariance_matrix, two temp_matrices, stored_sfid
+// (unless TURN_OFF_SFID) and max_nqdim stored_ikz_qi of max_size^2, the
+// size vectors, and with an oversampled resolution matrix matrix_lambda,
+// finer_matrix, v_matrix and z_matrix on its grid; dealloc frees them.  The
+// interpolation tables of the reference live on the device.
+namespace glmemory {
+extern int max_size_2;
+extern std::unique_ptr<double[]> covariance_matrix, temp_vector, weighted_data_vector, stored_sfid, matrix_lambda,
+    finer_matrix, v_matrix, z_matrix;
+extern std::vector<std::unique_ptr<double[]>> stored_ikz_qi, temp_matrices;
+
+void setMaxSizes(int size, int matrix_n, int nqdim, bool onsamp);
+void allocMemory();
+void dealloc();
+void updateMemUsed(double mem);
+}  // namespace glmemory
 
 class Chunk {
 protected:
     int DATA_SIZE_2;
     int _matrix_n, RES_INDEX;
     double LOWER_REDSHIFT, UPPER_REDSHIFT, MEDIAN_REDSHIFT, BIN_REDSHIFT;
+    // the matrix grid (finer with an oversampled resolution matrix) and the
+    // caller's C^-1 for _addMarginalizations; neither is owned
+    double *_matrix_lambda = nullptr, *inverse_covariance_matrix = nullptr;
     bool on_oversampling;
-    // (i_kz, matrix): the stored Q indices; the matrices live on the device
+    double *_vmatrix = nullptr, *_zmatrix = nullptr;
+    // DATA_SIZE x DATA_SIZE host buffers from glmemory (_initMatrices)
+    double *covariance_matrix = nullptr;
+    std::vector<double *> temp_matrix;
+    // (i_kz, matrix): the stored Q indices and, after _initMatrices, their buffers
     std::vector<std::pair<int, double *>> stored_ikz_qi;
 
     void _copyQSOFile(const qio::QSOFile &qmaster, int i1, int i2);
     void _findRedshiftBin();
     void _setNQandFisherIndex();
     void _setStoredMatrices();
+
+    // Points the matrices at glmemory's buffers (allocMemory first) and
+    // registers the chunk with the engine if a subclass built it by hand.
+    void _initMatrices();
+    // sm = S_fid (R S R^T with a resolution matrix), size^2
+    void _setFiducialSignalMatrix(double *sm);
+    // _vmatrix, _zmatrix on the matrix grid (_matrix_n^2)
+    void _setVZMatrices();
+    // qi = Q of chunk index i_kz before the redshift weights (a copy of
+    // stored_ikz_qi[kn] beyond the first z bin, as the reference)
+    void _setQiMatrix(double *qi, int i_kz);
+    // Multiplies each stored Q by its z bin's weights w_zm(z_ij) (uses temp_matrix[0])
+    void _applyRedshiftInterp();
+    // Multiplies each stored Q by ((1 + z_ij) / (1 + z_m))^B (uses temp_matrix[0])
+    void _scaleDerivativesWithRedshiftGrowth();
+    // inverse_covariance_matrix minus its projections on the continuum templates
+    void _addMarginalizations();
+
+    friend class TestOneQSOEstimate;
 
     Chunk() : DATA_SIZE_2(0), _matrix_n(0), N_Q_MATRICES(0) {}
 
@@ -86,6 +138,11 @@ public:
 
 private:
     std::unique_ptr<lyspeq_amd::Chunk> impl_;
+    // the master's pixels [i1, i2) (and resolution rows) from _copyQSOFile,
+    // until the engine chunk is registered from them
+    std::unique_ptr<lyspeq_amd::QSOSpectrum> src_;
+    std::unique_ptr<lyspeq_amd::ResolutionRows> src_rmat_;
     const double *ps_ = nullptr;
     int size_ = 0, real_size_ = 0;
+    lyspeq_amd::Chunk &engine_chunk();
 };

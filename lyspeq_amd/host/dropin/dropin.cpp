@@ -263,6 +263,69 @@ int max_k_index(double knyq) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------- glmemory --
+// The reference's global buffers (core/chunk_estimate.cpp:45-139), for the
+// protected builders; two temp matrices, the reference's minimum.
+namespace glmemory {
+int max_size = 0, max_size_2 = 0, max_matrix_n = 0, max_nqdim = 0;
+bool on_oversampling = false;
+double memUsed = 0;
+std::unique_ptr<double[]> covariance_matrix, temp_vector, weighted_data_vector, stored_sfid, matrix_lambda,
+    finer_matrix, v_matrix, z_matrix;
+std::vector<std::unique_ptr<double[]>> stored_ikz_qi, temp_matrices;
+
+void setMaxSizes(int size, int matrix_n, int nqdim, bool onsamp) {
+    max_size = std::max(size, max_size);
+    max_size_2 = max_size * max_size;
+    max_matrix_n = std::max(matrix_n, max_matrix_n);
+    max_nqdim = std::max(nqdim, max_nqdim);
+    on_oversampling |= onsamp;
+}
+
+void allocMemory() {
+    const size_t s2 = (size_t)max_size * max_size, hs = (size_t)max_matrix_n * max_matrix_n;
+    constexpr int ntemp = 2;
+    const double mb = 8.0 / 1048576.;
+    double mem = mb * (2.0 * max_size + (1 + ntemp + !specifics::TURN_OFF_SFID + max_nqdim) * (double)s2);
+    if (on_oversampling) mem += mb * (max_matrix_n + 3.0 * hs);
+    updateMemUsed(mem);
+    temp_vector = std::make_unique<double[]>(max_size);
+    weighted_data_vector = std::make_unique<double[]>(max_size);
+    covariance_matrix = std::make_unique<double[]>(s2);
+    temp_matrices.clear();
+    for (int i = 0; i < ntemp; ++i) temp_matrices.push_back(std::make_unique<double[]>(s2));
+    if (!specifics::TURN_OFF_SFID) stored_sfid = std::make_unique<double[]>(s2);
+    stored_ikz_qi.clear();
+    for (int i = 0; i < max_nqdim; ++i) stored_ikz_qi.push_back(std::make_unique<double[]>(s2));
+    if (on_oversampling) {
+        matrix_lambda = std::make_unique<double[]>(max_matrix_n);
+        finer_matrix = std::make_unique<double[]>(hs);
+        v_matrix = std::make_unique<double[]>(hs);
+        z_matrix = std::make_unique<double[]>(hs);
+    }
+}
+
+void dealloc() {
+    temp_vector.reset();
+    weighted_data_vector.reset();
+    covariance_matrix.reset();
+    temp_matrices.clear();
+    stored_sfid.reset();
+    stored_ikz_qi.clear();
+    matrix_lambda.reset();
+    finer_matrix.reset();
+    v_matrix.reset();
+    z_matrix.reset();
+    process::updateMemory(memUsed);
+    memUsed = 0;
+}
+
+void updateMemUsed(double mem) {
+    memUsed += mem;
+    process::updateMemory(-mem);
+}
+}  // namespace glmemory
+
 // ------------------------------------------------------------------- Chunk --
 Chunk::Chunk(const qio::QSOFile &qmaster, int i1, int i2) : DATA_SIZE_2(0), _matrix_n(0), N_Q_MATRICES(0) {
     _copyQSOFile(qmaster, i1, i2);
@@ -285,21 +348,51 @@ Chunk::Chunk(const qio::QSOFile &qmaster, int i1, int i2) : DATA_SIZE_2(0), _mat
     }
     fisher_matrix = std::make_unique<double[]>((size_t)N_Q_MATRICES * N_Q_MATRICES);
     std::fill_n(fisher_matrix.get(), (size_t)N_Q_MATRICES * N_Q_MATRICES, 0.0);
+    glmemory::setMaxSizes(size(), _matrix_n, (int)stored_ikz_qi.size(), on_oversampling);
 
-    // the engine's chunk: the same pixels [i1, i2) of the master spectrum in
-    // Angstrom (it cuts the masked boundary and decides the same bins), with
-    // the master's preprocessed resolution matrix
-    lyspeq_amd::QSOSpectrum q;
-    q.wave.assign(qmaster.wave() + i1, qmaster.wave() + i2);
-    q.delta.assign(qmaster.delta() + i1, qmaster.delta() + i2);
-    q.ivar.assign(qmaster.ivar() + i1, qmaster.ivar() + i2);
-    q.R_fwhm = qmaster.R_fwhm;
-    q.fname = qmaster.fname;
-    lyspeq_amd::ResolutionRows rr;
-    const lyspeq_amd::ResolutionRows *rrp = nullptr;
+    engine_chunk();
+    if (impl_->fisher_index_start != fisher_index_start || impl_->N_Q_MATRICES != N_Q_MATRICES ||
+        impl_->KBIN_UPP != KBIN_UPP)
+        throw std::logic_error("Chunk: the engine's bins differ from the constructor's");
+    impl_->result_fisher = fisher_matrix.get();
+    for (int k = 0; k < 3; ++k) impl_->result_dbt[k] = dbt_estimate_before_fisher_vector[k].get();
+    size_ = qFile->size();
+    real_size_ = qFile->realSize();
+}
+
+// The engine's chunk, registered on first use from the pixels _copyQSOFile
+// kept: the master's [i1, i2) in Angstrom (the engine cuts the masked
+// boundary and decides the same bins) with the master's preprocessed
+// resolution matrix.
+lyspeq_amd::Chunk &Chunk::engine_chunk() {
+    if (!impl_) {
+        if (!src_) throw std::logic_error("Chunk: no spectrum to register (_copyQSOFile was not called)");
+        impl_ = std::make_unique<lyspeq_amd::Chunk>(engine(), *src_, 0, src_->size(), src_rmat_.get());
+        src_.reset();
+        src_rmat_.reset();
+    }
+    return *impl_;
+}
+
+Chunk::~Chunk() { process::updateMemory(getMinMemUsage()); }
+
+// Chunk::_copyQSOFile (core/chunk_estimate.cpp:168-214)
+void Chunk::_copyQSOFile(const qio::QSOFile &qmaster, int i1, int i2) {
+    qFile = std::make_unique<qio::QSOFile>(qmaster, i1, i2);
+    // the engine's input: the same pixels of the master (wave in Angstrom)
+    // and its preprocessed resolution matrix's rows for them
+    src_ = std::make_unique<lyspeq_amd::QSOSpectrum>();
+    src_->wave.assign(qmaster.wave() + i1, qmaster.wave() + i2);
+    src_->delta.assign(qmaster.delta() + i1, qmaster.delta() + i2);
+    src_->ivar.assign(qmaster.ivar() + i1, qmaster.ivar() + i2);
+    src_->R_fwhm = qmaster.R_fwhm;
+    src_->fname = qmaster.fname;
+    src_rmat_.reset();
     if (specifics::USE_RESOLUTION_MATRIX && qmaster.Rmat) {
         const mxhelp::Resolution &R = *qmaster.Rmat;
         const int n = i2 - i1, nel = R.getNElemPerRow(), size = qmaster.size();
+        src_rmat_ = std::make_unique<lyspeq_amd::ResolutionRows>();
+        lyspeq_amd::ResolutionRows &rr = *src_rmat_;
         rr.nelem = nel;
         if (R.isDiaMatrix()) {
             rr.kind = QML_RMAT_DIA;
@@ -311,23 +404,7 @@ Chunk::Chunk(const qio::QSOFile &qmaster, int i1, int i2) : DATA_SIZE_2(0), _mat
             rr.oversampling = (R.getNCols() - nel + 1) / size;
             rr.values.assign(R.matrix() + (size_t)i1 * nel, R.matrix() + (size_t)i2 * nel);
         }
-        rrp = &rr;
     }
-    impl_ = std::make_unique<lyspeq_amd::Chunk>(engine(), q, 0, i2 - i1, rrp);
-    if (impl_->fisher_index_start != fisher_index_start || impl_->N_Q_MATRICES != N_Q_MATRICES ||
-        impl_->KBIN_UPP != KBIN_UPP)
-        throw std::logic_error("Chunk: the engine's bins differ from the constructor's");
-    impl_->result_fisher = fisher_matrix.get();
-    for (int k = 0; k < 3; ++k) impl_->result_dbt[k] = dbt_estimate_before_fisher_vector[k].get();
-    size_ = qFile->size();
-    real_size_ = qFile->realSize();
-}
-
-Chunk::~Chunk() { process::updateMemory(getMinMemUsage()); }
-
-// Chunk::_copyQSOFile (core/chunk_estimate.cpp:168-214)
-void Chunk::_copyQSOFile(const qio::QSOFile &qmaster, int i1, int i2) {
-    qFile = std::make_unique<qio::QSOFile>(qmaster, i1, i2);
     const double lenv = qFile->getLengthV();
     const int minnopix = (int)(lenv / specifics::MAX_PIXEL_LENGTH_V);
     if (qFile->realSize() < minnopix || lenv < specifics::MIN_FOREST_LENGTH_V) {
@@ -388,6 +465,104 @@ void Chunk::_setNQandFisherIndex() {
 // live on the device, whose scratch budget the engine checks when the chunk
 // is registered ("Not all matrices are stored." comes from there).
 void Chunk::_setStoredMatrices() {}
+
+// ------------------------------------------------- the protected builders --
+// Chunk::_initMatrices (core/chunk_estimate.cpp:917-957)
+void Chunk::_initMatrices() {
+    if (!glmemory::covariance_matrix || glmemory::max_size_2 < DATA_SIZE_2 || glmemory::temp_matrices.size() < 2 ||
+        glmemory::stored_ikz_qi.size() < stored_ikz_qi.size() || (on_oversampling && !glmemory::v_matrix))
+        throw std::logic_error("Chunk::_initMatrices: glmemory::allocMemory has not been called for this chunk");
+    covariance_matrix = glmemory::covariance_matrix.get();
+    temp_matrix.resize(glmemory::temp_matrices.size());
+    for (size_t i = 0; i < temp_matrix.size(); ++i) temp_matrix[i] = glmemory::temp_matrices[i].get();
+    for (size_t i = 0; i < stored_ikz_qi.size(); ++i) stored_ikz_qi[i].second = glmemory::stored_ikz_qi[i].get();
+    if (on_oversampling) {
+        // the oversampled grid, units of lambda_Lya (as the engine's chunk has it)
+        _matrix_lambda = glmemory::matrix_lambda.get();
+        const double fine_dlambda = qFile->dlambda / LYA_REST / specifics::OVERSAMPLING_FACTOR;
+        const int disp = qFile->Rmat->getNElemPerRow() / 2;
+        for (int i = 0; i < _matrix_n; ++i) _matrix_lambda[i] = qFile->wave()[0] + (i - disp) * fine_dlambda;
+        _vmatrix = glmemory::v_matrix.get();
+        _zmatrix = glmemory::z_matrix.get();
+    } else {
+        _matrix_lambda = qFile->wave();
+        _vmatrix = temp_matrix[0];
+        _zmatrix = temp_matrix[1];
+    }
+    engine_chunk();
+}
+
+namespace {
+// one matrix of the engine chunk into out, which holds n^2 values
+void fetch_matrix(lyspeq_amd::Chunk &c, int which, int arg, double *out, int n, const char *who) {
+    if (!out) throw std::logic_error(std::string(who) + ": null matrix (call _initMatrices first)");
+    const int m = engine().chunkMatrix(c, which, arg, nullptr);
+    if (m != n) throw std::logic_error(std::string(who) + ": the engine's matrix size differs from the chunk's");
+    engine().chunkMatrix(c, which, arg, out);
+}
+}  // namespace
+
+// Chunk::_setVZMatrices (core/chunk_estimate.cpp:384-398)
+void Chunk::_setVZMatrices() {
+    fetch_matrix(engine_chunk(), QML_CMAT_V, 0, _vmatrix, _matrix_n, "Chunk::_setVZMatrices");
+    fetch_matrix(engine_chunk(), QML_CMAT_Z, 0, _zmatrix, _matrix_n, "Chunk::_setVZMatrices");
+}
+
+// Chunk::_setFiducialSignalMatrix (core/chunk_estimate.cpp:400-426)
+void Chunk::_setFiducialSignalMatrix(double *sm) {
+    fetch_matrix(engine_chunk(), QML_CMAT_SFID, 0, sm, size(), "Chunk::_setFiducialSignalMatrix");
+}
+
+// Chunk::_setQiMatrix (core/chunk_estimate.cpp:429-480): beyond the first z
+// bin the band's matrix is the stored one of the first
+void Chunk::_setQiMatrix(double *qi, int i_kz) {
+    int kn, zm;
+    bins::getFisherMatrixBinNoFromIndex(i_kz + fisher_index_start, kn, zm);
+    if (i_kz > kn) {
+        if (kn >= (int)stored_ikz_qi.size() || !stored_ikz_qi[kn].second)
+            throw std::logic_error("Chunk::_setQiMatrix: the first z bin's matrix is not set");
+        std::copy_n(stored_ikz_qi[kn].second, DATA_SIZE_2, qi);
+    } else {
+        fetch_matrix(engine_chunk(), QML_CMAT_Q, i_kz, qi, size(), "Chunk::_setQiMatrix");
+    }
+}
+
+namespace {
+// stored_ikz_qi[KBIN_UPP iz ..] *= the z bin's factor matrix, for each z bin
+// of the chunk (core/chunk_estimate.cpp:483-573)
+void scale_stored(lyspeq_amd::Chunk &c, int which, std::vector<std::pair<int, double *>> &stored, int kbin_upp,
+                  int fisher_index_start, double *tmp, int n, const char *who) {
+    if (kbin_upp <= 0) return;
+    const int Nz = (int)stored.size() / kbin_upp;
+    const size_t n2 = (size_t)n * n;
+    for (int iz = 0; iz < Nz; ++iz) {
+        const int zm = fisher_index_start / bins::NUMBER_OF_K_BANDS + iz;
+        fetch_matrix(c, which, zm, tmp, n, who);
+        for (int k = kbin_upp * iz; k < kbin_upp * (iz + 1); ++k) {
+            double *q = stored[k].second;
+            if (!q) throw std::logic_error(std::string(who) + ": stored matrices not set (call _initMatrices first)");
+            for (size_t e = 0; e < n2; ++e) q[e] *= tmp[e];
+        }
+    }
+}
+}  // namespace
+
+void Chunk::_applyRedshiftInterp() {
+    scale_stored(engine_chunk(), QML_CMAT_ZWEIGHT, stored_ikz_qi, KBIN_UPP, fisher_index_start,
+                 temp_matrix.empty() ? nullptr : temp_matrix[0], size(), "Chunk::_applyRedshiftInterp");
+}
+
+void Chunk::_scaleDerivativesWithRedshiftGrowth() {
+    scale_stored(engine_chunk(), QML_CMAT_GROWTH, stored_ikz_qi, KBIN_UPP, fisher_index_start,
+                 temp_matrix.empty() ? nullptr : temp_matrix[0], size(), "Chunk::_scaleDerivativesWithRedshiftGrowth");
+}
+
+// Chunk::_addMarginalizations (core/chunk_estimate.cpp:596-665)
+void Chunk::_addMarginalizations() {
+    if (!inverse_covariance_matrix)
+        throw std::logic_error("Chunk::_addMarginalizations: inverse_covariance_matrix is not set");
+    engine().marginalize(engine_chunk(), inverse_covariance_matrix);
+}
 
 double Chunk::getMinMemUsage() { return process::getMemoryMB((N_Q_MATRICES + 1) * N_Q_MATRICES); }
 

@@ -134,6 +134,18 @@ void Engine::commit() {
 
 void Engine::setGraph(bool on) { check(qml_set_graph(ctx_, on ? 1 : 0)); }
 
+int Engine::chunkMatrix(const Chunk &c, int which, int arg, double *out) {
+    commit();
+    int n = 0;
+    check(qml_chunk_matrix(ctx_, c.id_, which, arg, out, &n));
+    return n;
+}
+
+void Engine::marginalize(const Chunk &c, double *cinv) {
+    commit();
+    check(qml_chunk_marginalize(ctx_, c.id_, cinv));
+}
+
 int Engine::autotune(const double *ps_estimate, const std::vector<int> &caps, int reps, double *best_ms) {
     commit();
     int best = 0;

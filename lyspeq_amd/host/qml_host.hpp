@@ -136,6 +136,14 @@ public:
     // median TOTAL_KZ_BINS values, mad_cov TOTAL_KZ_BINS^2.  Needs no chunks.
     void bootstrapMedian(int nboots, const double *realizations, double *median, double *mad_cov);
 
+    // One matrix of the reference Chunk's protected builders for chunk c
+    // (qml_chunk_matrix; which is a QML_CMAT_* value): writes n x n values to
+    // out unless it is null and returns n.  Commits the engine first.
+    int chunkMatrix(const Chunk &c, int which, int arg, double *out);
+    // Chunk::_addMarginalizations on c's size^2 cinv, in place
+    // (qml_chunk_marginalize).  Commits the engine first.
+    void marginalize(const Chunk &c, double *cinv);
+
     int totalKzBins() const { return bins_.totalKzBins(); }
     const Binning &binning() const { return bins_; }
     qml_ctx *ctx() { return ctx_; }

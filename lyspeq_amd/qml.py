@@ -118,7 +118,11 @@ EXPORTS = ["qml_create", "qml_destroy", "qml_last_error", "qml_set_bins", "qml_a
            "qml_get_schedule", "qml_bootstrap_complete", "qml_bootstrap_median", "qml_bootstrap_median_noalloc"]
 
 
-EXPORTS += ["qml_set_stack_slab", "qml_get_batch", "qml_set_np_buckets"]
+EXPORTS += ["qml_set_stack_slab", "qml_get_batch", "qml_set_np_buckets", "qml_chunk_matrix",
+            "qml_chunk_marginalize"]
+
+# qml_chunk_matrix's matrices (Chunk's protected builders)
+CMAT_V, CMAT_Z, CMAT_SFID, CMAT_Q, CMAT_ZWEIGHT, CMAT_GROWTH = range(6)
 
 
 class _Schedule(C.Structure):
@@ -169,6 +173,8 @@ def load_library(path: str = LIB_PATH):
     lib.qml_set_stack_slab.argtypes = [vp, C.c_int]
     lib.qml_get_batch.argtypes = [vp, C.c_int, C.POINTER(_BatchInfo)]
     lib.qml_set_np_buckets.argtypes = [vp, C.c_int]
+    lib.qml_chunk_matrix.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]
+    lib.qml_chunk_marginalize.argtypes = [vp, C.c_int, dp]
     lib.qml_bootstrap.argtypes = [vp, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_int), dp, dp, dp]
     lib.qml_poisson_draws.argtypes = [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_int)]
     ip = C.POINTER(C.c_int)
@@ -405,6 +411,22 @@ class QMLEngine:
         st = C.c_int(0)
         _check(self._lib.qml_get_chunk_result(self._ctx, cid, _dptr(F), _dptr(dbt), C.byref(st)), self._lib)
         return F, dbt, st.value
+
+    def chunk_matrix(self, cid: int, which: int, arg: int = 0) -> np.ndarray:
+        """One matrix of the reference Chunk's protected builders for chunk
+        cid (qml_chunk_matrix): which is one of the CMAT_* values, arg the
+        chunk-local i_kz (CMAT_Q) or the z bin (CMAT_ZWEIGHT, CMAT_GROWTH)."""
+        n = C.c_int(0)
+        _check(self._lib.qml_chunk_matrix(self._ctx, cid, int(which), int(arg), None, C.byref(n)), self._lib)
+        out = np.empty((n.value, n.value))
+        _check(self._lib.qml_chunk_matrix(self._ctx, cid, int(which), int(arg), _dptr(out), C.byref(n)), self._lib)
+        return out
+
+    def chunk_marginalize(self, cid: int, cinv: np.ndarray) -> np.ndarray:
+        """Chunk::_addMarginalizations on a copy of cinv (qml_chunk_marginalize)."""
+        out = np.ascontiguousarray(cinv, dtype=np.float64).copy()
+        _check(self._lib.qml_chunk_marginalize(self._ctx, cid, _dptr(out)), self._lib)
+        return out
 
     def set_pipeline(self, on: bool):
         """Stage overlap across two streams (default on); off gives per-kernel

@@ -16,11 +16,19 @@
 //   dropin_check read binary|picca <file> <out>
 //        one file through qio::QSOFile (readParameters, readData and, if the
 //        file has one, readAllocResolutionMatrix); fields and arrays in <out>
+//   dropin_check friend <config> <tables_dir> <tables_base> <out_dir>
+//        the first file through TestOneQSOEstimate, the friend class of the
+//        reference's tests/testSQCMatrices.cpp:24-95 (allocMemory,
+//        _initMatrices and _setVZMatrices in its constructor; the signal
+//        matrix into covariance_matrix; Q_0 weighted by its z bin), then the
+//        whole stored set and the marginalisation of S + I; each matrix as
+//        raw doubles in <out_dir>/<name>.bin
 //
 // <config>: whitespace-separated "key value..." lines: kedges <n> <values>,
 // zcenters <n> <values>, zwidth, method, logl_order, lam_order, max_forest,
 // nchunks, max_pixel, fast_bootstrap, nboots, weights <nqso*nboots ints>,
-// format binary|picca, rmat 0|1, deconv <m>, osamp <factor>,
+// format binary|picca, rmat 0|1, deconv <m>, osamp <factor>, growth <B>
+// (RedshiftGrowthOn with exponent B),
 // first_table <R> <dv> (the table every chunk uses with rmat 1),
 // save_chunks <base> (_saveChunkResults' file after the flush), save_boot
 // <base> (the realizations file of the bootstrap powers, with the Fisher sum
@@ -162,6 +170,9 @@ Config read_config(const char *path) {
             in >> c.b.resomat_deconvolution_m;
         } else if (key == "osamp") {
             in >> c.b.oversampling_factor;
+        } else if (key == "growth") {
+            c.b.redshift_growth_on = true;
+            in >> c.b.fiducial_redshift_power;
         } else if (key == "first_table") {
             in >> c.first_table.first >> c.first_table.second;
         } else if (key == "save_chunks") {
@@ -357,6 +368,128 @@ int read_one(const std::string &format, const char *fname, const char *out_path)
     return 0;
 }
 
+// --- glmemory (core/chunk_estimate.hpp:29-43)
+static_assert(std::is_same<decltype(&glmemory::setMaxSizes), void (*)(int, int, int, bool)>::value,
+              "void setMaxSizes(int, int, int, bool)");
+static_assert(std::is_same<decltype(&glmemory::allocMemory), void (*)()>::value, "void allocMemory()");
+static_assert(std::is_same<decltype(&glmemory::dealloc), void (*)()>::value, "void dealloc()");
+static_assert(std::is_same<decltype(&glmemory::updateMemUsed), void (*)(double)>::value, "void updateMemUsed(double)");
+static_assert(std::is_same<decltype(glmemory::max_size_2), int>::value, "int max_size_2");
+static_assert(std::is_same<decltype(glmemory::covariance_matrix), std::unique_ptr<double[]>>::value &&
+                  std::is_same<decltype(glmemory::temp_vector), std::unique_ptr<double[]>>::value &&
+                  std::is_same<decltype(glmemory::stored_sfid), std::unique_ptr<double[]>>::value,
+              "glmemory buffers");
+static_assert(std::is_same<decltype(glmemory::stored_ikz_qi), std::vector<std::unique_ptr<double[]>>>::value &&
+                  std::is_same<decltype(glmemory::temp_matrices), std::vector<std::unique_ptr<double[]>>>::value,
+              "glmemory matrix lists");
+
+// The friend class of tests/testSQCMatrices.cpp:24-36, with its constructor;
+// the checks write the matrices out for comparison.
+class TestOneQSOEstimate : public OneQSOEstimate {
+public:
+    TestOneQSOEstimate(const std::string &fname_qso) : OneQSOEstimate(fname_qso) {
+        glmemory::allocMemory();
+        chunks[0]->_initMatrices();
+        chunks[0]->_setVZMatrices();
+    }
+
+    // the protected surface of core/chunk_estimate.hpp:48-82, as the friend test
+    // and cross/exposure.hpp:19-109 use it
+    static void surface() {
+        static_assert(std::is_same<decltype(Chunk::DATA_SIZE_2), int>::value, "DATA_SIZE_2");
+        static_assert(std::is_same<decltype(Chunk::covariance_matrix), double *>::value &&
+                          std::is_same<decltype(Chunk::inverse_covariance_matrix), double *>::value &&
+                          std::is_same<decltype(Chunk::_vmatrix), double *>::value &&
+                          std::is_same<decltype(Chunk::_zmatrix), double *>::value &&
+                          std::is_same<decltype(Chunk::_matrix_lambda), double *>::value,
+                      "matrix pointers");
+        static_assert(std::is_same<decltype(Chunk::temp_matrix), std::vector<double *>>::value, "temp_matrix");
+        static_assert(std::is_same<decltype(Chunk::stored_ikz_qi), std::vector<std::pair<int, double *>>>::value,
+                      "stored_ikz_qi");
+        static_assert(std::is_same<decltype(&Chunk::_copyQSOFile), void (Chunk::*)(const qio::QSOFile &, int, int)>::value,
+                      "_copyQSOFile");
+        static_assert(std::is_same<decltype(&Chunk::_setNQandFisherIndex), void (Chunk::*)()>::value &&
+                          std::is_same<decltype(&Chunk::_initMatrices), void (Chunk::*)()>::value &&
+                          std::is_same<decltype(&Chunk::_setVZMatrices), void (Chunk::*)()>::value &&
+                          std::is_same<decltype(&Chunk::_applyRedshiftInterp), void (Chunk::*)()>::value &&
+                          std::is_same<decltype(&Chunk::_scaleDerivativesWithRedshiftGrowth), void (Chunk::*)()>::value &&
+                          std::is_same<decltype(&Chunk::_addMarginalizations), void (Chunk::*)()>::value,
+                      "builders without arguments");
+        static_assert(std::is_same<decltype(&Chunk::_setFiducialSignalMatrix), void (Chunk::*)(double *)>::value &&
+                          std::is_same<decltype(&Chunk::_setQiMatrix), void (Chunk::*)(double *, int)>::value,
+                      "_setFiducialSignalMatrix(double*), _setQiMatrix(double*, int)");
+    }
+
+    static bool dump(const std::string &path, const double *a, size_t n) {
+        FILE *f = std::fopen(path.c_str(), "wb");
+        if (!f) return false;
+        const bool ok = std::fwrite(a, sizeof(double), n, f) == n;
+        std::fclose(f);
+        return ok;
+    }
+
+    // tests/testSQCMatrices.cpp:38-65
+    bool test_setFiducialSignalMatrix(const std::string &dir) {
+        Chunk &c = *chunks[0];
+        c._setFiducialSignalMatrix(c.covariance_matrix);
+        return dump(dir + "/signal.bin", c.covariance_matrix, c.DATA_SIZE_2);
+    }
+
+    // tests/testSQCMatrices.cpp:67-95
+    bool test_setQiMatrix(const std::string &dir) {
+        Chunk &c = *chunks[0];
+        c._setQiMatrix(c.stored_ikz_qi[0].second, 0);
+        c._applyRedshiftInterp();
+        return dump(dir + "/q0.bin", c.stored_ikz_qi[0].second, c.DATA_SIZE_2);
+    }
+
+    bool vz(const std::string &dir) {
+        Chunk &c = *chunks[0];
+        const size_t n2 = (size_t)c._matrix_n * c._matrix_n;
+        return dump(dir + "/v.bin", c._vmatrix, n2) && dump(dir + "/z.bin", c._zmatrix, n2);
+    }
+
+    // every stored Q as oneQSOiteration builds them (core/chunk_estimate.cpp:860-882)
+    bool stored(const std::string &dir) {
+        Chunk &c = *chunks[0];
+        for (auto &q : c.stored_ikz_qi) c._setQiMatrix(q.second, q.first);
+        c._applyRedshiftInterp();
+        if (specifics::REDSHIFT_GROWTH_ON) c._scaleDerivativesWithRedshiftGrowth();
+        std::vector<double> all;
+        for (auto &q : c.stored_ikz_qi) all.insert(all.end(), q.second, q.second + c.DATA_SIZE_2);
+        return dump(dir + "/stored.bin", all.data(), all.size());
+    }
+
+    // _addMarginalizations on S + I (covariance_matrix holds S)
+    bool marginalize(const std::string &dir) {
+        Chunk &c = *chunks[0];
+        c._setFiducialSignalMatrix(c.covariance_matrix);
+        for (int i = 0; i < c.size(); ++i) c.covariance_matrix[(size_t)i * (c.size() + 1)] += 1.0;
+        if (!dump(dir + "/cinv_in.bin", c.covariance_matrix, c.DATA_SIZE_2)) return false;
+        c.inverse_covariance_matrix = c.covariance_matrix;
+        c._addMarginalizations();
+        return dump(dir + "/cinv_marg.bin", c.inverse_covariance_matrix, c.DATA_SIZE_2);
+    }
+};
+
+int friend_check(const Config &c, const char *dir, const char *base, const std::string &out_dir) {
+    TestOneQSOEstimate::surface();
+    lyspeq_amd::dropin::Options o;
+    o.fiducial_redshift_power = c.b.fiducial_redshift_power;
+    lyspeq_amd::dropin::setup(o, lyspeq_amd::dropin::sqFileTables(dir, base, c.first_table));
+    if (c.files.empty()) return 2;
+    try {
+        TestOneQSOEstimate t(c.files[0]);
+        const bool ok = t.vz(out_dir) && t.test_setFiducialSignalMatrix(out_dir) && t.test_setQiMatrix(out_dir) &&
+                        t.stored(out_dir) && t.marginalize(out_dir);
+        glmemory::dealloc();
+        return ok ? 0 : 3;
+    } catch (std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 4;
+    }
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: dropin_check cost|run <config> ... | read binary|picca <file> <out>\n");
@@ -370,6 +503,7 @@ int main(int argc, char **argv) {
     const std::string mode(argv[1]);
     if (mode == "cost") return cost(c);
     if (mode == "run" && argc == 7) return run(c, argv[3], argv[4], argv[5], argv[6]);
+    if (mode == "friend" && argc == 6) return friend_check(c, argv[3], argv[4], argv[5]);
     std::fprintf(stderr, "bad arguments\n");
     return 2;
 }

@@ -80,8 +80,9 @@ def fiducial_power(w):
 
 
 def run_engine(w, precision, steps, warmup, scratch_gb, graph=False, tune_caps=(), nr=0, nr_floor=0.0,
-               nr_smooth=False, boots=0, stack_slab=0, stages=None):
-    eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb, stack_slab=stack_slab)
+               nr_smooth=False, boots=0, stack_slab=0, stages=None, max_batch=0):
+    eng = QMLEngine(w.binning, precision=precision, scratch_gb=scratch_gb, stack_slab=stack_slab,
+                    max_batch_chunks=max_batch)
     t0 = time.perf_counter()
     accepted = synth.load_engine(eng, w)
     eng.commit()
@@ -384,6 +385,8 @@ def main():
                     help="operand staging of the split-BF16 left, right and Gram products (1, 2: through "
                          "registers with one or two LDS stages; 3, 4: direct-to-LDS loads with two or three "
                          "stages); default: the engine's")
+    ap.add_argument("--max-batch", type=int, default=0, metavar="N",
+                    help="cap on chunks per batch (0: the engine's default, 256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -409,7 +412,8 @@ def main():
     slab = args.stack_slab if args.stack_slab >= 0 else (16 if args.config == "hires" else 0)
     opt = dict(graph=args.launch == "graph", tune_caps=caps, nr=args.nr, nr_floor=args.nr_floor,
                nr_smooth=args.nr_smooth, boots=args.bootstrap, stack_slab=slab,
-               stages=tuple(int(x) for x in args.stages.split(",")) if args.stages else None)
+               stages=tuple(int(x) for x in args.stages.split(",")) if args.stages else None,
+               max_batch=args.max_batch)
     res64 = None
     if not args.no_fp64:
         res64 = run_engine(w, "fp64", args.steps, args.warmup, args.scratch_gb, **opt)

@@ -428,7 +428,7 @@ int qml_autotune(qml_ctx *ctx, const double *theta, const int *caps, int ncaps, 
  *      the right product;
  *   3  direct-to-LDS loads (global_load_lds), two stages (two per CU);
  *   4  direct-to-LDS loads, three stages (one per CU).
- * Defaults 1, 3 and 3.  Results are identical for every choice.  No
+ * Defaults 3, 3 and 3.  Results are identical for every choice.  No
  * counterpart in the reference. */
 int qml_set_kernel_stages(qml_ctx *ctx, int left, int right, int gram);
 

@@ -138,8 +138,9 @@ struct qml_ctx {
     int np_buckets = 0;   // 1: a batch also ends where the padded size changes
     int pipeline = 1;
     // staging of the split-BF16 products' operands (qml_set_kernel_stages):
-    // the defaults were the fastest on the 4096-chunk DESI batch
-    int nst_left = 1, nst_right = 3, nst_gram = 3;
+    // direct-to-LDS loads with two stages for all three products gave the
+    // fastest overlapped iteration on the 4096-chunk DESI batch
+    int nst_left = 3, nst_right = 3, nst_gram = 3;
     int slab = 0;                     // stack members per slab (0: the whole stack at once)
     int poison = 0;   // LYSPEQ_QML_POISON=1 at commit: NaN-filled scratch and LDS
     hipStream_t st = nullptr, st2 = nullptr;

@@ -170,9 +170,15 @@ bool read_hdus(const std::string &path, std::vector<Hdu> &out, std::string *err)
         }
         const int naxis = h.cards.count("NAXIS") ? std::stoi(h.cards["NAXIS"]) : 0;
         size_t nel = naxis > 0 ? 1 : 0;
+        const size_t left = (buf.size() - off) / 8;   // doubles left in the file
         for (int a = 1; a <= naxis; ++a) {
             const std::string k = "NAXIS" + std::to_string(a);
-            nel *= h.cards.count(k) ? (size_t)std::stoll(h.cards[k]) : 0;
+            const long long d = h.cards.count(k) ? std::stoll(h.cards[k]) : 0;
+            if (d < 0 || (d > 0 && nel > left / (size_t)d)) {
+                if (err) *err = "malformed or truncated FITS data";
+                return false;
+            }
+            nel *= (size_t)d;
         }
         const int bitpix = h.cards.count("BITPIX") ? std::stoi(h.cards["BITPIX"]) : -64;
         if (nel > 0) {
@@ -180,7 +186,7 @@ bool read_hdus(const std::string &path, std::vector<Hdu> &out, std::string *err)
                 if (err) *err = "only BITPIX = -64 images are read";
                 return false;
             }
-            if (off + nel * 8 > buf.size()) {
+            if (nel > left) {
                 if (err) *err = "truncated FITS data";
                 return false;
             }

@@ -92,13 +92,24 @@ size_t parse_header(const std::vector<unsigned char> &buf, size_t off, Header &h
     }
 }
 
-size_t data_bytes(const Header &h) {
-    const int naxis = h.has("NAXIS") ? (int)h.i("NAXIS") : 0;
-    size_t n = naxis > 0 ? 1 : 0;
-    for (int a = 1; a <= naxis; ++a) n *= (size_t)h.i("NAXIS" + std::to_string(a));
-    const int bitpix = h.has("BITPIX") ? std::abs((int)h.i("BITPIX")) : 8;
-    const size_t pcount = h.has("PCOUNT") ? (size_t)h.i("PCOUNT") : 0;
-    return n * (bitpix / 8) + pcount;
+// the HDU's data size in bytes; a size beyond `limit` (the bytes left in
+// the file), a negative axis or an overflow is a malformed file
+size_t data_bytes(const Header &h, size_t limit) {
+    auto bad = [] { return std::runtime_error("FITS ERROR malformed data size"); };
+    const long long naxis = h.has("NAXIS") ? h.i("NAXIS") : 0;
+    if (naxis < 0 || naxis > 999) throw bad();
+    const long long bitpix = h.has("BITPIX") ? std::llabs(h.i("BITPIX")) : 8;
+    const long long pcount = h.has("PCOUNT") ? h.i("PCOUNT") : 0;
+    if (bitpix % 8 != 0 || bitpix == 0 || bitpix > 64 || pcount < 0) throw bad();
+    size_t n = naxis > 0 ? (size_t)bitpix / 8 : 0;
+    for (long long a = 1; a <= naxis; ++a) {
+        const long long d = h.i("NAXIS" + std::to_string(a));
+        if (d < 0) throw bad();
+        if (d > 0 && n > limit / (size_t)d) throw bad();
+        n *= (size_t)d;
+    }
+    if ((size_t)pcount > limit - std::min(n, limit)) throw bad();
+    return n + (size_t)pcount;
 }
 
 struct Column {
@@ -159,13 +170,14 @@ void readPiccaFile(const std::string &fname_qso, QSOFile &q) {
             off = data;
             break;
         }
-        off = data + (data_bytes(hk) + BLOCK - 1) / BLOCK * BLOCK;
+        off = data + (data_bytes(hk, buf.size() - data) + BLOCK - 1) / BLOCK * BLOCK;
         if (off >= buf.size()) throw std::runtime_error("FITS ERROR HDU " + std::to_string(hdunum) + " not found");
     }
     if (!h.has("XTENSION") || h.at("XTENSION") != "BINTABLE") throw std::runtime_error("HDU type is not BINARY!");
-    const size_t rowb = (size_t)h.i("NAXIS1");
-    const long long N = h.i("NAXIS2");
-    const int nf = (int)h.i("TFIELDS");
+    const long long rowb_ll = h.i("NAXIS1"), N = h.i("NAXIS2"), nf_ll = h.i("TFIELDS");
+    if (rowb_ll <= 0 || N < 0 || nf_ll <= 0 || nf_ll > 999) throw std::runtime_error("FITS ERROR malformed table header");
+    const size_t rowb = (size_t)rowb_ll;
+    const int nf = (int)nf_ll;
     std::vector<Column> cols(nf);
     size_t pos = 0;
     for (int c = 0; c < nf; ++c) {
@@ -176,6 +188,7 @@ void readPiccaFile(const std::string &fname_qso, QSOFile &q) {
         size_t p = 0;
         while (p < form.size() && std::isdigit((unsigned char)form[p])) ++p;
         col.repeat = p ? std::stoll(form.substr(0, p)) : 1;
+        if (col.repeat < 0 || (size_t)col.repeat > rowb) throw std::runtime_error("FITS ERROR malformed TFORM" + k);
         col.type = (char)std::toupper((unsigned char)form[p]);
         col.offset = pos;
         pos += (size_t)col.repeat * type_bytes(col.type);
@@ -185,7 +198,7 @@ void readPiccaFile(const std::string &fname_qso, QSOFile &q) {
         }
     }
     if (pos != rowb) throw std::runtime_error("FITS ERROR row width does not match the columns");
-    if (off + rowb * (size_t)N > buf.size()) throw std::runtime_error("FITS ERROR truncated table");
+    if ((size_t)N > (buf.size() - off) / rowb) throw std::runtime_error("FITS ERROR truncated table");
     auto find = [&](const char *name) -> const Column * {
         for (const Column &c : cols)
             if (c.name == name) return &c;

@@ -346,6 +346,45 @@ def test_picca_reader_columns_keys_hdus_and_compression(tmp_path):
         assert bad is None and "not found" in err
 
 
+def _set_card(raw: bytes, key: str, value: str, hdu_start: int) -> bytes:
+    """raw with the first card `key` at or after hdu_start rewritten to hold value."""
+    k = raw.index(key.ljust(8).encode() + b"= ", hdu_start)
+    card = (key.ljust(8) + "= " + value.rjust(20)).ljust(80).encode()
+    return raw[:k] + card + raw[k + 80:]
+
+
+def test_picca_reader_refuses_malformed_files(tmp_path):
+    """A truncated table, a negative or huge row count, an oversized column
+    repeat and a negative axis in an HDU before the wanted one: each file is
+    refused with a FITS error, never read past its end."""
+    _need_exe()
+    rng = np.random.default_rng(9)
+    hdr = [("TARGETID", 1), ("Z", 2.5), ("RA", 0.0), ("DEC", 0.0), ("MEANRESO", 13.4), ("MEANSNR", 2.0)]
+    path = str(tmp_path / "ok.fits")
+    sqio.write_picca_deltas(path, [(hdr, _picca_spectrum(rng, 150, nd=5)), (hdr, _picca_spectrum(rng, 90))],
+                            compress=False)
+    raw = open(path, "rb").read()
+    good, err = _read(tmp_path, "picca", path + "[2]", "good.txt")
+    assert good is not None and good["wave"].size == 90, err
+    h1 = raw.index(b"XTENSION")            # the first spectrum's header
+    h2 = raw.index(b"XTENSION", h1 + 80)   # the second's
+    cases = {
+        "truncated": raw[:h1 + 2880 + 2880 // 2],
+        "negative_rows": _set_card(raw, "NAXIS2", "-5", h1),
+        "huge_rows": _set_card(raw, "NAXIS2", "4611686018427387904", h1),
+        "huge_repeat": _set_card(raw, "TFORM4", "'99999999999D'", h1),
+        "negative_axis_before": _set_card(raw, "NAXIS2", "-7", h1),
+    }
+    for name, data in cases.items():
+        p = str(tmp_path / f"{name}.fits")
+        with open(p, "wb") as f:
+            f.write(data)
+        which = "[2]" if name == "negative_axis_before" else "[1]"
+        rec, err = _read(tmp_path, "picca", p + which, f"{name}.txt")
+        assert rec is None and "FITS ERROR" in err, (name, err)
+    assert h2 > h1
+
+
 @pytest.mark.gpu
 def test_dropin_picca_files_with_resolution_matrices_against_oracle(tmp_path):
     """Spectra in a picca delta file with an 11-diagonal RESOMAT column, read

@@ -16,6 +16,12 @@
 //   dropin_check read binary|picca <file> <out>
 //        one file through qio::QSOFile (readParameters, readData and, if the
 //        file has one, readAllocResolutionMatrix); fields and arrays in <out>
+//   dropin_check exposure <config> <tables_dir> <tables_base> <out_dir>
+//        the first file's first chunk through a Chunk subclass built as the
+//        reference's cross/exposure.hpp:19-70 builds itself (Chunk(),
+//        _copyQSOFile, _setNQandFisherIndex, its own covariance buffer):
+//        the signal matrix and the marginalisation of S + I, as raw doubles
+//        in <out_dir>/ex_signal.bin, ex_cinv_in.bin, ex_cinv_marg.bin
 //   dropin_check friend <config> <tables_dir> <tables_base> <out_dir>
 //        the first file through TestOneQSOEstimate, the friend class of the
 //        reference's tests/testSQCMatrices.cpp:24-95 (allocMemory,
@@ -472,6 +478,62 @@ public:
     }
 };
 
+// A Chunk subclass that builds itself as cross/exposure.hpp:23-31,38-45 does:
+// the protected default constructor, _copyQSOFile and _setNQandFisherIndex,
+// then its own covariance buffer after _initMatrices.
+class TestExposure : public Chunk {
+    std::unique_ptr<double[]> local_cov_mat;
+
+public:
+    TestExposure(const qio::QSOFile &qmaster, int i1, int i2) : Chunk() {
+        _copyQSOFile(qmaster, i1, i2);
+        _setNQandFisherIndex();
+        glmemory::setMaxSizes(size(), size(), 2 * N_Q_MATRICES, false);
+        N_Q_MATRICES = 0;
+    }
+    void initMatrices() {
+        _initMatrices();
+        local_cov_mat = std::make_unique<double[]>(DATA_SIZE_2);
+        covariance_matrix = local_cov_mat.get();
+    }
+    bool check(const std::string &dir) {
+        _setVZMatrices();
+        _setFiducialSignalMatrix(covariance_matrix);
+        if (!TestOneQSOEstimate::dump(dir + "/ex_signal.bin", covariance_matrix, DATA_SIZE_2)) return false;
+        for (int i = 0; i < size(); ++i) covariance_matrix[(size_t)i * (size() + 1)] += 1.0;
+        if (!TestOneQSOEstimate::dump(dir + "/ex_cinv_in.bin", covariance_matrix, DATA_SIZE_2)) return false;
+        inverse_covariance_matrix = covariance_matrix;
+        _addMarginalizations();
+        return TestOneQSOEstimate::dump(dir + "/ex_cinv_marg.bin", inverse_covariance_matrix, DATA_SIZE_2);
+    }
+};
+
+// OneQSOEstimate's reading of a file (its protected _readQsoFile)
+struct FileReader : OneQSOEstimate {
+    std::unique_ptr<qio::QSOFile> read(const std::string &f) { return _readQsoFile(f); }
+};
+
+int exposure_check(const Config &c, const char *dir, const char *base, const std::string &out_dir) {
+    lyspeq_amd::dropin::Options o;
+    lyspeq_amd::dropin::setup(o, lyspeq_amd::dropin::sqFileTables(dir, base, c.first_table));
+    if (c.files.empty()) return 2;
+    try {
+        FileReader fr;
+        auto q = fr.read(c.files[0]);
+        if (!q) return 3;
+        const std::vector<int> idx = OneQSOEstimate::decideIndices(q->size(), q->wave());
+        TestExposure ex(*q, idx[0], idx[1]);
+        glmemory::allocMemory();
+        ex.initMatrices();
+        const bool ok = ex.check(out_dir);
+        glmemory::dealloc();
+        return ok ? 0 : 3;
+    } catch (std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 4;
+    }
+}
+
 int friend_check(const Config &c, const char *dir, const char *base, const std::string &out_dir) {
     TestOneQSOEstimate::surface();
     lyspeq_amd::dropin::Options o;
@@ -504,6 +566,7 @@ int main(int argc, char **argv) {
     if (mode == "cost") return cost(c);
     if (mode == "run" && argc == 7) return run(c, argv[3], argv[4], argv[5], argv[6]);
     if (mode == "friend" && argc == 6) return friend_check(c, argv[3], argv[4], argv[5]);
+    if (mode == "exposure" && argc == 6) return exposure_check(c, argv[3], argv[4], argv[5]);
     std::fprintf(stderr, "bad arguments\n");
     return 2;
 }

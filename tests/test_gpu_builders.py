@@ -218,7 +218,7 @@ def test_builder_matrix_arguments_are_checked():
         eng.close()
 
 
-def _run_friend(tmp_path, cfg_extra, files, tdir, tbase, b):
+def _run_friend(tmp_path, cfg_extra, files, tdir, tbase, b, mode="friend"):
     cfg = str(tmp_path / "friend.cfg")
     with open(cfg, "w") as f:
         ke, zc = b.kband_edges(), b.zbin_centers()
@@ -232,7 +232,7 @@ def _run_friend(tmp_path, cfg_extra, files, tdir, tbase, b):
             f.write(f"file {p}\n")
     out = tmp_path / "out"
     out.mkdir()
-    r = subprocess.run([DROPIN_CHECK, "friend", cfg, str(tdir), tbase, str(out)], capture_output=True, text=True,
+    r = subprocess.run([DROPIN_CHECK, mode, cfg, str(tdir), tbase, str(out)], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
 
@@ -277,6 +277,32 @@ def test_friend_test_against_reference_goldens(tmp_path):
         assert _err(stored[k], q) < TABLE_TOL, k
     cin = rd("cinv_in").reshape(N, N)
     assert _err(rd("cinv_marg").reshape(N, N), ch.add_marginalizations(cin)) < MARG_TOL
+
+
+def test_chunk_subclass_built_as_the_cross_exposure(tmp_path):
+    """A Chunk subclass that builds itself as cross/exposure.hpp:19-70 does
+    (the protected Chunk(), _copyQSOFile, _setNQandFisherIndex, its own
+    covariance buffer): its _initMatrices registers the chunk with the
+    engine, and its signal matrix and the marginalisation of S + I (one
+    log-lambda template) match the golden matrix and the oracle."""
+    if not os.path.exists(DROPIN_CHECK):
+        pytest.skip("tests/cpp/dropin_check not built (python -m lyspeq_amd.build)")
+    _case("Chunk subclass built as the cross exposure, golden chunk")
+    b = qml.Binning(**dict(gc.BINS_KW, cont_logl_order=1))
+    rd = _run_friend(tmp_path, "", [os.path.join(gc.GOLDEN, "desilite_seed332298_id0_0_z2.4_lognormal.dat")],
+                     gc.GOLDEN, "signal", b, mode="exposure")
+    N = 488
+    s_gold, _ = gc.chunk_matrices()
+    sig = rd("ex_signal").reshape(N, N)
+    assert gc.all_close(sig, s_gold)
+    from oracle.chunk import TableSet
+    t, sp = gc.table(), gc.spectrum()
+    tabs = [TableSet(70000, 20.0, t.nvpoints, t.v_length, t.nzpoints, t.deriv, t.signal.reshape(-1))]
+    ch = OracleChunk(P.oracle_bins(b), tabs, sp.wave, sp.flux, sp.ivar, sp.R_fwhm)
+    v, z = ch.vz_matrices()
+    assert _err(sig, ch.signal_matrix(v, z)) < TABLE_TOL
+    cin = rd("ex_cinv_in").reshape(N, N)
+    assert _err(rd("ex_cinv_marg").reshape(N, N), ch.add_marginalizations(cin)) < MARG_TOL
 
 
 def test_friend_test_resolution_matrix_oversampled_with_growth(tmp_path):

@@ -57,3 +57,29 @@ def test_switch_combinations_fp64_match_oracle(mode, method, sfid_off, orders):
         floor, xout, sx = None, None, None
     P.assert_chunks_fp64(out, oout, label, floor=floor, exact=xout)
     P.assert_fp64(F, dbt, oF, od, f"{label} sums", floor=floor, exact=sx)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("mode", ["table", "rmat-oversampled"])
+@pytest.mark.parametrize("method,sfid_off,orders", [(0, True, (0, 1)), (1, False, (0, 2))])
+def test_switch_combinations_mixed_precision_against_fp64(precision, mode, method, sfid_off, orders):
+    """The mixed paths on the same switch combinations, against FP64, to the
+    bars of the DESI and resolution-matrix workloads (parity.MIXED_TOL)."""
+    label = f"{mode}, {'top hat' if method == 0 else 'triangular'}, sfid {'off' if sfid_off else 'on'}, " \
+            f"orders {orders}, {precision}"
+    print(f"[case] switches mixed: {label}", flush=True)
+    if mode == "table":
+        w = synth.desi_config(nchunks=24)
+    else:
+        w = synth.rmat_config(nchunks=12, n_range=(100, 400), kind="oversampled")
+    w.binning.method = method
+    w.binning.turn_off_sfid = sfid_off
+    w.binning.cont_logl_order, w.binning.cont_lam_order = orders
+    th = _theta(w, sfid_off, 23)
+    idx = list(range(len(w.chunks)))
+    eng, out64, F64, d64 = P.run_engine(w, th, idx)
+    eng.close()
+    eng, outm, Fm, dm = P.run_engine(w, th, idx, precision=precision)
+    eng.close()
+    assert set(out64) == set(outm) and all(o[2] for o in outm.values())
+    P.assert_mixed(Fm, dm, F64, d64, precision, label, workload="desi" if mode == "table" else "rmat")

@@ -22,7 +22,10 @@
 // iteration keeps its matrices on the device.  The bookkeeping the
 // constructor does (_copyQSOFile, _findRedshiftBin, _setNQandFisherIndex,
 // _setStoredMatrices, stored_ikz_qi with its indices) is here too.
-#pragma once
+// an include guard, not #pragma once: the header may also be reached as a
+// copy under another path (a build that shadows the reference's header)
+#ifndef LYSPEQ_AMD_DROPIN_CHUNK_ESTIMATE_HPP
+#define LYSPEQ_AMD_DROPIN_CHUNK_ESTIMATE_HPP
 
 #include <memory>
 #include <utility>
@@ -146,3 +149,5 @@ private:
     int size_ = 0, real_size_ = 0;
     lyspeq_amd::Chunk &engine_chunk();
 };
+
+#endif  // LYSPEQ_AMD_DROPIN_CHUNK_ESTIMATE_HPP

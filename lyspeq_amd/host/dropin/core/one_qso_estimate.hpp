@@ -6,7 +6,10 @@
 // its deconvolution and oversampling, the redshift boundary cut and the
 // short-file test; then decideIndices and one Chunk per piece, skipping
 // pieces whose constructor throws.
-#pragma once
+// an include guard, not #pragma once: the header may also be reached as a
+// copy under another path (a build that shadows the reference's header)
+#ifndef LYSPEQ_AMD_DROPIN_ONE_QSO_ESTIMATE_HPP
+#define LYSPEQ_AMD_DROPIN_ONE_QSO_ESTIMATE_HPP
 
 #include <memory>
 #include <string>
@@ -44,3 +47,5 @@ public:
     void addBoot(int p, double *temppower, double *tempfisher);
     void addBootPowerOnly(int nboots, double *pcoeff, double *temppower);
 };
+
+#endif  // LYSPEQ_AMD_DROPIN_ONE_QSO_ESTIMATE_HPP

@@ -440,56 +440,72 @@ __global__ __launch_bounds__(256, (RBT * CBT <= 3) ? 3 : 1) void k_dpass(const C
 
 // d_j = y^T Q_j y; a_H = N' W; s_j = W^T a_j; and the projected template
 // weights At_j = a_j - 1/2 U s_j (m x MAX_NU x Np, pixel contiguous), with
-// which D (K Q_j K^T) D = D (Q_j - U At_j^T - At_j U^T) D.
+// which D (K Q_j K^T) D = D (Q_j - U At_j^T - At_j U^T) D.  One workgroup
+// per (member, chunk); the 1 + nu^2 sums of a member are formed in one pass
+// and reduced together (each by the summation order of block_sum).  The
+// member H (j = n) first forms its rows a_H.
 __global__ __launch_bounds__(256) void k_small(const ChunkDev *cs, const double *in,
                                                const double *vec, double *P, double *sj) {
-    const ChunkDev c = cs[blockIdx.x];
+    constexpr int NR = 1 + MAX_NU * MAX_NU;
+    const ChunkDev c = cs[blockIdx.y];
+    const int j = blockIdx.x;
+    if (j >= c.m) return;
     const int Np = c.Np, tid = threadIdx.x, nu = c.nu;
     const double *V = vec + c.vec_off;
     const double *y = V + V_Y * (size_t)Np;
     const double *Npr = in + c.in_off + IN_NPRIME * (size_t)Np;
-    __shared__ double red[4];
-    __shared__ double ssh[MAX_NU][MAX_NU];
+    __shared__ double red[4][NR];
+    __shared__ double tot[NR];
     double *s = sj + c.sj_off;
     double *d = s + (size_t)c.m * MAX_NU * MAX_NU;
-    for (int i = tid; i < Np; i += 256) {
-        double *dst = P + c.pj_off + ((size_t)c.n * Np + i) * (1 + MAX_NU);
-        dst[0] = 0.0;
-        for (int q = 0; q < nu; ++q) dst[1 + q] = (i < c.N) ? Npr[i] * V[(V_W0 + q) * (size_t)Np + i] : 0.0;
+    double *Pj = P + c.pj_off + (size_t)j * Np * (1 + MAX_NU);
+    if (j == c.n) {
+        for (int i = tid; i < Np; i += 256) {
+            double *dst = Pj + (size_t)i * (1 + MAX_NU);
+            dst[0] = 0.0;
+            for (int q = 0; q < nu; ++q) dst[1 + q] = (i < c.N) ? Npr[i] * V[(V_W0 + q) * (size_t)Np + i] : 0.0;
+        }
+        __syncthreads();
+    }
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    for (int i = tid; i < c.N; i += 256) {
+        const double *pr = Pj + (size_t)i * (1 + MAX_NU);
+        if (j < c.n) acc[0] += y[i] * pr[0];
+#pragma unroll
+        for (int t = 0; t < MAX_NU; ++t) {
+            if (t >= nu) break;
+            const double wt = V[(V_W0 + t) * (size_t)Np + i];
+#pragma unroll
+            for (int q = 0; q < MAX_NU; ++q)
+                if (q < nu) acc[1 + t * MAX_NU + q] += wt * pr[1 + q];
+        }
+    }
+    const int w = tid / 64, l = tid % 64;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (l == 0) red[w][k] = v;
     }
     __syncthreads();
-    for (int j = 0; j < c.m; ++j) {
-        const double *Pj = P + c.pj_off + (size_t)j * Np * (1 + MAX_NU);
-        if (j < c.n) {
-            double p = 0.0;
-            for (int i = tid; i < c.N; i += 256) p += y[i] * Pj[(size_t)i * (1 + MAX_NU)];
-            p = block_sum(p, red);
-            if (tid == 0) d[j] = p;
-        }
+    if (tid < NR) tot[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    __syncthreads();
+    if (tid == 0 && j < c.n) d[j] = tot[0];
+    if (tid < nu * nu) {
+        const int t = tid / nu, q = tid % nu;
+        s[(size_t)j * MAX_NU * MAX_NU + t * MAX_NU + q] = tot[1 + t * MAX_NU + q];
+    }
+    double *At = P + c.at_off + (size_t)j * MAX_NU * Np;
+    for (int i = tid; i < Np; i += 256) {
+        double u[MAX_NU];
+        for (int t = 0; t < nu; ++t) u[t] = in[c.in_off + (IN_U0 + t) * (size_t)Np + i];
         for (int t = 0; t < nu; ++t) {
-            const double *Wt = V + (V_W0 + t) * (size_t)Np;
-            for (int q = 0; q < nu; ++q) {
-                double p = 0.0;
-                for (int i = tid; i < c.N; i += 256) p += Wt[i] * Pj[(size_t)i * (1 + MAX_NU) + 1 + q];
-                p = block_sum(p, red);
-                if (tid == 0) {
-                    s[(size_t)j * MAX_NU * MAX_NU + t * MAX_NU + q] = p;
-                    ssh[t][q] = p;
-                }
-            }
+            double su = 0.0;
+            for (int q = 0; q < nu; ++q) su += tot[1 + t * MAX_NU + q] * u[q];
+            At[(size_t)t * Np + i] = (i < c.N) ? Pj[(size_t)i * (1 + MAX_NU) + 1 + t] - 0.5 * su : 0.0;
         }
-        __syncthreads();
-        double *At = P + c.at_off + (size_t)j * MAX_NU * Np;
-        for (int i = tid; i < Np; i += 256) {
-            double u[MAX_NU];
-            for (int t = 0; t < nu; ++t) u[t] = in[c.in_off + (IN_U0 + t) * (size_t)Np + i];
-            for (int t = 0; t < nu; ++t) {
-                double su = 0.0;
-                for (int q = 0; q < nu; ++q) su += ssh[t][q] * u[q];
-                At[(size_t)t * Np + i] = (i < c.N) ? Pj[(size_t)i * (1 + MAX_NU) + 1 + t] - 0.5 * su : 0.0;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -1033,7 +1049,10 @@ void launch_dpass(const LaunchCtx &L) {
         default: launch_dpass_rb<DQ_RB>(L, grid); break;
     }
 }
-void launch_small(const LaunchCtx &L) { k_small<<<L.nchunks, 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj); }
+void launch_small(const LaunchCtx &L) {
+    // members per chunk: the stored Q (at most N_Q_MATRICES), H and T
+    k_small<<<dim3(L.nq_max + 2, L.nchunks), 256, 0, L.stream>>>(L.chunks, L.in, L.vec, L.pj, L.sj);
+}
 template <class OutT, class CT>
 static void launch_xgen_t(const LaunchCtx &L, OutT *X) {
     const int tr = (sizeof(CT) == 4) ? 32 : 16;

@@ -20,17 +20,19 @@ namespace qml {
 // Theta-contracted derivative tables, per chunk and z bin of its stored Q:
 // Th_m[iv] = sum_kn theta[(zm0+m) nk + kn] T[iv][kn], so that
 // sum_k theta_k Q_k(i,j) = sum_m w_m(z_ij) ((1-f) Th_m[iv] + f Th_m[iv+1]).
+// The table is read band-major (qt), so that consecutive lanes read
+// consecutive rows.
 __global__ __launch_bounds__(256) void k_theta_tables(const ChunkDev *cs, const TableDev *tabs, BinsDev bins,
                                                       const double *theta, double *aux) {
     const ChunkDev c = cs[blockIdx.y];
     const int iv = blockIdx.x * 256 + threadIdx.x;
     if (iv >= c.nvu) return;
     const TableDev t = tabs[c.table];
-    const double *row = t.q + (size_t)iv * bins.nk;
+    const double *col = t.qt + iv;
     for (int m = 0; m < c.nzq; ++m) {
         const double *th = theta + (size_t)(c.zm0 + m) * bins.nk;
         double s = 0.0;
-        for (int kn = 0; kn < c.kbu; ++kn) s += th[kn] * row[kn];
+        for (int kn = 0; kn < c.kbu; ++kn) s += th[kn] * col[(size_t)kn * t.nv];
         aux[c.th_off + (size_t)m * c.nvu + iv] = s;
     }
 }

@@ -39,7 +39,7 @@ struct TableHost {
     double dv;
     int nv, nz;
     double v_length;
-    double *d_q = nullptr, *d_qt = nullptr, *d_q2 = nullptr, *d_s = nullptr;
+    double *d_qt = nullptr, *d_q2 = nullptr, *d_s = nullptr;
 };
 
 struct ChunkHost {
@@ -246,7 +246,7 @@ int qml_destroy(qml_ctx *c) {
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
     if (c->st2) hipStreamSynchronize(c->st2);
-    for (auto &t : c->tables) { hipFree(t.d_q); hipFree(t.d_qt); hipFree(t.d_q2); hipFree(t.d_s); }
+    for (auto &t : c->tables) { hipFree(t.d_qt); hipFree(t.d_q2); hipFree(t.d_s); }
     release_plan(c);
     void *ptrs[] = {c->d_tables, c->d_in, c->d_outs, c->d_fc, c->d_dbt, c->d_status, c->d_skip,
                     c->d_theta, c->d_F, c->d_dbtsum, c->d_rin};
@@ -319,11 +319,9 @@ int qml_add_table(qml_ctx *c, int R_fwhm, double dv_kms, int nv, double v_length
         for (int j = 0; j < nv; ++j) q[(size_t)j * nk + kn] = deriv[(size_t)kn * nv + j];
     std::vector<double> s((size_t)nv * t.nz, 0.0);
     if (signal && !c->bins.turn_off_sfid) std::copy(signal, signal + (size_t)nv * nz, s.begin());
-    HIPCHK(hipMalloc(&t.d_q, q.size() * sizeof(double)));
     HIPCHK(hipMalloc(&t.d_qt, q.size() * sizeof(double)));
     HIPCHK(hipMemcpy(t.d_qt, deriv, q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&t.d_s, s.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(t.d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
     // interpolation pairs (q[j][kn], q[j+1][kn]); the last row repeats its node
     std::vector<double> q2(2 * q.size());
     for (int j = 0; j < nv; ++j)
@@ -852,7 +850,6 @@ int qml_commit(qml_ctx *c) {
     std::vector<TableDev> td(c->tables.size());
     for (size_t r = 0; r < c->tables.size(); ++r) {
         const auto &t = c->tables[r];
-        td[r].q = t.d_q;
         td[r].qt = t.d_qt;
         td[r].q2 = t.d_q2;
         td[r].s = t.d_s;

@@ -36,8 +36,7 @@ struct BinsDev {
 };
 
 struct TableDev {
-    const double *q;   // nv x nk, k contiguous (interleaved for row gathers)
-    const double *qt;  // nk x nv, v contiguous (for per-band slices)
+    const double *qt;  // nk x nv, v contiguous (per-band slices, and rows read band by band)
     const double *q2;  // nv x nk pairs (q[v][k], q[v+1][k]): both nodes of a gather in one load
     const double *s;   // nv x nz, z contiguous
     int nv, nz;

@@ -38,7 +38,8 @@ __global__ __launch_bounds__(256) void k_theta_tables(const ChunkDev *cs, const 
 }
 
 // A = I + D C D and S, C = S_fid + sum_k theta_k Q_k, on upper 32x32 tiles,
-// mirrored (core/chunk_estimate.cpp:384-536,668-698).
+// mirrored (core/chunk_estimate.cpp:384-536,668-698).  The mirror image of
+// a tile leaves through LDS, so that both copies are written row by row.
 __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const double *in,
                                                   const TableDev *tabs, BinsDev bins,
                                                   const double *aux, const double *theta, double *mat) {
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const doub
     const int n32 = c.Np / 32;
     const int I = blockIdx.x / n32, J = blockIdx.x % n32;
     if (blockIdx.x >= n32 * n32 || I > J) return;
+    __shared__ double ta[32][33], ts[32][33];
     const TableDev t = tabs[c.table];
     const double *wave = in + c.in_off + IN_WAVE * (size_t)c.Np;
     const double *D = in + c.in_off + IN_D * (size_t)c.Np;
@@ -93,9 +95,19 @@ __global__ __launch_bounds__(256) void k_assemble(const ChunkDev *cs, const doub
             a = D[i] * cij * D[j] + ((i == j) ? 1.0 : 0.0);
         }
         A[(size_t)i * c.Np + j] = a;
-        A[(size_t)j * c.Np + i] = a;
         S[(size_t)i * c.Np + j] = s;
-        S[(size_t)j * c.Np + i] = s;
+        ta[e / 32][e % 32] = a;
+        ts[e / 32][e % 32] = s;
+    }
+    __syncthreads();
+    // the mirror: row J*32 + r, column I*32 + q holds tile element (q, r);
+    // on a diagonal tile only below the diagonal
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int r = e / 32, q = e % 32;
+        if (I == J && q >= r) continue;
+        const size_t o = (size_t)(J * 32 + r) * c.Np + I * 32 + q;
+        A[o] = ta[q][r];
+        S[o] = ts[q][r];
     }
 }
 

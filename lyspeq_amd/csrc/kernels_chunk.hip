@@ -604,6 +604,15 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
     __shared__ CT slice[2][SL];
     __shared__ CT atv[2][DP_ZB * MAX_NU * (TC + TR)];   // [m][x][64 columns | 32 rows]
     __shared__ int red_lo[4], red_hi[4];
+    // where the thread's four values of row e land in a member's plane:
+    // row-major, or the split products' blocked layout (8 bytes inside one
+    // 16-byte chunk, as bb is a multiple of 4)
+    size_t xoff[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int i = i0 + rl + 16 * e;
+        xoff[e] = UPPER ? (size_t)bf3_tiled(i, bb, Np) : (size_t)i * Np + bb;
+    }
 
     // pair geometry
     int iv[EPT][4];
@@ -794,7 +803,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
                     // outside the chunk q, U and D are zero: an exact zero
                     val[u] = sc[e][u] * acc;
                 }
-                XStore4<OutT>::put(Xc, msz, j - jlo, (size_t)(i0 + rl + 16 * e) * Np + bb, val);
+                XStore4<OutT>::put(Xc, msz, j - jlo, xoff[e], val);
             }
         }
         __syncthreads();
@@ -837,7 +846,7 @@ __global__ __launch_bounds__(256, OCC) void k_xgen(const ChunkDev *cs, const dou
                 for (int x = 0; x < NUM; ++x) acc -= ur[e][x] * ac[u][x] + ar[x] * ucl[u][x];
                 val[u] = sc[e][u] * acc;
             }
-            XStore4<OutT>::put(Xc, msz, j - jlo, (size_t)i * Np + bb, val);
+            XStore4<OutT>::put(Xc, msz, j - jlo, xoff[e], val);
         }
     }
 }
@@ -848,15 +857,27 @@ __global__ void k_convert_li(const ChunkDev *cs, const double *mat, OutT *lic) {
     const ChunkDev c = cs[blockIdx.y];
     const size_t n = (size_t)c.Np * c.Np;
     const double *Li = mat + c.mat_off + 2 * n;
-    for (size_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
-        if constexpr (sizeof(OutT) == 2) {
-            const double v = Li[e];
-            const __bf16 h = (__bf16)(float)v;
-            lic[c.lic_off + e] = h;
-            lic[c.lic_off + n + e] = (__bf16)(float)(v - (double)h);
-        } else {
-            lic[c.lic_off + e] = (OutT)Li[e];
+    if constexpr (sizeof(OutT) == 2) {
+        // eight consecutive columns per thread, into one 16-byte chunk of
+        // each plane of the blocked layout
+        typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+        const int Np = c.Np, cpr = Np / 8;
+        for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n / 8; q += (size_t)gridDim.x * 256) {
+            const int r = (int)(q / cpr), k = 8 * (int)(q % cpr);
+            bf8v h, l;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double v = Li[(size_t)r * Np + k + u];
+                h[u] = (__bf16)(float)v;
+                l[u] = (__bf16)(float)(v - (double)h[u]);
+            }
+            const size_t o = c.lic_off + bf3_tiled(r, k, Np);
+            *(bf8v *)(lic + o) = h;
+            *(bf8v *)(lic + n + o) = l;
         }
+    } else {
+        for (size_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
+            lic[c.lic_off + e] = (OutT)Li[e];
     }
 }
 

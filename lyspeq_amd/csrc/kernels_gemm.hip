@@ -283,9 +283,12 @@ void launch_gemm_gram(hipStream_t st, int prec, int nst, const ChunkDev *chunks,
 namespace qml {
 
 // ------------------------------------------------- split-BF16x3 GEMMs --
-// Operands are stored pre-split: element (r, k) of an operand is hi[r*ld + k]
-// + hi[lo_off + r*ld + k] (two bf16 planes), so the main loop moves bytes
-// and issues MFMAs only.  Every product is A B^T with both operands K-major
+// Operands are stored pre-split: element (r, k) of an operand is hi[o] +
+// hi[lo_off + o] (two bf16 planes), so the main loop moves bytes and issues
+// MFMAs only.  The whitening products' operands (Li, the stack, P) are in
+// the blocked layout of bf3_tiled, o = bf3_tiled(r, k, Np), so that a
+// staged 16 x 32 block is one contiguous kilobyte; the Gram's (the G store,
+// one row per member) are row-major, o = r ld + k.  Every product is A B^T with both operands K-major
 // (X_j is symmetric; Li^T and the Gram are NT by construction).
 //
 // A workgroup computes a 128x128 tile with one wave per 64x64 quarter
@@ -314,7 +317,7 @@ constexpr int BK = 32;
 
 // the swizzle repeats every 16 rows, so fragment block x sits at a fixed
 // offset x * 16 * BK from block 0
-__device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ (((r >> 1) ^ ((r >> 3) & 1)) & 3)) << 3); }
+__device__ __forceinline__ int bf3_slot(int r, int c) { return r * BK + ((c ^ bf3_swz(r & 15)) << 3); }
 
 // 128x128 output tiles of four waves (2 x 2), each a 64x64 quarter
 constexpr int BF3_BM = 128, BF3_BN = 128;
@@ -434,7 +437,8 @@ __device__ __forceinline__ void bf3_epilogue(f4 (&acc)[4][4], const ChunkDev &c,
     __bf16 *C = (__bf16 *)Cbase;
     long long base, ldc, lo;
     if (MODE == MODE_LEFT) {
-        base = c.stk_off + 2 * (wk.j - jlo) * msz + (long long)r0 * Np + c0;
+        // P in the blocked layout: the quarter's eight 1 KB blocks
+        base = c.stk_off + 2 * (wk.j - jlo) * msz;
         ldc = Np;
         lo = msz;
     } else {
@@ -460,12 +464,24 @@ __device__ __forceinline__ void bf3_epilogue(f4 (&acc)[4][4], const ChunkDev &c,
                 *(bf4 *)(ep + (16 * x + orow) * BF3_EP + 16 * y + ocol) = h;
             }
         __bf16 *Cp = C + base + plane * lo;
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        if (MODE == MODE_LEFT) {
+            // store s8 writes block (r0/16 + s8/2, c0/32 + s8%2) whole: lane i
+            // fills slot i, row i/4 of the block, chunk (i%4) ^ bf3_swz(i/4)
+            const int rr = lane / 4, ch = (lane % 4) ^ bf3_swz(rr);
 #pragma unroll
-        for (int s8 = 0; s8 < 8; ++s8) {
-            const int r = 8 * s8 + lane / 8, ch = lane % 8;
-            typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-            __builtin_nontemporal_store(*(const u4 *)(ep + r * BF3_EP + 8 * ch),
-                                        (u4 *)(Cp + (long long)r * ldc + 8 * ch));
+            for (int s8 = 0; s8 < 8; ++s8) {
+                const int rb = s8 / 2, kb = s8 % 2;
+                __builtin_nontemporal_store(*(const u4 *)(ep + (16 * rb + rr) * BF3_EP + 32 * kb + 8 * ch),
+                                            (u4 *)(Cp + bf3_tiled(r0 + 16 * rb, c0 + 32 * kb, Np) + 8 * lane));
+            }
+        } else {
+#pragma unroll
+            for (int s8 = 0; s8 < 8; ++s8) {
+                const int r = 8 * s8 + lane / 8, ch = lane % 8;
+                __builtin_nontemporal_store(*(const u4 *)(ep + r * BF3_EP + 8 * ch),
+                                            (u4 *)(Cp + (long long)r * ldc + 8 * ch));
+            }
         }
     }
 }
@@ -514,9 +530,16 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     // workgroup-uniform.  The whitening products' offsets stay below Np^2
     // (32 bits); the Gram's rows are whole members of 2 ntu TILE^2 elements,
     // so m ntu >= 2^18 (e.g. N ~ 2900 with 250 stored Q) needs 64 bits.
+    // In the blocked layout chunk lc of row r at K step 0 is bf3_tiled(r,
+    // 8 lc, Np), and K step k0 (a multiple of 32) adds 16 k0.
     using off_t = typename std::conditional<MODE == MODE_GRAM, long long, int>::type;
-    const off_t oa0 = (off_t)arow(0) * (off_t)lda + 8 * lc, oa1 = (off_t)arow(PA - 1) * (off_t)lda + 8 * lc;
-    const off_t ob0 = (off_t)brow(0) * (off_t)ldb + 8 * lc, ob1 = (off_t)brow(PB - 1) * (off_t)ldb + 8 * lc;
+    const auto roff = [&](int r, long long ld) -> off_t {
+        if constexpr (MODE == MODE_GRAM) return (off_t)r * (off_t)ld + 8 * lc;
+        else return (off_t)bf3_tiled(r, 8 * lc, (int)ld);
+    };
+    const off_t oa0 = roff(arow(0), lda), oa1 = roff(arow(PA - 1), lda);
+    const off_t ob0 = roff(brow(0), ldb), ob1 = roff(brow(PB - 1), ldb);
+    constexpr int KSC = (MODE == MODE_GRAM) ? 1 : 16;   // element offset per unit of k
     const int wa0 = bf3_slot(lr, lc), wa1 = bf3_slot(lr + RP * (PA - 1), lc);
     const int wb0 = bf3_slot(lr, lc), wb1 = bf3_slot(lr + RP * (PB - 1), lc);
     // two staging register sets (P, Q): stage k+2 is in flight while k+1 waits
@@ -527,17 +550,17 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
     const int rr = lane % 16, cq = lane / 16;
     const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
 #define QML_BF3_GLOADX(R, XA, XB, k0)                                            \
-    R##rah0 = *(const uint4 *)((XA) + oa0 + (k0));                               \
-    R##ral0 = *(const uint4 *)((XA) + alo + oa0 + (k0));                         \
-    R##rbh0 = *(const uint4 *)((XB) + ob0 + (k0));                               \
-    R##rbl0 = *(const uint4 *)((XB) + blo + ob0 + (k0));                         \
+    R##rah0 = *(const uint4 *)((XA) + oa0 + KSC * (k0));                         \
+    R##ral0 = *(const uint4 *)((XA) + alo + oa0 + KSC * (k0));                   \
+    R##rbh0 = *(const uint4 *)((XB) + ob0 + KSC * (k0));                         \
+    R##rbl0 = *(const uint4 *)((XB) + blo + ob0 + KSC * (k0));                   \
     if constexpr (PA > 1) {                                                      \
-        R##rah1 = *(const uint4 *)((XA) + oa1 + (k0));                           \
-        R##ral1 = *(const uint4 *)((XA) + alo + oa1 + (k0));                     \
+        R##rah1 = *(const uint4 *)((XA) + oa1 + KSC * (k0));                     \
+        R##ral1 = *(const uint4 *)((XA) + alo + oa1 + KSC * (k0));               \
     }                                                                            \
     if constexpr (PB > 1) {                                                      \
-        R##rbh1 = *(const uint4 *)((XB) + ob1 + (k0));                           \
-        R##rbl1 = *(const uint4 *)((XB) + blo + ob1 + (k0));                     \
+        R##rbh1 = *(const uint4 *)((XB) + ob1 + KSC * (k0));                     \
+        R##rbl1 = *(const uint4 *)((XB) + blo + ob1 + KSC * (k0));               \
     }
 #define QML_BF3_GLOAD(R, k0) QML_BF3_GLOADX(R, Ahi, Bhi, k0)
 #define QML_BF3_SSTORE(R, st)                                                    \
@@ -666,9 +689,11 @@ void k_gemm_bf3(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *A
 // memory straight into LDS at (M0 base) + 16 x lane, with no register
 // round trip, so NS stages can be in flight at once without registers held
 // for them.  Wave w fills plane w of a stage (A hi, A lo, B hi, B lo), eight
-// 16-row blocks of 1 KB; lane i of a block loads row i/4 of the block at the
-// chunk that the swizzle places in slot i, (i%4) ^ (((r >> 1) ^ ((r >> 3) &
-// 1)) & 3) with r = i/4.  Rows past the operand read row 0, as in k_gemm_bf3.
+// 16-row blocks of 1 KB.  For the whitening products a block is stored in
+// slot order (bf3_tiled), so lane i loads the block's 16 bytes at 16 i; for
+// the Gram lane i loads row i/4 of the block at the chunk that the swizzle
+// places in slot i, (i%4) ^ bf3_swz(i/4).  Rows past the operand read row
+// 0, as in k_gemm_bf3.
 //
 // Step s waits for its own stage (s_waitcnt vmcnt counting the stages issued
 // after it), passes one barrier, issues stage s + NS - 1 into the buffer
@@ -721,13 +746,18 @@ void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *
     const long long pld = pa ? it.lda : it.ldb, plo = (w & 1) ? (pa ? it.alo : it.blo) : 0;
     using off_t = typename std::conditional<MODE == MODE_GRAM, long long, int>::type;
     const int rl = lane / 4;
-    const int ch = (lane % 4) ^ (((rl >> 1) ^ ((rl >> 3) & 1)) & 3);
+    const int ch = (lane % 4) ^ bf3_swz(rl);
+    // blocked layout: the 16 rows of a block lie all inside the operand or
+    // all past it (Np is a multiple of 64), and lane i reads slot i of the
+    // block; K step k (a multiple of 32) adds 16 k
     off_t roff[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
         const int r = prow0 + 16 * b + rl;
-        roff[b] = (off_t)(r < prows ? r : 0) * (off_t)pld + (off_t)plo + 8 * ch;
+        if constexpr (MODE == MODE_GRAM) roff[b] = (off_t)(r < prows ? r : 0) * (off_t)pld + (off_t)plo + 8 * ch;
+        else roff[b] = (off_t)bf3_tiled(r < prows ? prow0 + 16 * b : 0, 0, (int)pld) + (off_t)plo + 8 * lane;
     }
+    constexpr int KSC = (MODE == MODE_GRAM) ? 1 : 16;
     const int kbeg = it.kbeg;
     const int ns = (it.kend - kbeg) / BK;
     const int total = (MODE == MODE_RIGHT) ? 2 * ns : ns;
@@ -739,7 +769,7 @@ void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *
         __bf16 *dst = smem + (s % NS) * BF3_STG + w * BF3_PL;
 #pragma unroll
         for (int b = 0; b < 8; ++b)
-            __builtin_amdgcn_global_load_lds((const void *)(x + roff[b] + k),
+            __builtin_amdgcn_global_load_lds((const void *)(x + roff[b] + KSC * k),
                                              (__attribute__((address_space(3))) void *)(dst + b * 16 * BK), 16, 0, 0);
     };
     const int rr = lane % 16, cq = lane / 16;

@@ -25,6 +25,18 @@ enum VecSlot { V_X = 0, V_Y = 1, V_W0 = 2, V_R0 = 2 + MAX_NU, V_T0 = 3 + 2 * MAX
                V_E = 5 + 4 * MAX_NU, V_F = 6 + 4 * MAX_NU, V_G = 7 + 4 * MAX_NU };
 static_assert(V_G < VEC_SLOTS, "vector slots");
 
+// Split-BF16 operand layout of the whitening products (Li, the stack, P):
+// each Np x Np plane is cut into 1 KB blocks of 16 rows x 32 columns, block
+// (r/16, k/32) at element ((r/16) (Np/32) + k/32) 512, and inside a block
+// the 16-byte chunk c (columns 8c..8c+7) of row r sits at slot
+// 4 (r%16) + (c ^ bf3_swz(r%16)).  That is the order in which the products
+// stage a block in LDS, so one contiguous 1 KB load fills one LDS block.
+__device__ __forceinline__ int bf3_swz(int r) { return ((r >> 1) ^ ((r >> 3) & 1)) & 3; }
+__device__ __forceinline__ int bf3_tiled(int r, int k, int Np) {
+    const int rr = r & 15;
+    return (((r >> 4) * (Np >> 5) + (k >> 5)) << 9) + ((4 * rr + (((k >> 3) & 3) ^ bf3_swz(rr))) << 3) + (k & 7);
+}
+
 struct BinsDev {
     double zc[MAX_ZB];
     double ke[MAX_NK + 1];           // k-band edges (resolution-matrix mode)

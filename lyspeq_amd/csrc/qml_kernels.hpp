@@ -25,9 +25,9 @@ struct LaunchCtx {
     int np_max;               // largest Np in the batch
     int nvu_max;              // largest theta-table length in the batch
     int nu_max;               // most continuum templates of a chunk in the batch
-    int dp_rb, dp_cb;
+    int dp_rb, dp_cb;         // d-pass operand blocks: most (KBIN_UPP + 1)/16 and (z bins + 1)(1 + nu)/16, rounded up
     int jlo = 0, jhi = 1 << 30;   // members [jlo, jhi) of the stack slab (all members: 0, large)
-    int nq_max;               // most N_Q_MATRICES of a chunk in the batch         // d-pass operand blocks: most (KBIN_UPP + 1)/16 and (z bins + 1)(1 + nu)/16, rounded up
+    int nq_max;               // most N_Q_MATRICES of a chunk in the batch
     const double *in;         // per-chunk input vectors
     const TableDev *tables;   // device table descriptors
     BinsDev bins;

@@ -28,6 +28,8 @@
 #include "qml_common.hpp"
 #include "qml_kernels.hpp"
 
+#include <climits>
+
 namespace qml {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -774,6 +776,13 @@ void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *
     };
     const int rr = lane % 16, cq = lane / 16;
     const int aoff = bf3_slot(64 * wm + rr, cq), boff = bf3_slot(64 * wn + rr, cq);
+    // K steps whose products are exact zeros for this wave are not
+    // multiplied: Li[i][k] = 0 for k > i, the stored T[n][k] = 0 for k < n,
+    // and P is lower triangular.  LEFT (rows i of Li, rows n of T) needs k in
+    // [c0, r0 + 64); RIGHT (P or Li rows i in both terms) k < r0 + 64.  The
+    // accumulators start at +0 and no sum of exact zeros changes them, so the
+    // results are those of the full sweep, bit for bit.
+    const int klo = (MODE == MODE_LEFT) ? c0 : INT_MIN, khi = (MODE == MODE_GRAM) ? INT_MAX : r0 + 64;
     if (total > 0) {
 #pragma unroll
         for (int s = 0; s < NS - 1; ++s)
@@ -781,7 +790,8 @@ void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *
         for (int s = 0; s < total; ++s) {
             bf3_wait_stage<NS>(s + NS - 2 >= total);
             if (s + NS - 1 < total) issue(s + NS - 1);
-            if (active) bf3_mfma<NPROD>(smem + (s % NS) * BF3_STG, aoff, boff, acc);
+            const int kk = kbeg + ((MODE == MODE_RIGHT && s >= ns) ? s - ns : s) * BK;
+            if (active && kk + BK > klo && kk < khi) bf3_mfma<NPROD>(smem + (s % NS) * BF3_STG, aoff, boff, acc);
         }
         // every wave is done with the stages before the epilogue reuses LDS
         __builtin_amdgcn_s_waitcnt(7 << 4);

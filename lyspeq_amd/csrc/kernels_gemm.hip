@@ -779,10 +779,15 @@ void k_gemm_bf3d(const ChunkDev *cs, const Work *work, int nwork, const __bf16 *
     // K steps whose products are exact zeros for this wave are not
     // multiplied: Li[i][k] = 0 for k > i, the stored T[n][k] = 0 for k < n,
     // and P is lower triangular.  LEFT (rows i of Li, rows n of T) needs k in
-    // [c0, r0 + 64); RIGHT (P or Li rows i in both terms) k < r0 + 64.  The
+    // [c0, r0 + 64); RIGHT (P or Li rows i in both terms) k < r0 + 64.  Past
+    // the chunk's N pixels (D = 0) T, P and G vanish, so k < N and a wave
+    // whose rows or columns all lie past N multiplies nothing.  The
     // accumulators start at +0 and no sum of exact zeros changes them, so the
     // results are those of the full sweep, bit for bit.
-    const int klo = (MODE == MODE_LEFT) ? c0 : INT_MIN, khi = (MODE == MODE_GRAM) ? INT_MAX : r0 + 64;
+    const int kn32 = (c.N + BK - 1) / BK * BK;
+    const bool pad = MODE != MODE_GRAM && (r0 >= c.N || c0 >= c.N);
+    const int klo = (MODE == MODE_LEFT) ? c0 : INT_MIN;
+    const int khi = (MODE == MODE_GRAM) ? INT_MAX : pad ? INT_MIN : min(r0 + 64, kn32);
     if (total > 0) {
 #pragma unroll
         for (int s = 0; s < NS - 1; ++s)

@@ -374,6 +374,12 @@ __device__ __forceinline__ Bf3Item bf3_item(const ChunkDev &c, const Work &wk, i
         t.kend = t.kbeg + TILE * TILE;
         t.Mv = t.Nv = c.m;
     }
+    if (MODE != MODE_GRAM) {
+        // past the chunk's N pixels every product is an exact zero (D = 0
+        // there), so the sweep ends at N rounded up to a K step
+        const int kn32 = (c.N + BK - 1) / BK * BK;
+        t.kend = max(t.kbeg, min(t.kend, kn32));
+    }
     return t;
 }
 
